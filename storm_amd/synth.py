@@ -1,0 +1,226 @@
+"""Deterministic synthetic inputs for the mapping hot path (genomes, reads, filter candidates).
+
+Everything is driven by numpy's PCG64 generator with an explicit seed, so the same call gives the
+same bytes on every machine.  Codes: A=0 C=1 G=2 T=3 (reference codes.h:16-19); a color is the
+XOR of two adjacent bases (codes.h:85-88).
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+BASES = np.frombuffer(b"ACGT", dtype=np.uint8)
+COLORS = np.frombuffer(b"0123", dtype=np.uint8)
+
+
+def rng(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.PCG64(seed))
+
+
+def random_genome(n: int, seed: int) -> np.ndarray:
+    """i.i.d. uniform ACGT codes."""
+    return rng(seed).integers(0, 4, size=n, dtype=np.uint8)
+
+
+def gc_window_genome(n: int, seed: int, window: int = 10_000, gc_lo: float = 0.40, gc_hi: float = 0.60) -> np.ndarray:
+    """GC fraction drawn uniformly in [gc_lo, gc_hi] per window of `window` bases."""
+    g = rng(seed)
+    nw = (n + window - 1) // window
+    gc = g.uniform(gc_lo, gc_hi, size=nw)
+    p_gc = np.repeat(gc, window)[:n]
+    u = g.random(n)
+    is_gc = u < p_gc
+    low = g.integers(0, 2, size=n, dtype=np.uint8)          # choose within class
+    # A/T = 0/3, C/G = 1/2
+    codes = np.where(is_gc, 1 + low, np.where(low == 0, 0, 3)).astype(np.uint8)
+    return codes
+
+
+def repeat_genome(n: int, seed: int, seg_dup_frac: float = 0.10, tandem_frac: float = 0.05) -> np.ndarray:
+    """i.i.d. background with segmental duplications (copies of 1-20 kb, 1-5 % divergence)
+    and tandem repeats (unit 2-50 bp) covering the requested fractions."""
+    g = rng(seed)
+    codes = g.integers(0, 4, size=n, dtype=np.uint8)
+    target = int(seg_dup_frac * n)
+    done = 0
+    while done < target:
+        ln = int(g.integers(1000, 20001))
+        if ln >= n // 2:
+            break
+        src = int(g.integers(0, n - ln))
+        dst = int(g.integers(0, n - ln))
+        seg = codes[src:src + ln].copy()
+        div = g.uniform(0.01, 0.05)
+        mut = g.random(ln) < div
+        seg[mut] = (seg[mut] + g.integers(1, 4, size=int(mut.sum()), dtype=np.uint8)) & 3
+        codes[dst:dst + ln] = seg
+        done += ln
+    target = int(tandem_frac * n)
+    done = 0
+    while done < target:
+        unit = int(g.integers(2, 51))
+        copies = int(g.integers(5, 60))
+        ln = unit * copies
+        dst = int(g.integers(0, n - ln))
+        u = g.integers(0, 4, size=unit, dtype=np.uint8)
+        codes[dst:dst + ln] = np.tile(u, copies)
+        done += ln
+    return codes
+
+
+def write_fasta(path: str, name: str, codes: np.ndarray, width: int = 70) -> None:
+    letters = BASES[codes]
+    with open(path, "wb") as f:
+        f.write(b">" + name.encode() + b"\n")
+        for s in range(0, len(letters), width):
+            f.write(letters[s:s + width].tobytes() + b"\n")
+
+
+@dataclass
+class SimReads:
+    seqs: list          # list of np.uint8 code arrays (bases, or colors for color reads)
+    quals: list         # list of np.int16 phred arrays
+    origin: np.ndarray  # true origin (0-based genome position of the first base)
+    strand: np.ndarray  # 0 forward, 1 reverse
+    first_base: np.ndarray | None = None   # color reads: actual first base
+
+
+def _mutate(seq: np.ndarray, g: np.random.Generator, sub: float, indel: float, L: int, src: np.ndarray, pos: int):
+    """Apply substitutions and 1-3 bp indels to a genome slice, return exactly L codes."""
+    out = []
+    i = pos
+    n_src = len(src)
+    while len(out) < L and i < n_src:
+        u = g.random()
+        if u < indel / 2:          # insertion in the read
+            k = int(g.integers(1, 4))
+            out.extend(int(x) for x in g.integers(0, 4, size=k))
+            continue
+        if u < indel:              # deletion from the read
+            i += int(g.integers(1, 4))
+            continue
+        c = int(src[i])
+        if g.random() < sub:
+            c = (c + int(g.integers(1, 4))) & 3
+        out.append(c)
+        i += 1
+    while len(out) < L:
+        out.append(int(g.integers(0, 4)))
+    return np.array(out[:L], dtype=np.uint8)
+
+
+def simulate_nucleotide_reads(genome: np.ndarray, n: int, L, seed: int, sub: float = 0.02, indel: float = 0.001,
+                              qlo: int = 20, qhi: int = 40) -> SimReads:
+    """Reads of length L (int, or (lo, hi) for uniform lengths) from uniform positions and strands."""
+    g = rng(seed)
+    G = len(genome)
+    lens = np.full(n, L, dtype=np.int64) if np.isscalar(L) else g.integers(L[0], L[1] + 1, size=n)
+    span = int(lens.max()) + 16
+    origin = g.integers(0, G - span, size=n)
+    strand = g.integers(0, 2, size=n)
+    seqs, quals = [], []
+    for k in range(n):
+        Lk = int(lens[k])
+        o = int(origin[k])
+        sl = genome[o:o + Lk + 16]
+        if indel > 0 and g.random() < Lk * indel * 1.5:
+            r = _mutate(sl, g, sub, indel * 4, Lk, sl, 0)
+        else:
+            r = sl[:Lk].copy()
+            m = g.random(Lk) < sub
+            if m.any():
+                r[m] = (r[m] + g.integers(1, 4, size=int(m.sum()), dtype=np.uint8)) & 3
+        if strand[k]:
+            r = (3 - r[::-1]).astype(np.uint8)
+        seqs.append(r)
+        quals.append(g.integers(qlo, qhi + 1, size=Lk).astype(np.int16))
+    return SimReads(seqs, quals, origin, strand)
+
+
+def write_fastq(path: str, reads: SimReads, qual_base: int = 33, prefix: str = "r") -> None:
+    with open(path, "wb") as f:
+        for k, (s, q) in enumerate(zip(reads.seqs, reads.quals)):
+            f.write(b"@%s%d\n" % (prefix.encode(), k))
+            f.write(BASES[s].tobytes() + b"\n+\n")
+            f.write((q.astype(np.int64) + qual_base).astype(np.uint8).tobytes() + b"\n")
+
+
+def simulate_color_reads(genome: np.ndarray, n: int, L, seed: int, snp: float = 0.01, color_err: float = 0.03,
+                         qlo: int = 10, qhi: int = 35) -> SimReads:
+    """SOLiD-like reads: L colors (int or (lo, hi)) covering L+1 bases; SNPs in base space,
+    then color errors.  `seqs` holds the L colors after the first one, `first_base` the real
+    first base (what a csfasta line 'T' + c0 + colors decodes to)."""
+    g = rng(seed)
+    G = len(genome)
+    lens = np.full(n, L, dtype=np.int64) if np.isscalar(L) else g.integers(L[0], L[1] + 1, size=n)
+    span = int(lens.max()) + 4
+    origin = g.integers(0, G - span, size=n)
+    strand = g.integers(0, 2, size=n)
+    seqs, quals, firsts = [], [], []
+    for k in range(n):
+        Lk = int(lens[k])
+        o = int(origin[k])
+        b = genome[o:o + Lk + 1].copy()
+        m = g.random(Lk + 1) < snp
+        if m.any():
+            b[m] = (b[m] + g.integers(1, 4, size=int(m.sum()), dtype=np.uint8)) & 3
+        if strand[k]:
+            b = (3 - b[::-1]).astype(np.uint8)
+        col = (b[1:] ^ b[:-1]).astype(np.uint8)
+        e = g.random(Lk) < color_err
+        if e.any():
+            col[e] = (col[e] ^ g.integers(1, 4, size=int(e.sum()), dtype=np.uint8)) & 3
+        seqs.append(col)
+        firsts.append(int(b[0]))
+        quals.append(g.integers(qlo, qhi + 1, size=Lk + 1).astype(np.int16))   # primer color + L colors
+    return SimReads(seqs, quals, origin, strand, np.array(firsts, dtype=np.uint8))
+
+
+def write_csfasta(path: str, qual_path: str | None, reads: SimReads, prefix: str = "r") -> None:
+    """'T' + first color (T xor first base) + L colors; the .qual holds L+1 integers."""
+    T = 3
+    with open(path, "wb") as f:
+        for k, s in enumerate(reads.seqs):
+            c0 = T ^ int(reads.first_base[k])
+            f.write(b">%s%d\n" % (prefix.encode(), k))
+            f.write(b"T" + bytes([48 + c0]) + COLORS[s].tobytes() + b"\n")
+    if qual_path:
+        with open(qual_path, "wb") as f:
+            for k, q in enumerate(reads.quals):
+                f.write(b">%s%d\n" % (prefix.encode(), k))
+                f.write(" ".join(str(int(x)) for x in q).encode() + b"\n")
+
+
+def pack_codes(codes: np.ndarray, nbytes: int | None = None) -> np.ndarray:
+    """2-bit packing, code n at bits 2*(n&3) of byte n>>2 (codes.h:125-130)."""
+    n = len(codes)
+    nb = (n + 3) // 4 if nbytes is None else nbytes
+    out = np.zeros(nb, dtype=np.uint8)
+    c = np.zeros(((n + 3) // 4) * 4, dtype=np.uint8)
+    c[:n] = codes
+    c = c.reshape(-1, 4)
+    packed = c[:, 0] | (c[:, 1] << 2) | (c[:, 2] << 4) | (c[:, 3] << 6)
+    out[:len(packed)] = packed[:nb]
+    return out
+
+
+def filter_candidates(genome: np.ndarray, n_reads: int, L: int, W: int, d: int, seed: int, sub: float = 0.02):
+    """Filter workload: each read gets one true-origin window (origin - d) and one uniform random
+    decoy window.  Returns (reads codes [n, L], window starts [n, 2])."""
+    g = rng(seed)
+    G = len(genome)
+    origin = g.integers(d + 1, G - (L + 2 * W + 2), size=n_reads)
+    idx = origin[:, None] + np.arange(L)[None, :]
+    reads = genome[idx]
+    m = g.random(reads.shape) < sub
+    reads = np.where(m, (reads + g.integers(1, 4, size=reads.shape, dtype=np.uint8)) & 3, reads).astype(np.uint8)
+    decoy = g.integers(0, G - (L + 2 * W + 2), size=n_reads)
+    pos = np.stack([origin - d, decoy], axis=1).astype(np.int32)
+    return reads, pos
+
+
+def ensure_dir(p: str) -> str:
+    os.makedirs(p, exist_ok=True)
+    return p
