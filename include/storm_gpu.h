@@ -1,0 +1,167 @@
+/*
+ * storm_gpu.h -- C ABI of the MI355X (gfx950) implementation of the SToRM mapping hot path.
+ *
+ * Library: storm_amd/_build/libstorm_hip.so.  Every function returns an int status
+ * (STORM_OK = 0, negative on error) and never exits the process; storm_last_error() gives the
+ * message of the calling thread's last failure.  Host buffers belong to the caller; device
+ * buffers belong to the context or plan that created them.  A context is used by one host
+ * thread at a time.
+ *
+ * Two groups of entry points:
+ *   1. drop-in rows for the reference's call sites (same names/signatures pattern as the
+ *      reference, gfx950 in place of the ISA), one GPU launch per call;
+ *   2. batch entry points that replace the reference's per-read loop (process_read and the
+ *      passes around it, run.c:555-1017) for many reads at once.
+ */
+#ifndef STORM_GPU_H
+#define STORM_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STORM_OK 0
+#define STORM_ERR_INVALID (-1)     /* bad argument (mirrors RETURN_INPUT_ERR, load_data.h:8) */
+#define STORM_ERR_MEMORY (-2)      /* allocation failed (mirrors RETURN_MEM_ERR, load_data.h:9) */
+#define STORM_ERR_DEVICE (-3)      /* HIP runtime error */
+#define STORM_ERR_IO (-4)          /* file could not be read or written */
+
+const char *storm_last_error(void);
+
+/* ------------------------------------------------------------------------------------------
+ * Mapping parameters: the reference's command-line options (main.c:188-433) that the -M
+ * mapping path reads.
+ */
+typedef struct storm_params {
+  int color;              /* 0: nucleotide (storm-nucleotide), 1: color space (storm-color) */
+  int match;              /* -m, default +5 (alignment.h:24) */
+  int mismatch;           /* -x, default -4 */
+  int gap_open;           /* -d, default -9 */
+  int gap_extend;         /* -e, default -4 */
+  int allowed_indels;     /* -i, default 3; the filter uses min(i, 15) (main.c:302-313) */
+  int t;                  /* -t, alignment threshold, default 115 */
+  int z;                  /* -z, filter threshold, default 100 */
+  int qual_base;          /* -b, default 33 */
+  int stdev_distance;     /* -l, repeat filter, default 100; <= 0 disables it */
+  int sense_mask;         /* 1 forward (-F), 2 reverse (-R), 3 both (default) */
+  int max_hits;           /* -M, 1..4 */
+  int uniqueness;         /* -u threshold, 0 = off */
+  int print_all;          /* -A */
+  int ignore_lowercase;   /* -j */
+  int strict_length;      /* 1: reject mixed read lengths like the reference (reads_data.c:156-163) */
+  const char *seeds;      /* -s, ';'-separated spaced seeds; NULL = reference defaults (seed.h:29-32) */
+} storm_params;
+
+void storm_params_default(storm_params *p, int color);
+
+/* ------------------------------------------------------------------------------------------
+ * Group 1: gfx950 row of the reference's SIMD filter table.
+ * Replaces alignment_<isa>__init_<w> / __align_<w> / __clean / __compatible_proc
+ * (alignment_simd.h:32-300), reached through simd_init_fct_table / simd_fct_table /
+ * simd_clean_fct (run.c:15-221).  W = 2/4/8/16 lanes for octa/quad/pair/mono, filtering
+ * simd_mul = 8/4/2/1 windows per call exactly as the SSE2 row does (run.c:109).
+ * `genome` must point inside a buffer registered with alignment_gfx950__set_genome (the
+ * reference passes ref_seq, run.c:660), `read` is a packed read (codes.h:125-130).
+ */
+int alignment_gfx950__set_genome(const unsigned char *sequence_alloc, size_t alloc_bytes, size_t code0_offset_bytes);
+void alignment_gfx950__init_octa(unsigned match, unsigned mismatch, unsigned gapopen, unsigned gapextends,
+                                 unsigned threshold, unsigned readlength);
+void alignment_gfx950__init_quad(unsigned match, unsigned mismatch, unsigned gapopen, unsigned gapextends,
+                                 unsigned threshold, unsigned readlength);
+void alignment_gfx950__init_pair(unsigned match, unsigned mismatch, unsigned gapopen, unsigned gapextends,
+                                 unsigned threshold, unsigned readlength);
+void alignment_gfx950__init_mono(unsigned match, unsigned mismatch, unsigned gapopen, unsigned gapextends,
+                                 unsigned threshold, unsigned readlength);
+unsigned alignment_gfx950__align_octa(unsigned char *genome, int *pos_genome, unsigned char *read);
+unsigned alignment_gfx950__align_quad(unsigned char *genome, int *pos_genome, unsigned char *read);
+unsigned alignment_gfx950__align_pair(unsigned char *genome, int *pos_genome, unsigned char *read);
+unsigned alignment_gfx950__align_mono(unsigned char *genome, int *pos_genome, unsigned char *read);
+void alignment_gfx950__clean(void);
+int alignment_gfx950__compatible_proc(void);
+
+/* ------------------------------------------------------------------------------------------
+ * Group 2: batch filter (stage c) over many candidates, with inputs resident in HBM.
+ * Replaces the simd_fct_table calls of process_read (run.c:660, :714) for a whole batch.
+ *   genome_packed : packed codes of one sequence (code n at bits 2*(n&3) of byte n>>2), size codes
+ *   reads_packed  : n_reads packed reads, read_stride bytes apart; read_len[n_reads]
+ *   cand_read/cand_pos : per candidate, read index and window start (hit - min(i,15), run.c:651)
+ *   W             : 2, 4, 8 or 16 (run.c:109-127 for -i)
+ * storm_filter_run launches the kernel `reps` times on the resident data and reports the mean
+ * kernel time in ms (HIP events on the plan's stream); storm_filter_fetch copies the u8 scores
+ * (0..255, the reference's running maximum) and pass flags (score > min(z,254)) back.
+ */
+typedef struct storm_filter_plan storm_filter_plan;
+int storm_filter_plan_create(int device, const uint8_t *genome_packed, int64_t genome_codes,
+                             const uint8_t *reads_packed, int64_t read_stride, const int32_t *read_len, int64_t n_reads,
+                             const int32_t *cand_read, const int32_t *cand_pos, int64_t n_cand,
+                             int W, int match, int mismatch, int gap_open, int gap_extend, int z,
+                             storm_filter_plan **out);
+int storm_filter_run(storm_filter_plan *plan, int reps, double *ms_per_launch);
+int storm_filter_fetch(storm_filter_plan *plan, uint8_t *scores, uint8_t *pass);
+void storm_filter_plan_destroy(storm_filter_plan *plan);
+
+/* ------------------------------------------------------------------------------------------
+ * Group 2: whole mapping runs (reads_against_references, run.c:764-1109, with -M).
+ * storm_map_files reads the same files as the reference (FASTA genome; FASTQ, FASTA or
+ * csfasta(+.qual) reads), maps on the GPU and writes the reference's -M SAM.
+ * mode: STORM_MODE_PLAIN runs the stages as separate launches per batch of equal-length reads;
+ *       STORM_MODE_BATCHED runs every read of the batch, whatever its length, in one pass of
+ *       persistent fused kernels.
+ * argv/argc are echoed in the @PG line as the reference does (hit_map.c:310-315).
+ */
+#define STORM_MODE_PLAIN 0
+#define STORM_MODE_BATCHED 1
+
+typedef struct storm_map_stats {
+  int64_t n_reads, n_mapped;
+  int64_t n_candidates, n_filter_pass, n_accepted;
+  double seconds_load, seconds_index, seconds_map, seconds_sam;
+  double gpu_ms_seed, gpu_ms_filter, gpu_ms_align, gpu_ms_replay;
+} storm_map_stats;
+
+int storm_map_files(int device, const char *genome_path, const char *reads_path, const char *qual_path,
+                    const storm_params *params, int mode, const char *sam_path, int argc, char **argv,
+                    storm_map_stats *stats);
+
+/* In-memory variant: reads already packed (codes.h layout), one length per read.
+ *   names        : n_reads NUL-terminated names (SAM QNAME)
+ *   codes        : n_reads * code_stride bytes; qual: n_reads * qual_stride bytes of 2-bit
+ *                  quality levels (read_quality.h:52-58) or NULL
+ *   first_base / first_qual : color mode only (reads_data.h:22-27), else NULL
+ * The reads are taken in the order given (the caller sorts them by tag if it wants the
+ * reference's order, reads_data.c:59-67).  The SAM text is written to sam_path.
+ */
+typedef struct storm_reads_view {
+  int64_t n_reads;
+  const char *const *names;
+  const uint8_t *codes;
+  int64_t code_stride;
+  const uint8_t *qual;
+  int64_t qual_stride;
+  const int32_t *len;
+  const uint8_t *first_base;
+  const uint8_t *first_qual;
+  const char *db_name;   /* @RG ID/SM; NULL -> "*" */
+} storm_reads_view;
+
+typedef struct storm_mapper storm_mapper;
+/* Loads a FASTA genome, builds the seed indexes (host) and keeps genome + indexes resident in
+ * HBM (index__build_reference, index.c:19-195; load_reference_db, reference_data.c:195-252). */
+int storm_mapper_create(int device, const char *genome_path, const storm_params *params, storm_mapper **out);
+int storm_mapper_map(storm_mapper *m, const storm_reads_view *reads, int mode, const char *sam_path,
+                     int argc, char **argv, storm_map_stats *stats);
+/* Per-read results without SAM text: for read r and rank k < 4, score/ref/start/sense
+ * (HitInfoType, hit_map.h:17-40); unmapped slots have ref_id = -1. */
+int storm_mapper_map_hits(storm_mapper *m, const storm_reads_view *reads, int mode,
+                          int32_t *score, int32_t *ref_id, int32_t *ref_start, int32_t *sense,
+                          storm_map_stats *stats);
+int64_t storm_mapper_free_bytes(storm_mapper *m);
+void storm_mapper_destroy(storm_mapper *m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STORM_GPU_H */

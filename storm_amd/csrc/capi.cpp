@@ -1,0 +1,999 @@
+// capi.cpp -- C ABI of the gfx950 mapping path (include/storm_gpu.h): device buffers, the
+// per-batch pipeline (seed -> lookup -> candidates -> filter -> DP -> hit map) and the drop-in
+// rows of the reference's filter table.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "host_io.h"
+#include "storm_kernels.h"
+
+namespace storm {
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPOK(expr)                                                                         \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess) return fail(STORM_ERR_DEVICE, "%s: %s (%s:%d)", #expr,            \
+                                      hipGetErrorString(_e), __FILE__, __LINE__);           \
+  } while (0)
+
+// device allocation owned by a C++ object
+struct DevMem {
+  void* p = nullptr;
+  size_t n = 0;
+  DevMem() = default;
+  DevMem(const DevMem&) = delete;
+  DevMem& operator=(const DevMem&) = delete;
+  DevMem(DevMem&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevMem& operator=(DevMem&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevMem() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= n && p) return hipSuccess;
+    release();
+    size_t b = bytes ? bytes : 16;
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) n = b;
+    return e;
+  }
+  template <class T> T* as() const { return (T*)p; }
+};
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// ------------------------------------------------------------------------------ genome planes
+struct DevGenome {
+  DevMem lo, hi, mask;
+  int64_t size = 0;
+  GenomeDev view() const { return GenomeDev{lo.as<uint32_t>(), hi.as<uint32_t>(), mask.as<uint32_t>(), size}; }
+};
+
+static int64_t plane_words(int64_t size) { return (GPAD + size + GPAD) / 32 + 4; }
+
+static int upload_genome(const uint8_t* code, const uint8_t* mask, int64_t size, DevGenome& g, hipStream_t st) {
+  const int64_t nw = plane_words(size);
+  std::vector<uint32_t> lo((size_t)nw, 0), hi((size_t)nw, 0), mk((size_t)nw, 0xffffffffu);
+  for (int64_t i = 0; i < size; ++i) {
+    const int64_t b = i + GPAD;
+    const uint32_t bit = 1u << (b & 31);
+    if (code[i] & 1) lo[(size_t)(b >> 5)] |= bit;
+    if (code[i] & 2) hi[(size_t)(b >> 5)] |= bit;
+    if (!(mask && mask[i])) mk[(size_t)(b >> 5)] &= ~bit;
+  }
+  HIPOK(g.lo.ensure(nw * 4));
+  HIPOK(g.hi.ensure(nw * 4));
+  HIPOK(g.mask.ensure(nw * 4));
+  HIPOK(hipMemcpyAsync(g.lo.p, lo.data(), nw * 4, hipMemcpyHostToDevice, st));
+  HIPOK(hipMemcpyAsync(g.hi.p, hi.data(), nw * 4, hipMemcpyHostToDevice, st));
+  HIPOK(hipMemcpyAsync(g.mask.p, mk.data(), nw * 4, hipMemcpyHostToDevice, st));
+  HIPOK(hipStreamSynchronize(st));
+  g.size = size;
+  return STORM_OK;
+}
+
+static int filter_lanes(int d) { return d <= 1 ? 2 : d <= 3 ? 4 : d <= 7 ? 8 : 16; }   // run.c:109-127
+
+static void fill_score(const storm_params& p, ScoreDev& sc) {
+  sc.fmatch = std::abs(p.match) & 0xff;
+  sc.fmismatch = std::abs(p.mismatch) & 0xff;
+  sc.fopen = std::abs(p.gap_open) & 0xff;
+  sc.fext = std::abs(p.gap_extend) & 0xff;
+  unsigned u = (unsigned)p.z;
+  if (u >= 255) u = 254;
+  sc.fthr = (int)u;
+  for (int q = 0; q < 4; ++q) {   // SCALE_SCORE (alignment.h:30), alignment.c:276-279
+    const short m = (short)p.match, x = (short)p.mismatch;
+    sc.ps[0][q] = (int)((m * (q + 1) + 0.) / 4);
+    sc.ps[1][q] = (int)((x * (q + 1) + 0.) / 4);
+  }
+  sc.gap_open = (short)p.gap_open;
+  sc.gap_ext = (short)p.gap_extend;
+  sc.color = p.color;
+  sc.I = p.allowed_indels;
+  sc.d = p.allowed_indels >= 16 ? 15 : p.allowed_indels;
+  sc.W = filter_lanes(sc.d);
+  sc.t = p.t;
+}
+
+// ------------------------------------------------------------------------------ mapper
+struct DevIndex {
+  DevMem first, map;
+  SeedSpec spec;
+};
+
+struct RefState {
+  DevGenome g;
+  std::vector<DevIndex> idx;
+  bool resident = false;
+};
+
+struct PassScratch {   // reused across passes
+  DevMem seg_count, seg_off, hits, kept, overflow, n_overflow, seg_begin, seg_end, sorted;
+  DevMem cand_off, cand_rs, cand_diag, cand_pos, pass, pass_scan;
+  DevMem rec_rs, rec_diag, rec_off, recs, tb, align_scratch, cub_tmp;
+  DevMem mark_rec, mark_slot, mark_count, gath;
+};
+
+}  // namespace storm
+
+struct storm_mapper {
+  int device = 0;
+  hipStream_t st = nullptr;
+  storm_params p{};
+  std::string seeds_str;
+  std::vector<storm::HostRef> refs;
+  std::vector<storm::SeedSpec> seeds;
+  std::vector<std::unique_ptr<storm::RefState>> rstate;
+  storm::PassScratch ps;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double index_seconds = 0;
+};
+
+namespace storm {
+
+static int ensure_ref_resident(storm_mapper* m, size_t r) {
+  RefState& rs = *m->rstate[r];
+  if (rs.resident) return STORM_OK;
+  const HostRef& ref = m->refs[r];
+  int rc = upload_genome(ref.code.data(), ref.mask.data(), ref.size, rs.g, m->st);
+  if (rc) return rc;
+  const double t0 = now_s();
+  rs.idx.clear();
+  rs.idx.resize(m->seeds.size());
+  std::vector<HostIndex> hidx(m->seeds.size());
+  std::vector<std::thread> th;
+  for (size_t s = 0; s < m->seeds.size(); ++s)
+    th.emplace_back([&, s]() { build_index(ref, m->seeds[s], m->p, hidx[s]); });
+  for (auto& t : th) t.join();
+  for (size_t s = 0; s < m->seeds.size(); ++s) {
+    rs.idx[s].spec = m->seeds[s];
+    HIPOK(rs.idx[s].first.ensure(hidx[s].first.size() * 4));
+    HIPOK(rs.idx[s].map.ensure(hidx[s].map.size() * 4 + 16));
+    HIPOK(hipMemcpyAsync(rs.idx[s].first.p, hidx[s].first.data(), hidx[s].first.size() * 4, hipMemcpyHostToDevice, m->st));
+    HIPOK(hipMemcpyAsync(rs.idx[s].map.p, hidx[s].map.data(), hidx[s].map.size() * 4, hipMemcpyHostToDevice, m->st));
+  }
+  HIPOK(hipStreamSynchronize(m->st));
+  m->index_seconds += now_s() - t0;
+  rs.resident = true;
+  return STORM_OK;
+}
+
+static void release_ref(storm_mapper* m, size_t r) {
+  RefState& rs = *m->rstate[r];
+  rs.g.lo.release(); rs.g.hi.release(); rs.g.mask.release();
+  rs.idx.clear();
+  rs.resident = false;
+}
+
+static SeedSetDev seed_set(const RefState& rs) {
+  SeedSetDev s{};
+  s.n = (int)rs.idx.size();
+  for (int k = 0; k < s.n; ++k) {
+    const SeedSpec& sp = rs.idx[k].spec;
+    s.s[k].weight = sp.weight;
+    s.s[k].span = sp.span;
+    s.s[k].n_pos = (int)sp.pos.size();
+    for (int j = 0; j < sp.weight; ++j) s.s[k].offs[j] = sp.offs[j];
+    for (size_t j = 0; j < sp.pos.size(); ++j) s.s[k].pos[j] = sp.pos[j];
+    s.s[k].first = rs.idx[k].first.as<int32_t>();
+    s.s[k].map = rs.idx[k].map.as<int32_t>();
+  }
+  return s;
+}
+
+// reads resident on the device for one map call (global read ids = device order)
+struct DevReads {
+  DevMem codes, qual, first_qual, len, planes, qplanes;
+  int64_t n = 0;
+  int has_qual = 0;
+  ReadsDev view() const {
+    return ReadsDev{planes.as<uint32_t>(), qplanes.as<uint32_t>(), len.as<int32_t>(), n, has_qual};
+  }
+};
+
+static int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order, DevReads& d) {
+  const int64_t n = v->n_reads;
+  d.n = n;
+  d.has_qual = v->qual != nullptr;
+  const int64_t cs = v->code_stride, qs = v->qual ? v->qual_stride : 0;
+  std::vector<uint8_t> codes((size_t)(n * cs));
+  std::vector<uint8_t> qual(qs ? (size_t)(n * qs) : 0);
+  std::vector<int32_t> len((size_t)n);
+  std::vector<uint8_t> fq(v->first_qual ? (size_t)n : 0);
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t j = order[(size_t)i];
+    memcpy(&codes[(size_t)(i * cs)], v->codes + j * cs, (size_t)cs);
+    if (qs) memcpy(&qual[(size_t)(i * qs)], v->qual + j * qs, (size_t)qs);
+    len[(size_t)i] = v->len[j];
+    if (!fq.empty()) fq[(size_t)i] = v->first_qual[j];
+    if (len[(size_t)i] <= 0 || len[(size_t)i] > MAXL)
+      return fail(STORM_ERR_INVALID, "read %lld has length %d (1..%d supported)", (long long)j, len[(size_t)i], MAXL);
+    const int w = filter_lanes(m->p.allowed_indels >= 16 ? 15 : m->p.allowed_indels);
+    if (len[(size_t)i] < w)
+      return fail(STORM_ERR_INVALID, "read %lld is shorter (%d) than the filter band lanes (%d)", (long long)j,
+                  len[(size_t)i], w);
+  }
+  HIPOK(d.codes.ensure(codes.size()));
+  HIPOK(d.len.ensure(len.size() * 4));
+  HIPOK(hipMemcpyAsync(d.codes.p, codes.data(), codes.size(), hipMemcpyHostToDevice, m->st));
+  HIPOK(hipMemcpyAsync(d.len.p, len.data(), len.size() * 4, hipMemcpyHostToDevice, m->st));
+  if (qs) {
+    HIPOK(d.qual.ensure(qual.size()));
+    HIPOK(hipMemcpyAsync(d.qual.p, qual.data(), qual.size(), hipMemcpyHostToDevice, m->st));
+  }
+  if (!fq.empty()) {
+    HIPOK(d.first_qual.ensure(fq.size()));
+    HIPOK(hipMemcpyAsync(d.first_qual.p, fq.data(), fq.size(), hipMemcpyHostToDevice, m->st));
+  }
+  HIPOK(d.planes.ensure((size_t)(2 * n) * 2 * RW * 4));
+  HIPOK(d.qplanes.ensure((size_t)(2 * n) * QW * 4));
+  PrepArgs pa{};
+  pa.codes = d.codes.as<uint8_t>();
+  pa.code_stride = (int)cs;
+  pa.qual = qs ? d.qual.as<uint8_t>() : nullptr;
+  pa.qual_stride = (int)qs;
+  pa.first_qual = fq.empty() ? nullptr : d.first_qual.as<uint8_t>();
+  pa.len = d.len.as<int32_t>();
+  pa.n = n;
+  pa.color = m->p.color;
+  pa.planes = d.planes.as<uint32_t>();
+  pa.qplanes = d.qplanes.as<uint32_t>();
+  HIPOK(launch_prep_reads(pa, m->st));
+  return STORM_OK;
+}
+
+struct PassStats {
+  int64_t cand = 0, rec = 0;
+  float ms_seed = 0, ms_filter = 0, ms_align = 0, ms_replay = 0;
+};
+
+// traceback bytes of records still referenced by a slot, kept on the host per (pass, rec)
+using TbStore = std::unordered_map<int64_t, std::vector<uint8_t>>;
+
+static float elapsed(storm_mapper* m, hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, a, b);
+  return ms;
+}
+
+// One mapping pass of reads [r0, r1) (device order) against one reference sequence: the stages
+// of process_read (run.c:555-747) as separate launches, with host synchronisations to size
+// the buffers of the next stage.
+static int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int pass_id, int64_t r0, int64_t r1,
+                    TbStore& store, PassStats& stats) {
+  PassScratch& P = m->ps;
+  hipStream_t st = m->st;
+  RefState& rs = *m->rstate[(size_t)ref_id];
+  const int64_t n_rs = 2 * (r1 - r0);
+  if (n_rs == 0) return STORM_OK;
+  ScoreDev sc{};
+  fill_score(m->p, sc);
+  hipEvent_t e[6];
+  for (auto& x : e) HIPOK(hipEventCreate(&x));
+  struct EvGuard { hipEvent_t* e; ~EvGuard() { for (int i = 0; i < 6; ++i) (void)hipEventDestroy(e[i]); } } eg{e};
+  HIPOK(hipEventRecord(e[0], st));
+
+  SeedArgs sa{};
+  sa.r = dr.view();
+  sa.seeds = seed_set(rs);
+  sa.I = m->p.allowed_indels;
+  sa.sense_mask = m->p.sense_mask;
+  sa.rs_begin = 2 * r0;
+  sa.n_rs = n_rs;
+  HIPOK(P.seg_count.ensure((size_t)(n_rs + 1) * 8));
+  HIPOK(P.seg_off.ensure((size_t)(n_rs + 1) * 8));
+  HIPOK(P.cand_off.ensure((size_t)(n_rs + 1) * 8));
+  HIPOK(P.overflow.ensure((size_t)n_rs));
+  HIPOK(P.n_overflow.ensure(4));
+  HIPOK(hipMemsetAsync(P.seg_count.p, 0, (size_t)(n_rs + 1) * 8, st));
+  HIPOK(hipMemsetAsync(P.n_overflow.p, 0, 4, st));
+  sa.seg_count = P.seg_count.as<int64_t>();
+  HIPOK(launch_seed_count(sa, st));
+  size_t tmpb = 0;
+  HIPOK(scan_exclusive_i64(nullptr, nullptr, n_rs + 1, nullptr, tmpb, st));
+  size_t tmpb2 = 0;
+  HIPOK(scan_exclusive_u8(nullptr, nullptr, 1 << 20, nullptr, tmpb2, st));
+  HIPOK(P.cub_tmp.ensure(std::max(tmpb, tmpb2) + (1 << 20)));
+  tmpb = P.cub_tmp.n;
+  HIPOK(scan_exclusive_i64(P.seg_count.as<int64_t>(), P.seg_off.as<int64_t>(), n_rs + 1, P.cub_tmp.p, tmpb, st));
+  int64_t total = 0;
+  HIPOK(hipMemcpyAsync(&total, P.seg_off.as<int64_t>() + n_rs, 8, hipMemcpyDeviceToHost, st));
+  HIPOK(hipStreamSynchronize(st));
+  HIPOK(P.hits.ensure((size_t)std::max<int64_t>(total, 1) * 4));
+  HIPOK(P.kept.ensure((size_t)std::max<int64_t>(total, 1) * 4));
+  sa.seg_off = P.seg_off.as<int64_t>();
+  sa.hits = P.hits.as<int32_t>();
+  sa.kept = P.kept.as<int32_t>();
+  sa.overflow = P.overflow.as<uint8_t>();
+  sa.n_overflow = P.n_overflow.as<int32_t>();
+  HIPOK(launch_seed_gather(sa, st));
+  int32_t n_over = 0;
+  HIPOK(hipMemcpyAsync(&n_over, P.n_overflow.p, 4, hipMemcpyDeviceToHost, st));
+  HIPOK(hipStreamSynchronize(st));
+  if (n_over > 0) {   // segments beyond LDS: device segmented radix sort, then the keep rule
+    HIPOK(P.seg_begin.ensure((size_t)n_rs * 8));
+    HIPOK(P.seg_end.ensure((size_t)n_rs * 8));
+    HIPOK(P.sorted.ensure((size_t)std::max<int64_t>(total, 1) * 4));
+    HIPOK(launch_overflow_ranges(P.seg_off.as<int64_t>(), P.overflow.as<uint8_t>(), n_rs, P.seg_begin.as<int64_t>(),
+                                 P.seg_end.as<int64_t>(), st));
+    size_t sb = 0;
+    HIPOK(segsort_i32(nullptr, nullptr, total, n_rs, nullptr, nullptr, nullptr, sb, st));
+    DevMem tmp;
+    HIPOK(tmp.ensure(sb + 256));
+    sb = tmp.n;
+    HIPOK(segsort_i32(P.hits.as<int32_t>(), P.sorted.as<int32_t>(), total, n_rs, P.seg_begin.as<int64_t>(),
+                      P.seg_end.as<int64_t>(), tmp.p, sb, st));
+    SeedArgs su = sa;
+    su.hits = P.sorted.as<int32_t>();
+    HIPOK(launch_unique_segments(su, st));
+    HIPOK(hipStreamSynchronize(st));
+  }
+  // candidate offsets and the dense list
+  HIPOK(hipMemsetAsync(P.seg_count.as<int64_t>() + n_rs, 0, 8, st));
+  tmpb = P.cub_tmp.n;
+  HIPOK(scan_exclusive_i64(P.seg_count.as<int64_t>(), P.cand_off.as<int64_t>(), n_rs + 1, P.cub_tmp.p, tmpb, st));
+  int64_t n_cand = 0;
+  HIPOK(hipMemcpyAsync(&n_cand, P.cand_off.as<int64_t>() + n_rs, 8, hipMemcpyDeviceToHost, st));
+  HIPOK(hipStreamSynchronize(st));
+  stats.cand += n_cand;
+  const size_t cb = (size_t)std::max<int64_t>(n_cand, 1);
+  HIPOK(P.cand_rs.ensure(cb * 4));
+  HIPOK(P.cand_diag.ensure(cb * 4));
+  HIPOK(P.cand_pos.ensure(cb * 4));
+  HIPOK(P.pass.ensure(cb + 16));
+  HIPOK(P.pass_scan.ensure((cb + 1) * 8));
+  CompactArgs ca{};
+  ca.seg_off = P.seg_off.as<int64_t>();
+  ca.seg_count = P.seg_count.as<int64_t>();
+  ca.cand_off = P.cand_off.as<int64_t>();
+  ca.kept = P.kept.as<int32_t>();
+  ca.rs_begin = 2 * r0;
+  ca.n_rs = n_rs;
+  ca.d = sc.d;
+  ca.cand_rs = P.cand_rs.as<int32_t>();
+  ca.cand_diag = P.cand_diag.as<int32_t>();
+  ca.cand_pos = P.cand_pos.as<int32_t>();
+  HIPOK(launch_compact_candidates(ca, st));
+  HIPOK(hipEventRecord(e[1], st));
+  // filter
+  FilterArgs fa{};
+  fa.g = rs.g.view();
+  fa.rplanes = dr.planes.as<uint32_t>();
+  fa.rlen = dr.len.as<int32_t>();
+  fa.len_shift = 1;
+  fa.cand_read = P.cand_rs.as<int32_t>();
+  fa.cand_pos = P.cand_pos.as<int32_t>();
+  fa.n = n_cand;
+  fa.score = nullptr;
+  fa.pass = P.pass.as<uint8_t>();
+  fa.mt = sc.fmatch; fa.mm = sc.fmismatch; fa.op = sc.fopen; fa.ex = sc.fext; fa.thr = sc.fthr;
+  HIPOK(launch_filter(fa, sc.W, st));
+  HIPOK(hipEventRecord(e[2], st));
+  // records
+  HIPOK(hipMemsetAsync(P.pass.as<uint8_t>() + n_cand, 0, 1, st));
+  size_t tb3 = 0;
+  HIPOK(scan_exclusive_u8(nullptr, nullptr, n_cand + 1, nullptr, tb3, st));
+  HIPOK(P.cub_tmp.ensure(tb3 + 256));
+  tb3 = P.cub_tmp.n;
+  HIPOK(scan_exclusive_u8(P.pass.as<uint8_t>(), P.pass_scan.as<int64_t>(), n_cand + 1, P.cub_tmp.p, tb3, st));
+  int64_t n_rec = 0;
+  HIPOK(hipMemcpyAsync(&n_rec, P.pass_scan.as<int64_t>() + n_cand, 8, hipMemcpyDeviceToHost, st));
+  HIPOK(hipStreamSynchronize(st));
+  stats.rec += n_rec;
+  const size_t rb = (size_t)std::max<int64_t>(n_rec, 1);
+  HIPOK(P.rec_rs.ensure(rb * 4));
+  HIPOK(P.rec_diag.ensure(rb * 4));
+  HIPOK(P.rec_off.ensure((size_t)(n_rs + 1) * 8));
+  HIPOK(P.recs.ensure(rb * sizeof(AlignRec)));
+  HIPOK(P.tb.ensure(rb * TBB));
+  RecArgs ra{};
+  ra.pass = P.pass.as<uint8_t>();
+  ra.pass_scan = P.pass_scan.as<int64_t>();
+  ra.cand_rs = P.cand_rs.as<int32_t>();
+  ra.cand_diag = P.cand_diag.as<int32_t>();
+  ra.n_cand = n_cand;
+  ra.cand_off = P.cand_off.as<int64_t>();
+  ra.n_rs = n_rs;
+  ra.rec_rs = P.rec_rs.as<int32_t>();
+  ra.rec_diag = P.rec_diag.as<int32_t>();
+  ra.rec_off = P.rec_off.as<int64_t>();
+  HIPOK(launch_make_records(ra, st));
+  // DP + traceback, chunked by the scratch budget
+  AlignArgs aa{};
+  aa.g = rs.g.view();
+  aa.r = dr.view();
+  aa.sc = sc;
+  aa.rec_rs = P.rec_rs.as<int32_t>();
+  aa.rec_diag = P.rec_diag.as<int32_t>();
+  aa.out = P.recs.as<AlignRec>();
+  aa.tb = P.tb.as<uint8_t>();
+  int maxL = 0;
+  {
+    std::vector<int32_t> lens((size_t)(r1 - r0));
+    HIPOK(hipMemcpyAsync(lens.data(), dr.len.as<int32_t>() + r0, lens.size() * 4, hipMemcpyDeviceToHost, st));
+    HIPOK(hipStreamSynchronize(st));
+    for (int32_t x : lens) maxL = std::max(maxL, (int)x);
+  }
+  aa.scratch_stride = (((int64_t)maxL * (2 * sc.I + 1)) + 15) & ~15LL;
+  const int64_t budget = 256LL << 20;
+  int64_t chunk = std::max<int64_t>(1, budget / std::max<int64_t>(aa.scratch_stride, 1));
+  HIPOK(P.align_scratch.ensure((size_t)(std::min(chunk, std::max<int64_t>(n_rec, 1)) * aa.scratch_stride)));
+  aa.scratch = P.align_scratch.as<uint8_t>();
+  for (int64_t b0 = 0; b0 < n_rec; b0 += chunk) {
+    aa.base = b0;
+    aa.n = std::min(chunk, n_rec - b0);
+    HIPOK(launch_align(aa, st));
+  }
+  HIPOK(hipEventRecord(e[3], st));
+  // hit map
+  ReplayArgs pa{};
+  pa.rec = P.recs.as<AlignRec>();
+  pa.rec_off = P.rec_off.as<int64_t>();
+  pa.len = dr.len.as<int32_t>();
+  pa.read_begin = r0;
+  pa.n_reads = r1 - r0;
+  pa.ref_id = ref_id;
+  pa.pass = pass_id;
+  pa.t = m->p.t;
+  pa.sense_mask = m->p.sense_mask;
+  pa.slots = slots;
+  HIPOK(launch_replay(pa, st));
+  HIPOK(hipEventRecord(e[4], st));
+  // tracebacks still referenced by a slot go to the host store
+  const int64_t n_slots = 4 * dr.n;
+  HIPOK(P.mark_rec.ensure(rb * 4));
+  HIPOK(P.mark_slot.ensure(rb * 4));
+  HIPOK(P.mark_count.ensure(4));
+  HIPOK(hipMemsetAsync(P.mark_count.p, 0, 4, st));
+  HIPOK(launch_mark_referenced(slots + 4 * r0, 4 * (r1 - r0), pass_id, P.mark_rec.as<int32_t>(), P.mark_slot.as<int32_t>(),
+                               P.mark_count.as<int32_t>(), st));
+  (void)n_slots;
+  int32_t n_mark = 0;
+  HIPOK(hipMemcpyAsync(&n_mark, P.mark_count.p, 4, hipMemcpyDeviceToHost, st));
+  HIPOK(hipStreamSynchronize(st));
+  if (n_mark > 0) {
+    HIPOK(P.gath.ensure((size_t)n_mark * TBB));
+    HIPOK(launch_gather_tb(P.tb.as<uint8_t>(), P.mark_rec.as<int32_t>(), n_mark, P.gath.as<uint8_t>(), st));
+    std::vector<int32_t> ids((size_t)n_mark);
+    std::vector<uint8_t> bytes((size_t)n_mark * TBB);
+    HIPOK(hipMemcpyAsync(ids.data(), P.mark_rec.p, ids.size() * 4, hipMemcpyDeviceToHost, st));
+    HIPOK(hipMemcpyAsync(bytes.data(), P.gath.p, bytes.size(), hipMemcpyDeviceToHost, st));
+    HIPOK(hipStreamSynchronize(st));
+    for (int32_t k = 0; k < n_mark; ++k) {
+      const int64_t key = ((int64_t)pass_id << 32) | (uint32_t)ids[(size_t)k];
+      store[key].assign(bytes.begin() + (size_t)k * TBB, bytes.begin() + (size_t)(k + 1) * TBB);
+    }
+  }
+  HIPOK(hipEventRecord(e[5], st));
+  HIPOK(hipEventSynchronize(e[5]));
+  stats.ms_seed += elapsed(m, e[0], e[1]);
+  stats.ms_filter += elapsed(m, e[1], e[2]);
+  stats.ms_align += elapsed(m, e[2], e[3]);
+  stats.ms_replay += elapsed(m, e[3], e[4]);
+  return STORM_OK;
+}
+
+// whole map call: reads of the view in the given order; results per read in view order
+static int map_view(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
+                    storm_map_stats* st_out) {
+  const int64_t n = v->n_reads;
+  if (n <= 0) return fail(STORM_ERR_INVALID, "no reads");
+  // plain mode: equal-length batches (the reference maps one read length per run); the device
+  // order groups reads by length, stable within a length
+  std::vector<int64_t> order((size_t)n);
+  for (int64_t i = 0; i < n; ++i) order[(size_t)i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return v->len[a] < v->len[b]; });
+  std::vector<std::pair<int64_t, int64_t>> batches;
+  if (mode == STORM_MODE_PLAIN) {
+    int64_t s = 0;
+    for (int64_t i = 1; i <= n; ++i)
+      if (i == n || v->len[order[(size_t)i]] != v->len[order[(size_t)s]]) { batches.emplace_back(s, i); s = i; }
+  } else {
+    batches.emplace_back(0, n);
+  }
+  const double t0 = now_s();
+  DevReads dr;
+  int rc = upload_reads(m, v, order, dr);
+  if (rc) return rc;
+  DevMem slots;
+  HIPOK(slots.ensure((size_t)(4 * n) * sizeof(Slot)));
+  HIPOK(launch_slots_init(slots.as<Slot>(), 4 * n, m->st));
+  TbStore store;
+  PassStats ps;
+  int pass_id = 0;
+  const double ti = m->index_seconds;
+  for (size_t r = 0; r < m->refs.size(); ++r) {
+    rc = ensure_ref_resident(m, r);
+    if (rc) return rc;
+    for (auto& b : batches) {
+      // bound the per-pass buffers: at most 1M reads per launch group
+      for (int64_t c0 = b.first; c0 < b.second; c0 += (1 << 20)) {
+        const int64_t c1 = std::min<int64_t>(b.second, c0 + (1 << 20));
+        rc = run_pass(m, dr, slots.as<Slot>(), (int)r, pass_id++, c0, c1, store, ps);
+        if (rc) return rc;
+      }
+    }
+    if (m->refs.size() > 1) release_ref(m, r);
+  }
+  std::vector<Slot> hs((size_t)(4 * n));
+  HIPOK(hipMemcpyAsync(hs.data(), slots.p, hs.size() * sizeof(Slot), hipMemcpyDeviceToHost, m->st));
+  HIPOK(hipStreamSynchronize(m->st));
+  out_slots.assign((size_t)(4 * n), HostSlot{});
+  int64_t mapped = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t j = order[(size_t)i];
+    if (hs[(size_t)(4 * i)].ref_id != -1) mapped++;
+    for (int k = 0; k < 4; ++k) {
+      const Slot& s = hs[(size_t)(4 * i + k)];
+      HostSlot& h = out_slots[(size_t)(4 * j + k)];
+      h.score = s.score; h.ref_id = s.ref_id; h.ref_start = s.ref_start; h.ref_end = s.ref_end;
+      h.read_start = s.read_start; h.read_end = s.read_end; h.sense = s.sense;
+      h.tb_len = s.tb_len;
+      if (s.ref_id != -1) {
+        auto it = store.find(((int64_t)s.pass << 32) | (uint32_t)s.rec);
+        if (it == store.end()) return fail(STORM_ERR_DEVICE, "traceback of a hit was not gathered");
+        h.tb = it->second;
+      }
+    }
+  }
+  if (st_out) {
+    st_out->n_reads = n;
+    st_out->n_mapped = mapped;
+    st_out->n_candidates = ps.cand;
+    st_out->n_filter_pass = ps.rec;
+    st_out->n_accepted = 0;
+    st_out->seconds_index = m->index_seconds - ti;
+    st_out->seconds_map = now_s() - t0 - st_out->seconds_index;
+    st_out->gpu_ms_seed = ps.ms_seed;
+    st_out->gpu_ms_filter = ps.ms_filter;
+    st_out->gpu_ms_align = ps.ms_align;
+    st_out->gpu_ms_replay = ps.ms_replay;
+  }
+  return STORM_OK;
+}
+
+}  // namespace storm
+
+// =============================================================================== C ABI
+using namespace storm;
+
+extern "C" {
+
+const char* storm_last_error(void) { return g_err.c_str(); }
+
+void storm_params_default(storm_params* p, int color) {   // alignment.h:24-28,51-52; main.c:37-38
+  memset(p, 0, sizeof(*p));
+  p->color = color ? 1 : 0;
+  p->match = 5; p->mismatch = -4; p->gap_open = -9; p->gap_extend = -4;
+  p->allowed_indels = 3;
+  p->t = 115; p->z = 100;
+  p->qual_base = 33;
+  p->stdev_distance = 100;
+  p->sense_mask = 3;
+  p->max_hits = 1;
+  p->seeds = nullptr;
+}
+
+static const char* default_seeds(int color) {   // seed.h:29-32
+  return color ? "####----##--##-####;####-###--#----####;####-####-####"
+               : "####--#-#--##--####;#####--###-#-###;###-##-#######";
+}
+
+static int check_params(const storm_params* p) {
+  if (!p) return fail(STORM_ERR_INVALID, "no parameters");
+  if (p->allowed_indels < 0 || p->allowed_indels > MAX_I)
+    return fail(STORM_ERR_INVALID, "-i %d outside 0..%d", p->allowed_indels, MAX_I);
+  if (p->max_hits < 1 || p->max_hits > 4) return fail(STORM_ERR_INVALID, "-M must be in 1..4");
+  if (p->match <= 0 || p->mismatch > 0 || p->gap_open > 0 || p->gap_extend > 0)
+    return fail(STORM_ERR_INVALID, "scores: match > 0, mismatch/gap penalties <= 0 (main.c:262-291)");
+  if (p->sense_mask < 1 || p->sense_mask > 3) return fail(STORM_ERR_INVALID, "sense_mask must be 1, 2 or 3");
+  return STORM_OK;
+}
+
+int storm_mapper_create(int device, const char* genome_path, const storm_params* params, storm_mapper** out) {
+  *out = nullptr;
+  int rc = check_params(params);
+  if (rc) return rc;
+  std::unique_ptr<storm_mapper> m(new storm_mapper());
+  m->device = device;
+  m->p = *params;
+  m->seeds_str = params->seeds ? params->seeds : default_seeds(params->color);
+  m->p.seeds = m->seeds_str.c_str();
+  std::string err;
+  rc = parse_seeds(m->seeds_str.c_str(), m->seeds, err);
+  if (rc) return fail(rc, "%s", err.c_str());
+  HIPOK(hipSetDevice(device));
+  HIPOK(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
+  rc = load_reference(genome_path, m->p, m->refs, err);
+  if (rc) return fail(rc, "%s", err.c_str());
+  for (size_t r = 0; r < m->refs.size(); ++r) m->rstate.emplace_back(new RefState());
+  if (m->refs.size() == 1) {   // one sequence: keep genome + indexes resident in HBM
+    rc = ensure_ref_resident(m.get(), 0);
+    if (rc) return rc;
+  }
+  *out = m.release();
+  return STORM_OK;
+}
+
+void storm_mapper_destroy(storm_mapper* m) {
+  if (!m) return;
+  (void)hipSetDevice(m->device);
+  m->rstate.clear();
+  m->ps = PassScratch();
+  if (m->st) (void)hipStreamDestroy(m->st);
+  delete m;
+}
+
+int64_t storm_mapper_free_bytes(storm_mapper* m) {
+  size_t fr = 0, tot = 0;
+  if (!m || hipSetDevice(m->device) != hipSuccess || hipMemGetInfo(&fr, &tot) != hipSuccess) return -1;
+  return (int64_t)fr;
+}
+
+static int build_host_reads(const storm_reads_view* v, int color, HostReads& hr) {
+  hr = HostReads();
+  hr.n = v->n_reads;
+  hr.db_name = v->db_name ? v->db_name : "";
+  hr.code_stride = v->code_stride;
+  hr.qual_stride = v->qual ? v->qual_stride : 0;
+  hr.has_qual = v->qual != nullptr;
+  hr.names.resize((size_t)hr.n);
+  hr.len.assign(v->len, v->len + hr.n);
+  hr.codes.assign(v->codes, v->codes + hr.n * v->code_stride);
+  if (v->qual) hr.qual.assign(v->qual, v->qual + hr.n * v->qual_stride);
+  if (color) {
+    hr.first_base.assign(v->first_base ? v->first_base : nullptr, v->first_base ? v->first_base + hr.n : nullptr);
+    hr.first_qual.assign(v->first_qual ? v->first_qual : nullptr, v->first_qual ? v->first_qual + hr.n : nullptr);
+    if (hr.first_base.empty()) hr.first_base.assign((size_t)hr.n, 0);
+    if (hr.first_qual.empty()) hr.first_qual.assign((size_t)hr.n, 0);
+  }
+  for (int64_t i = 0; i < hr.n; ++i) {
+    hr.names[(size_t)i] = v->names ? v->names[i] : std::to_string(i);
+    hr.max_len = std::max(hr.max_len, (int)v->len[i]);
+  }
+  return STORM_OK;
+}
+
+int storm_mapper_map(storm_mapper* m, const storm_reads_view* reads, int mode, const char* sam_path, int argc,
+                     char** argv, storm_map_stats* stats) {
+  if (!m || !reads) return fail(STORM_ERR_INVALID, "null argument");
+  HIPOK(hipSetDevice(m->device));
+  std::vector<HostSlot> slots;
+  storm_map_stats local{};
+  int rc = map_view(m, reads, mode, slots, &local);
+  if (rc) return rc;
+  const double t0 = now_s();
+  HostReads hr;
+  build_host_reads(reads, m->p.color, hr);
+  std::string err;
+  rc = write_sam(sam_path, m->refs, hr, slots, m->p, argc, argv, err);
+  if (rc) return fail(rc, "%s", err.c_str());
+  local.seconds_sam = now_s() - t0;
+  if (stats) *stats = local;
+  return STORM_OK;
+}
+
+int storm_mapper_map_hits(storm_mapper* m, const storm_reads_view* reads, int mode, int32_t* score, int32_t* ref_id,
+                          int32_t* ref_start, int32_t* sense, storm_map_stats* stats) {
+  if (!m || !reads) return fail(STORM_ERR_INVALID, "null argument");
+  HIPOK(hipSetDevice(m->device));
+  std::vector<HostSlot> slots;
+  storm_map_stats local{};
+  int rc = map_view(m, reads, mode, slots, &local);
+  if (rc) return rc;
+  for (size_t k = 0; k < slots.size(); ++k) {
+    if (score) score[k] = slots[k].score;
+    if (ref_id) ref_id[k] = slots[k].ref_id;
+    if (ref_start) ref_start[k] = slots[k].ref_start;
+    if (sense) sense[k] = slots[k].sense;
+  }
+  if (stats) *stats = local;
+  return STORM_OK;
+}
+
+int storm_map_files(int device, const char* genome_path, const char* reads_path, const char* qual_path,
+                    const storm_params* params, int mode, const char* sam_path, int argc, char** argv,
+                    storm_map_stats* stats) {
+  int rc = check_params(params);
+  if (rc) return rc;
+  const double t0 = now_s();
+  HostReads hr;
+  std::string err;
+  rc = load_reads(reads_path, qual_path, *params, hr, err);
+  if (rc) return fail(rc, "%s", err.c_str());
+  sort_reads_by_tag(hr);   // run.c:826
+  const double t1 = now_s();
+  storm_mapper* m = nullptr;
+  rc = storm_mapper_create(device, genome_path, params, &m);
+  if (rc) return rc;
+  std::vector<const char*> names((size_t)hr.n);
+  for (int64_t i = 0; i < hr.n; ++i) names[(size_t)i] = hr.names[(size_t)i].c_str();
+  storm_reads_view v{};
+  v.n_reads = hr.n;
+  v.names = names.data();
+  v.codes = hr.codes.data();
+  v.code_stride = hr.code_stride;
+  v.qual = hr.has_qual ? hr.qual.data() : nullptr;
+  v.qual_stride = hr.qual_stride;
+  v.len = hr.len.data();
+  v.first_base = hr.first_base.empty() ? nullptr : hr.first_base.data();
+  v.first_qual = hr.first_qual.empty() ? nullptr : hr.first_qual.data();
+  v.db_name = hr.db_name.c_str();
+  storm_map_stats local{};
+  rc = storm_mapper_map(m, &v, mode, sam_path, argc, argv, &local);
+  local.seconds_index += m->index_seconds;
+  storm_mapper_destroy(m);
+  if (rc) return rc;
+  local.seconds_load = t1 - t0;
+  if (stats) *stats = local;
+  return STORM_OK;
+}
+
+// ------------------------------------------------------------------------------ filter plan
+struct storm_filter_plan {
+  int device = 0;
+  hipStream_t st = nullptr;
+  DevGenome g;
+  DevMem planes, len, cand_read, cand_pos, score, pass;
+  FilterArgs a{};
+  int W = 0;
+  int64_t n = 0;
+};
+
+static void pack_read_planes(const uint8_t* codes, int L, uint32_t* out) {
+  for (int w = 0; w < 2 * RW; ++w) out[w] = 0;
+  for (int i = 0; i < L; ++i) {
+    const int c = (codes[i >> 2] >> ((i & 3) << 1)) & 3;
+    out[i >> 5] |= (uint32_t)(c & 1) << (i & 31);
+    out[RW + (i >> 5)] |= (uint32_t)((c >> 1) & 1) << (i & 31);
+  }
+}
+
+int storm_filter_plan_create(int device, const uint8_t* genome_packed, int64_t genome_codes, const uint8_t* reads_packed,
+                             int64_t read_stride, const int32_t* read_len, int64_t n_reads, const int32_t* cand_read,
+                             const int32_t* cand_pos, int64_t n_cand, int W, int match, int mismatch, int gap_open,
+                             int gap_extend, int z, storm_filter_plan** out) {
+  *out = nullptr;
+  if (!(W == 2 || W == 4 || W == 8 || W == 16)) return fail(STORM_ERR_INVALID, "W must be 2, 4, 8 or 16");
+  for (int64_t r = 0; r < n_reads; ++r)
+    if (read_len[r] < W || read_len[r] > MAXL)
+      return fail(STORM_ERR_INVALID, "read %lld length %d outside [W=%d, %d]", (long long)r, read_len[r], W, MAXL);
+  for (int64_t c = 0; c < n_cand; ++c) {
+    if (cand_read[c] < 0 || cand_read[c] >= n_reads) return fail(STORM_ERR_INVALID, "candidate %lld: bad read", (long long)c);
+    const int64_t p = cand_pos[c];
+    if (p < -(GPAD - 64) || p + read_len[cand_read[c]] + 2 * W > genome_codes + GPAD - 64)
+      return fail(STORM_ERR_INVALID, "candidate %lld: window start %lld outside the padded genome", (long long)c,
+                  (long long)p);
+  }
+  std::unique_ptr<storm_filter_plan> pl(new storm_filter_plan());
+  pl->device = device;
+  HIPOK(hipSetDevice(device));
+  HIPOK(hipStreamCreateWithFlags(&pl->st, hipStreamNonBlocking));
+  std::vector<uint8_t> codes((size_t)genome_codes);
+  for (int64_t i = 0; i < genome_codes; ++i) codes[(size_t)i] = (genome_packed[i >> 2] >> ((i & 3) << 1)) & 3;
+  int rc = upload_genome(codes.data(), nullptr, genome_codes, pl->g, pl->st);
+  if (rc) return rc;
+  std::vector<uint32_t> planes((size_t)n_reads * 2 * RW);
+  for (int64_t r = 0; r < n_reads; ++r) pack_read_planes(reads_packed + r * read_stride, read_len[r], &planes[(size_t)r * 2 * RW]);
+  HIPOK(pl->planes.ensure(planes.size() * 4 + 16));
+  HIPOK(pl->len.ensure((size_t)n_reads * 4 + 16));
+  HIPOK(pl->cand_read.ensure((size_t)n_cand * 4 + 16));
+  HIPOK(pl->cand_pos.ensure((size_t)n_cand * 4 + 16));
+  HIPOK(pl->score.ensure((size_t)n_cand + 16));
+  HIPOK(pl->pass.ensure((size_t)n_cand + 16));
+  HIPOK(hipMemcpyAsync(pl->planes.p, planes.data(), planes.size() * 4, hipMemcpyHostToDevice, pl->st));
+  HIPOK(hipMemcpyAsync(pl->len.p, read_len, (size_t)n_reads * 4, hipMemcpyHostToDevice, pl->st));
+  HIPOK(hipMemcpyAsync(pl->cand_read.p, cand_read, (size_t)n_cand * 4, hipMemcpyHostToDevice, pl->st));
+  HIPOK(hipMemcpyAsync(pl->cand_pos.p, cand_pos, (size_t)n_cand * 4, hipMemcpyHostToDevice, pl->st));
+  HIPOK(hipStreamSynchronize(pl->st));
+  FilterArgs& a = pl->a;
+  a.g = pl->g.view();
+  a.rplanes = pl->planes.as<uint32_t>();
+  a.rlen = pl->len.as<int32_t>();
+  a.len_shift = 0;
+  a.cand_read = pl->cand_read.as<int32_t>();
+  a.cand_pos = pl->cand_pos.as<int32_t>();
+  a.n = n_cand;
+  a.score = pl->score.as<uint8_t>();
+  a.pass = pl->pass.as<uint8_t>();
+  a.mt = std::abs(match) & 0xff; a.mm = std::abs(mismatch) & 0xff;
+  a.op = std::abs(gap_open) & 0xff; a.ex = std::abs(gap_extend) & 0xff;
+  unsigned u = (unsigned)z;
+  if (u >= 255) u = 254;
+  a.thr = (int)u;
+  pl->W = W;
+  pl->n = n_cand;
+  *out = pl.release();
+  return STORM_OK;
+}
+
+int storm_filter_run(storm_filter_plan* pl, int reps, double* ms_per_launch) {
+  if (!pl || reps < 1) return fail(STORM_ERR_INVALID, "bad plan or reps");
+  HIPOK(hipSetDevice(pl->device));
+  hipEvent_t a, b;
+  HIPOK(hipEventCreate(&a));
+  HIPOK(hipEventCreate(&b));
+  HIPOK(hipEventRecord(a, pl->st));
+  for (int k = 0; k < reps; ++k) HIPOK(launch_filter(pl->a, pl->W, pl->st));
+  HIPOK(hipEventRecord(b, pl->st));
+  HIPOK(hipEventSynchronize(b));
+  float ms = 0;
+  HIPOK(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if (ms_per_launch) *ms_per_launch = ms / reps;
+  return STORM_OK;
+}
+
+int storm_filter_fetch(storm_filter_plan* pl, uint8_t* scores, uint8_t* pass) {
+  if (!pl) return fail(STORM_ERR_INVALID, "bad plan");
+  HIPOK(hipSetDevice(pl->device));
+  if (scores) HIPOK(hipMemcpyAsync(scores, pl->score.p, (size_t)pl->n, hipMemcpyDeviceToHost, pl->st));
+  if (pass) HIPOK(hipMemcpyAsync(pass, pl->pass.p, (size_t)pl->n, hipMemcpyDeviceToHost, pl->st));
+  HIPOK(hipStreamSynchronize(pl->st));
+  return STORM_OK;
+}
+
+void storm_filter_plan_destroy(storm_filter_plan* pl) {
+  if (!pl) return;
+  (void)hipSetDevice(pl->device);
+  if (pl->st) (void)hipStreamSynchronize(pl->st);
+  pl->g = DevGenome();
+  if (pl->st) (void)hipStreamDestroy(pl->st);
+  delete pl;
+}
+
+// ------------------------------------------------------------------------------ per-call rows
+// Process-global state like the reference filter's (alignment_simd.c:1032-1043): init sets the
+// constants and read length; the genome registered once is resident in HBM.
+namespace {
+struct CallState {
+  DevGenome g;
+  const unsigned char* code0 = nullptr;
+  int64_t genome_codes = 0;
+  int64_t lead_codes = 0;
+  DevMem planes, len, cand_read, cand_pos, pass;
+  hipStream_t st = nullptr;
+  int W = 0, L = 0, thr = 0, mt = 5, mm = 4, op = 9, ex = 4;
+  bool ready = false;
+};
+CallState g_call;
+}  // namespace
+
+int alignment_gfx950__set_genome(const unsigned char* sequence_alloc, size_t alloc_bytes, size_t code0_offset_bytes) {
+  if (!sequence_alloc || code0_offset_bytes > alloc_bytes) return fail(STORM_ERR_INVALID, "bad genome buffer");
+  CallState& c = g_call;
+  HIPOK(hipSetDevice(0));
+  if (!c.st) HIPOK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
+  // the whole allocation (padding included) becomes the device genome; code 0 sits at
+  // code0_offset_bytes * 4 codes into it
+  const int64_t codes = (int64_t)alloc_bytes * 4;
+  std::vector<uint8_t> cv((size_t)codes);
+  for (int64_t i = 0; i < codes; ++i) cv[(size_t)i] = (sequence_alloc[i >> 2] >> ((i & 3) << 1)) & 3;
+  int rc = upload_genome(cv.data(), nullptr, codes, c.g, c.st);
+  if (rc) return rc;
+  c.code0 = sequence_alloc + code0_offset_bytes;
+  c.genome_codes = codes;
+  c.lead_codes = (int64_t)code0_offset_bytes * 4;
+  HIPOK(c.planes.ensure(2 * RW * 4));
+  HIPOK(c.len.ensure(16));
+  HIPOK(c.cand_read.ensure(64 * 4));
+  HIPOK(c.cand_pos.ensure(64 * 4));
+  HIPOK(c.pass.ensure(64));
+  return STORM_OK;
+}
+
+static void call_init(int W, unsigned match, unsigned mismatch, unsigned gapopen, unsigned gapextends, unsigned threshold,
+                      unsigned readlength) {
+  CallState& c = g_call;
+  c.W = W;
+  c.L = (int)readlength;
+  c.mt = (int)(match & 0xff); c.mm = (int)(mismatch & 0xff);
+  c.op = (int)(gapopen & 0xff); c.ex = (int)(gapextends & 0xff);
+  unsigned u = threshold;
+  if (u >= 255) u = 254;   // alignment_simd.c:3643-3645
+  c.thr = (int)u;
+  c.ready = true;
+}
+
+void alignment_gfx950__init_octa(unsigned a, unsigned b, unsigned c, unsigned d, unsigned e, unsigned f) { call_init(2, a, b, c, d, e, f); }
+void alignment_gfx950__init_quad(unsigned a, unsigned b, unsigned c, unsigned d, unsigned e, unsigned f) { call_init(4, a, b, c, d, e, f); }
+void alignment_gfx950__init_pair(unsigned a, unsigned b, unsigned c, unsigned d, unsigned e, unsigned f) { call_init(8, a, b, c, d, e, f); }
+void alignment_gfx950__init_mono(unsigned a, unsigned b, unsigned c, unsigned d, unsigned e, unsigned f) { call_init(16, a, b, c, d, e, f); }
+
+// One call filters simd_mul windows of one read (run.c:660); returns bit k for window k.
+static unsigned call_align(int W, int mul, unsigned char* genome, int* pos_genome, unsigned char* read) {
+  CallState& c = g_call;
+  if (!c.ready || c.W != W || !c.code0 || genome != c.code0) {
+    fail(STORM_ERR_INVALID, "alignment_gfx950: init/set_genome missing or genome pointer not registered");
+    return 0;
+  }
+  if (c.L < W || c.L > MAXL) { fail(STORM_ERR_INVALID, "read length %d not supported", c.L); return 0; }
+  uint32_t planes[2 * RW];
+  pack_read_planes(read, c.L, planes);
+  int32_t rd[64], ps[64];
+  for (int k = 0; k < mul; ++k) { rd[k] = 0; ps[k] = (int32_t)(pos_genome[k] + c.lead_codes); }
+  int32_t L = c.L;
+  uint8_t pass[64];
+  if (hipMemcpyAsync(c.planes.p, planes, sizeof planes, hipMemcpyHostToDevice, c.st) != hipSuccess ||
+      hipMemcpyAsync(c.len.p, &L, 4, hipMemcpyHostToDevice, c.st) != hipSuccess ||
+      hipMemcpyAsync(c.cand_read.p, rd, (size_t)mul * 4, hipMemcpyHostToDevice, c.st) != hipSuccess ||
+      hipMemcpyAsync(c.cand_pos.p, ps, (size_t)mul * 4, hipMemcpyHostToDevice, c.st) != hipSuccess) {
+    fail(STORM_ERR_DEVICE, "alignment_gfx950: copy failed");
+    return 0;
+  }
+  FilterArgs a{};
+  a.g = c.g.view();
+  a.rplanes = c.planes.as<uint32_t>();
+  a.rlen = c.len.as<int32_t>();
+  a.len_shift = 0;
+  a.cand_read = c.cand_read.as<int32_t>();
+  a.cand_pos = c.cand_pos.as<int32_t>();
+  a.n = mul;
+  a.pass = c.pass.as<uint8_t>();
+  a.mt = c.mt; a.mm = c.mm; a.op = c.op; a.ex = c.ex; a.thr = c.thr;
+  if (launch_filter(a, W, c.st) != hipSuccess ||
+      hipMemcpyAsync(pass, c.pass.p, (size_t)mul, hipMemcpyDeviceToHost, c.st) != hipSuccess ||
+      hipStreamSynchronize(c.st) != hipSuccess) {
+    fail(STORM_ERR_DEVICE, "alignment_gfx950: launch failed");
+    return 0;
+  }
+  unsigned x = 0;
+  for (int k = 0; k < mul; ++k) x |= (unsigned)(pass[k] & 1) << k;
+  return x;
+}
+
+// simd_mul of the SSE2 row (run.c:109): octa 8, quad 4, pair 2, mono 1
+unsigned alignment_gfx950__align_octa(unsigned char* g, int* p, unsigned char* r) { return call_align(2, 8, g, p, r); }
+unsigned alignment_gfx950__align_quad(unsigned char* g, int* p, unsigned char* r) { return call_align(4, 4, g, p, r); }
+unsigned alignment_gfx950__align_pair(unsigned char* g, int* p, unsigned char* r) { return call_align(8, 2, g, p, r); }
+unsigned alignment_gfx950__align_mono(unsigned char* g, int* p, unsigned char* r) { return call_align(16, 1, g, p, r); }
+
+void alignment_gfx950__clean(void) {
+  CallState& c = g_call;
+  c.g = DevGenome();
+  c.planes.release(); c.len.release(); c.cand_read.release(); c.cand_pos.release(); c.pass.release();
+  if (c.st) { (void)hipStreamDestroy(c.st); c.st = nullptr; }
+  c.code0 = nullptr;
+  c.ready = false;
+}
+
+int alignment_gfx950__compatible_proc(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return 0;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 0;
+  return strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+// k_replay.hip -- per-read hit map (stage d tail) for gfx950.
+//
+// Semantics: the sequential part of process_read (run.c:660-697) and hit_map__update
+// (hit_map.c:50-144).  For one read, one reference sequence and one strand the running
+// threshold T starts at -t; the records (filter passes, ascending diagonal) are visited in order,
+// a record is accepted when S >= T and score > T (the early exit of alignment.c:613-616), it
+// updates the four slots of the read with the duplicate / ordered-insert rules, and T rises to
+// slot[3].score when that is larger.  One thread owns one read; the slots persist across
+// reference sequences and strands in HBM.
+#include "storm_kernels.h"
+
+namespace storm {
+
+__global__ __launch_bounds__(128) void k_replay(ReplayArgs a) {
+  const int64_t lr = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (lr >= a.n_reads) return;
+  const int64_t read = a.read_begin + lr;
+  Slot s[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) s[k] = a.slots[read * 4 + k];
+  const int L = a.len[read];
+  bool dirty = false;
+  for (int strand = 0; strand < 2; ++strand) {
+    if (!((a.sense_mask >> strand) & 1)) continue;
+    const int64_t b = 2 * lr + strand;
+    int T = a.t;
+    for (int64_t r = a.rec_off[b]; r < a.rec_off[b + 1]; ++r) {
+      const AlignRec x = a.rec[r];
+      if (!(x.S >= T && x.score > T)) continue;
+      const int score = x.score;
+      const int ref_start = x.win + x.best_j0, ref_end = x.win + x.best_j;
+      const int read_start = x.best_i0, read_end = L - 1;
+      int k;
+      bool drop = false;
+      for (k = 0; k < 4 && s[k].ref_id != -1; ++k) {
+        if (s[k].ref_id == a.ref_id &&
+            ((s[k].ref_start == ref_start && s[k].read_start == read_start) ||
+             (s[k].ref_end == ref_end && s[k].read_end == read_end))) {
+          if (score > s[k].score) break;
+          drop = true;
+          break;
+        }
+      }
+      if (!drop) {
+        if (k == 4) k = 3;
+        int i = k;
+        while (i >= 0 && score > s[i].score) --i;
+        if (i != 3) {
+          for (int j = k; j > i + 1; --j) s[j] = s[j - 1];
+          Slot n;
+          n.score = score; n.ref_id = a.ref_id; n.ref_start = ref_start; n.ref_end = ref_end;
+          n.read_start = read_start; n.read_end = read_end; n.sense = strand ? -1 : 1;
+          n.pass = a.pass; n.rec = (int32_t)r; n.tb_len = x.tb_len;
+          s[i + 1] = n;
+          dirty = true;
+        }
+      }
+      if (s[3].score > T) T = s[3].score;
+    }
+  }
+  if (dirty) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a.slots[read * 4 + k] = s[k];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_slots_init(Slot* slots, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  Slot x;
+  x.score = MIN_SCORE; x.ref_id = -1; x.ref_start = -1; x.ref_end = -1;
+  x.read_start = -1; x.read_end = -1; x.sense = 0; x.pass = -1; x.rec = -1; x.tb_len = 0;
+  slots[t] = x;
+}
+
+// records of pass `pass` that a slot still references: their ids, compacted with an atomic
+__global__ __launch_bounds__(256) void k_mark_referenced(const Slot* slots, int64_t n_slots, int pass,
+                                                         int32_t* out_rec, int32_t* out_slot, int32_t* counter) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_slots) return;
+  if (slots[t].pass == pass && slots[t].rec >= 0) {
+    const int o = atomicAdd(counter, 1);
+    out_rec[o] = slots[t].rec;
+    out_slot[o] = (int32_t)t;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_gather_tb(const uint8_t* tb, int tbb, const int32_t* recs, int n, uint8_t* out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n * tbb) return;
+  const int64_t e = t / tbb, b = t % tbb;
+  out[t] = tb[(int64_t)recs[e] * tbb + b];
+}
+
+
+hipError_t launch_replay(const ReplayArgs& a, hipStream_t st) {
+  if (a.n_reads == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_replay, dim3((unsigned)((a.n_reads + 127) / 128)), dim3(128), 0, st, a);
+  return hipGetLastError();
+}
+hipError_t launch_slots_init(Slot* slots, int64_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_slots_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, slots, n);
+  return hipGetLastError();
+}
+hipError_t launch_mark_referenced(const Slot* slots, int64_t n_slots, int pass, int32_t* out_rec, int32_t* out_slot,
+                                  int32_t* counter, hipStream_t st) {
+  if (n_slots == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_mark_referenced, dim3((unsigned)((n_slots + 255) / 256)), dim3(256), 0, st, slots, n_slots, pass,
+                     out_rec, out_slot, counter);
+  return hipGetLastError();
+}
+hipError_t launch_gather_tb(const uint8_t* tb, const int32_t* recs, int n, uint8_t* out, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  const int64_t total = (int64_t)n * TBB;
+  hipLaunchKernelGGL(k_gather_tb, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, tb, (int)TBB, recs, n, out);
+  return hipGetLastError();
+}
+
+}  // namespace storm

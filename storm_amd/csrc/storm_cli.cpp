@@ -1,0 +1,62 @@
+// storm_cli.cpp -- command line with the reference's flags for the -M mapping path (main.c:188-433),
+// running the gfx950 pipeline through the C ABI.  Built as storm-gpu-nucleotide / storm-gpu-color.
+#include <unistd.h>
+
+#include <cstdio>
+#include <cstdlib>
+
+#include "../../include/storm_gpu.h"
+
+#ifndef STORM_CLI_COLOR
+#define STORM_CLI_COLOR 0
+#endif
+
+int main(int argc, char** argv) {
+  storm_params p;
+  storm_params_default(&p, STORM_CLI_COLOR);
+  const char *genome = nullptr, *reads = nullptr, *quals = nullptr, *out = "/dev/stdout";
+  int mode = STORM_MODE_BATCHED, device = 0, opt;
+  p.max_hits = 0;
+  while ((opt = getopt(argc, argv, "g:r:q:s:b:m:x:d:e:t:z:i:v:o:l:u:M:N:jFRAPD:")) != -1) {
+    switch (opt) {
+      case 'g': genome = optarg; break;
+      case 'r': reads = optarg; break;
+      case 'q': quals = optarg; break;
+      case 's': p.seeds = optarg; break;
+      case 'b': p.qual_base = atoi(optarg); break;
+      case 'm': p.match = atoi(optarg); break;
+      case 'x': p.mismatch = atoi(optarg); break;
+      case 'd': p.gap_open = atoi(optarg); break;
+      case 'e': p.gap_extend = atoi(optarg); break;
+      case 't': p.t = atoi(optarg); break;
+      case 'z': p.z = atoi(optarg); break;
+      case 'i': p.allowed_indels = atoi(optarg); break;
+      case 'o': out = optarg; break;
+      case 'l': p.stdev_distance = atoi(optarg); break;
+      case 'u': p.uniqueness = atoi(optarg); break;
+      case 'M': p.max_hits = atoi(optarg); break;
+      case 'j': p.ignore_lowercase = 1; break;
+      case 'F': p.sense_mask = 1; break;
+      case 'R': p.sense_mask = 2; break;
+      case 'A': p.print_all = 1; break;
+      case 'P': mode = STORM_MODE_PLAIN; break;     // plain per-batch path
+      case 'D': device = atoi(optarg); break;
+      case 'v': case 'N': break;
+      default: return 1;
+    }
+  }
+  if (!genome || !reads || p.max_hits < 1) {
+    fprintf(stderr, "usage: %s -g genome.fa -r reads.{fq,fa,csfasta} [-q reads.qual] -M <1..4> [reference flags] [-P]\n", argv[0]);
+    return 1;
+  }
+  storm_map_stats st{};
+  int rc = storm_map_files(device, genome, reads, quals, &p, mode, out, argc, argv, &st);
+  if (rc) {
+    fprintf(stderr, "%s: %s\n", argv[0], storm_last_error());
+    return 2;
+  }
+  fprintf(stderr, "mapped %lld of %lld reads; candidates %lld, filter passes %lld; load %.3fs index %.3fs map %.3fs sam %.3fs\n",
+          (long long)st.n_mapped, (long long)st.n_reads, (long long)st.n_candidates, (long long)st.n_filter_pass,
+          st.seconds_load, st.seconds_index, st.seconds_map, st.seconds_sam);
+  return 0;
+}

@@ -1,0 +1,98 @@
+// storm_dev.h -- device-side data layout and helpers shared by the gfx950 kernels.
+//
+// HBM layout (all little-endian 32-bit words):
+//   genome   : three bit planes (lo = code bit 0, hi = code bit 1, mask = 1 when masked), each
+//              with GPAD codes of padding before code 0 and after the last code.  Padding codes
+//              are 0 (A) and masked, as in the reference buffers (reference_data.c:75-88,184-188).
+//   reads    : per read and strand, RW words of the lo plane then RW words of the hi plane
+//              (row r, strand s at (2r+s)*2*RW), plus QW words of 2-bit quality levels.
+//   index    : per seed, first[4^w + 1] and map[] (CSR, index.c:157-188), int32.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace storm {
+
+constexpr int GPAD = 1024;          // padding codes on each side of a genome (multiple of 32)
+constexpr int MAXL = 256;           // longest read handled by the kernels
+constexpr int RW = MAXL / 32;       // words per read bit plane
+constexpr int QW = MAXL / 16;       // words of 2-bit quality levels per read strand
+constexpr int MIN_SCORE = -16383;   // alignment.h:47
+constexpr int MAX_SEEDS = 8;
+constexpr int MAX_SEED_WEIGHT = 14; // first[] holds 4^w + 1 entries
+constexpr int MAX_I = 31;           // largest -i the DP kernel accepts
+
+struct GenomeDev {
+  const uint32_t* lo;
+  const uint32_t* hi;
+  const uint32_t* mask;
+  int64_t size;       // codes
+};
+
+struct ReadsDev {
+  const uint32_t* planes;   // (2*n)*(2*RW) words
+  const uint32_t* qual;     // (2*n)*QW words, 2-bit levels
+  const int32_t* len;       // n
+  int64_t n;
+  int has_qual;
+};
+
+struct SeedDev {
+  int weight, span, n_pos;
+  int offs[32];             // offsets of the '#' symbols
+  int pos[64];              // explicit read positions when n_pos > 0
+  const int32_t* first;
+  const int32_t* map;
+};
+
+struct SeedSetDev {
+  int n;
+  SeedDev s[MAX_SEEDS];
+};
+
+// scoring constants of the filter (alignment_simd.c:3640-3717) and the DP (alignment.c:255-285)
+struct ScoreDev {
+  int fmatch, fmismatch, fopen, fext;   // |m| |x| |o| |e| as u8
+  int fthr;                             // min(z, 254)
+  int ps[2][4];                         // pair_scores[match/mismatch][level]
+  int gap_open, gap_ext;                // signed
+  int color;
+  int I;                                // -i
+  int d;                                // min(I, 15)
+  int W;                                // filter lanes
+  int t;                                // -t
+};
+
+__device__ __forceinline__ uint32_t gbit_word(const uint32_t* plane, int64_t bit) {
+  // 32 bits of `plane` starting at bit index `bit` (bit >= 0)
+  int64_t w = bit >> 5;
+  uint32_t s = (uint32_t)(bit & 31);
+  return __builtin_amdgcn_alignbit(plane[w + 1], plane[w], s);
+}
+
+__device__ __forceinline__ int gcode(const GenomeDev& g, int64_t pos) {
+  int64_t b = pos + GPAD;
+  uint32_t l = (g.lo[b >> 5] >> (b & 31)) & 1u;
+  uint32_t h = (g.hi[b >> 5] >> (b & 31)) & 1u;
+  return (int)(l | (h << 1));
+}
+
+__device__ __forceinline__ int gmask(const GenomeDev& g, int64_t pos) {
+  int64_t b = pos + GPAD;
+  return (int)((g.mask[b >> 5] >> (b & 31)) & 1u);
+}
+
+__device__ __forceinline__ const uint32_t* read_planes(const ReadsDev& r, int64_t rs) {
+  return r.planes + rs * (2 * RW);
+}
+
+__device__ __forceinline__ int rcode(const uint32_t* pl, int i) {
+  return (int)(((pl[i >> 5] >> (i & 31)) & 1u) | (((pl[RW + (i >> 5)] >> (i & 31)) & 1u) << 1));
+}
+
+__device__ __forceinline__ int rqual(const ReadsDev& r, int64_t rs, int i) {
+  if (!r.has_qual) return 3;   // alignment.c:151-162: no quality -> level of Q93
+  return (int)((r.qual[rs * QW + (i >> 4)] >> ((i & 15) << 1)) & 3u);
+}
+
+}  // namespace storm

@@ -1,0 +1,137 @@
+// storm_kernels.h -- argument blocks and launchers of the gfx950 kernels (host and device).
+#pragma once
+#include "storm_dev.h"
+
+namespace storm {
+
+// one DP result (alignment__align + alignment__traceback outputs, alignment.h:189-229)
+struct AlignRec {
+  int32_t score;     // best score of the last row (MIN_SCORE if none)
+  int32_t S;         // survival bound: min over rows >= 1 of max_on_line + best_possible
+  int32_t win;       // window start (run.c:667)
+  int16_t best_j, best_i0, best_j0, tb_len;
+};
+
+constexpr int TBB = (MAXL + MAX_I + 2) / 2;   // traceback bytes per record, 2 codes per byte
+
+// one hit map entry (HitInfoType, hit_map.h:17-40); the traceback is (pass, rec)
+struct Slot {
+  int32_t score, ref_id, ref_start, ref_end;
+  int32_t read_start, read_end, sense, pass, rec, tb_len;
+};
+
+struct FilterArgs {
+  GenomeDev g;
+  const uint32_t* rplanes;   // read planes, candidate read index -> rplanes + idx * 2*RW
+  const int32_t* rlen;       // read length, indexed by (read plane index >> len_shift)
+  int len_shift;             // 1 when plane indices are read strands (2*read + strand)
+  const int32_t* cand_read;  // read plane index per candidate
+  const int32_t* cand_pos;   // window start per candidate
+  int64_t n;
+  uint8_t* score;            // optional u8 score per candidate
+  uint8_t* pass;             // optional pass flag per candidate
+  int mt, mm, op, ex, thr;
+};
+
+struct AlignArgs {
+  GenomeDev g;
+  ReadsDev r;
+  ScoreDev sc;
+  const int32_t* rec_rs;     // read-strand index (2*read + strand), global read numbering
+  const int32_t* rec_diag;   // diagonal h
+  int64_t n;                 // records in this launch
+  int64_t base;              // index of the first record (into rec_* and outputs)
+  uint8_t* scratch;          // n * scratch_stride bytes
+  int64_t scratch_stride;
+  AlignRec* out;
+  uint8_t* tb;               // TBB bytes per record, traceback order (end to start)
+};
+
+struct PrepArgs {
+  const uint8_t* codes;      // n * code_stride bytes, 2-bit packed forward codes (codes.h:125-130)
+  int code_stride;
+  const uint8_t* qual;       // n * qual_stride bytes, 2-bit packed levels, or null
+  int qual_stride;
+  const uint8_t* first_qual; // color: level of the primer-side color (reads_data.h:27), or null
+  const int32_t* len;
+  int64_t n;
+  int color;
+  uint32_t* planes;          // out: (2n) * 2RW
+  uint32_t* qplanes;         // out: (2n) * QW
+};
+
+struct SeedArgs {
+  ReadsDev r;
+  SeedSetDev seeds;
+  int I;
+  int sense_mask;            // bit 0 forward, bit 1 reverse
+  int64_t rs_begin;          // first read strand of this launch (2*read + strand)
+  int64_t n_rs;
+  int64_t* seg_count;        // hits (pass 1) then kept candidates (pass 2) per read strand
+  const int64_t* seg_off;    // exclusive scan of the pass-1 counts (n_rs + 1)
+  int32_t* hits;             // raw diagonals of overflow segments at seg_off
+  int32_t* kept;             // kept candidates of each segment at seg_off
+  uint8_t* overflow;         // 1 when the segment did not fit in LDS
+  int32_t* n_overflow;       // count of such segments
+};
+
+struct CompactArgs {
+  const int64_t* seg_off;
+  const int64_t* seg_count;
+  const int64_t* cand_off;   // exclusive scan of seg_count (n_rs + 1)
+  const int32_t* kept;
+  int64_t rs_begin, n_rs;
+  int d;
+  int32_t* cand_rs;
+  int32_t* cand_diag;
+  int32_t* cand_pos;         // diag - d (filter window start, run.c:651)
+};
+
+struct RecArgs {
+  const uint8_t* pass;
+  const int64_t* pass_scan;  // exclusive scan of pass flags (n_cand + 1 entries)
+  const int32_t* cand_rs;
+  const int32_t* cand_diag;
+  int64_t n_cand;
+  const int64_t* cand_off;   // n_rs + 1
+  int64_t n_rs;
+  int32_t* rec_rs;
+  int32_t* rec_diag;
+  int64_t* rec_off;          // n_rs + 1
+};
+
+struct ReplayArgs {
+  const AlignRec* rec;
+  const int64_t* rec_off;    // per read strand of the launch, n_rs + 1 entries
+  const int32_t* len;
+  int64_t read_begin;        // first read of the launch
+  int64_t n_reads;
+  int ref_id;
+  int pass;
+  int t;
+  int sense_mask;
+  Slot* slots;               // 4 per read, global read id
+};
+
+hipError_t launch_filter(const FilterArgs& a, int W, hipStream_t st);
+hipError_t launch_align(const AlignArgs& a, hipStream_t st);
+hipError_t launch_prep_reads(const PrepArgs& a, hipStream_t st);
+hipError_t launch_seed_count(const SeedArgs& a, hipStream_t st);
+hipError_t launch_seed_gather(const SeedArgs& a, hipStream_t st);
+hipError_t launch_unique_segments(const SeedArgs& a, hipStream_t st);
+hipError_t launch_compact_candidates(const CompactArgs& a, hipStream_t st);
+hipError_t launch_make_records(const RecArgs& a, hipStream_t st);
+hipError_t launch_replay(const ReplayArgs& a, hipStream_t st);
+hipError_t launch_slots_init(Slot* slots, int64_t n, hipStream_t st);
+hipError_t launch_mark_referenced(const Slot* slots, int64_t n_slots, int pass, int32_t* out_rec, int32_t* out_slot,
+                                  int32_t* counter, hipStream_t st);
+hipError_t launch_gather_tb(const uint8_t* tb, const int32_t* recs, int n, uint8_t* out, hipStream_t st);
+hipError_t launch_overflow_ranges(const int64_t* seg_off, const uint8_t* overflow, int64_t n_rs, int64_t* begin,
+                                  int64_t* end, hipStream_t st);
+// device-wide helpers (hipcub)
+hipError_t scan_exclusive_i64(const int64_t* in, int64_t* out, int64_t n, void* tmp, size_t& tmp_bytes, hipStream_t st);
+hipError_t scan_exclusive_u8(const uint8_t* in, int64_t* out, int64_t n, void* tmp, size_t& tmp_bytes, hipStream_t st);
+hipError_t segsort_i32(const int32_t* in, int32_t* out, int64_t n_items, int64_t n_seg, const int64_t* begin,
+                       const int64_t* end, void* tmp, size_t& tmp_bytes, hipStream_t st);
+
+}  // namespace storm

@@ -1,0 +1,158 @@
+"""ctypes binding of libstorm_hip.so (include/storm_gpu.h).
+
+The HIP library is required: importing the binding fails loudly when it has not been built,
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libstorm_hip.so")
+
+STORM_MODE_PLAIN = 0
+STORM_MODE_BATCHED = 1
+
+
+class StormError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    """storm_params: the reference's -M mapping options (main.c:188-433)."""
+    _fields_ = [(n, C.c_int) for n in (
+        "color", "match", "mismatch", "gap_open", "gap_extend", "allowed_indels", "t", "z", "qual_base",
+        "stdev_distance", "sense_mask", "max_hits", "uniqueness", "print_all", "ignore_lowercase",
+        "strict_length")] + [("seeds", C.c_char_p)]
+
+
+class MapStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in ("n_reads", "n_mapped", "n_candidates", "n_filter_pass", "n_accepted")] + \
+               [(n, C.c_double) for n in ("seconds_load", "seconds_index", "seconds_map", "seconds_sam",
+                                          "gpu_ms_seed", "gpu_ms_filter", "gpu_ms_align", "gpu_ms_replay")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ReadsView(C.Structure):
+    _fields_ = [("n_reads", C.c_int64), ("names", C.POINTER(C.c_char_p)), ("codes", C.c_void_p),
+                ("code_stride", C.c_int64), ("qual", C.c_void_p), ("qual_stride", C.c_int64),
+                ("len", C.c_void_p), ("first_base", C.c_void_p), ("first_qual", C.c_void_p),
+                ("db_name", C.c_char_p)]
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise StormError(f"{LIB_PATH} is missing: run __graft_entry__.build() (or make -C storm_amd/csrc)")
+    lib = C.CDLL(LIB_PATH)
+    P, I, L, D = C.c_void_p, C.c_int, C.c_int64, C.c_double
+    lib.storm_last_error.restype = C.c_char_p
+    lib.storm_params_default.argtypes = [C.POINTER(Params), I]
+    lib.storm_filter_plan_create.argtypes = [I, P, L, P, L, P, L, P, P, L, I, I, I, I, I, I, C.POINTER(P)]
+    lib.storm_filter_run.argtypes = [P, I, C.POINTER(D)]
+    lib.storm_filter_fetch.argtypes = [P, P, P]
+    lib.storm_filter_plan_destroy.argtypes = [P]
+    lib.storm_map_files.argtypes = [I, C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(Params), I, C.c_char_p, I,
+                                    P, C.POINTER(MapStats)]
+    lib.storm_mapper_create.argtypes = [I, C.c_char_p, C.POINTER(Params), C.POINTER(P)]
+    lib.storm_mapper_map.argtypes = [P, C.POINTER(ReadsView), I, C.c_char_p, I, P, C.POINTER(MapStats)]
+    lib.storm_mapper_map_hits.argtypes = [P, C.POINTER(ReadsView), I, P, P, P, P, C.POINTER(MapStats)]
+    lib.storm_mapper_free_bytes.argtypes = [P]
+    lib.storm_mapper_free_bytes.restype = L
+    lib.storm_mapper_destroy.argtypes = [P]
+    lib.alignment_gfx950__set_genome.argtypes = [P, C.c_size_t, C.c_size_t]
+    for w in ("octa", "quad", "pair", "mono"):
+        getattr(lib, f"alignment_gfx950__init_{w}").argtypes = [C.c_uint] * 6
+        f = getattr(lib, f"alignment_gfx950__align_{w}")
+        f.argtypes = [P, P, P]
+        f.restype = C.c_uint
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        raise StormError(load().storm_last_error().decode())
+
+
+def params(color: int = 0, **kw) -> Params:
+    p = Params()
+    load().storm_params_default(C.byref(p), color)
+    for k, v in kw.items():
+        if k == "seeds":
+            v = v.encode() if isinstance(v, str) else v
+        setattr(p, k, v)
+    return p
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+class FilterPlan:
+    """Batch filter over candidates with all inputs resident in HBM (storm_filter_plan_*)."""
+
+    def __init__(self, genome_codes: np.ndarray, reads_codes, read_len: np.ndarray, cand_read: np.ndarray,
+                 cand_pos: np.ndarray, W: int, match=5, mismatch=-4, gap_open=-9, gap_extend=-4, z=100, device=0):
+        from .synth import pack_codes
+        lib = load()
+        g = pack_codes(np.ascontiguousarray(genome_codes, dtype=np.uint8))
+        read_len = np.ascontiguousarray(read_len, dtype=np.int32)
+        stride = (int(read_len.max()) + 3) // 4 + 8
+        rp = np.zeros(len(read_len) * stride + 64, np.uint8)
+        for k in range(len(read_len)):
+            rp[k * stride:k * stride + (int(read_len[k]) + 3) // 4] = pack_codes(np.asarray(reads_codes[k], np.uint8))
+        self._keep = (g, rp)
+        cand_read = np.ascontiguousarray(cand_read, dtype=np.int32)
+        cand_pos = np.ascontiguousarray(cand_pos, dtype=np.int32)
+        self.n = len(cand_read)
+        h = C.c_void_p()
+        check(lib.storm_filter_plan_create(device, _ptr(g), len(genome_codes), _ptr(rp), stride, _ptr(read_len),
+                                           len(read_len), _ptr(cand_read), _ptr(cand_pos), self.n, W, match, mismatch,
+                                           gap_open, gap_extend, z, C.byref(h)))
+        self.h = h
+
+    def run(self, reps: int = 1) -> float:
+        ms = C.c_double()
+        check(load().storm_filter_run(self.h, reps, C.byref(ms)))
+        return ms.value
+
+    def fetch(self):
+        s = np.zeros(self.n, np.uint8)
+        p = np.zeros(self.n, np.uint8)
+        check(load().storm_filter_fetch(self.h, _ptr(s), _ptr(p)))
+        return s, p
+
+    def close(self):
+        if self.h:
+            load().storm_filter_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def map_files(genome: str, reads: str, sam: str, qual: str | None = None, mode: int = STORM_MODE_BATCHED,
+              device: int = 0, argv=None, **kw) -> dict:
+    """storm_map_files: the reference's whole -M run on the GPU, SAM written to `sam`."""
+    lib = load()
+    p = params(**kw)
+    st = MapStats()
+    argv = argv or ["storm-gpu"]
+    arr = (C.c_char_p * len(argv))(*[a.encode() for a in argv])
+    check(lib.storm_map_files(device, genome.encode(), reads.encode(), qual.encode() if qual else None, C.byref(p),
+                              mode, sam.encode(), len(argv), C.cast(arr, C.c_void_p), C.byref(st)))
+    return st.as_dict()

@@ -1,0 +1,32 @@
+"""The C-ABI library loads and exports every function include/storm_gpu.h declares (no GPU call)."""
+import ctypes as C
+import os
+import re
+
+from conftest import ROOT
+
+
+def test_library_exports_every_declared_symbol():
+    from storm_amd import lib
+    so = lib.load()
+    hdr = open(os.path.join(ROOT, "include", "storm_gpu.h")).read()
+    names = set(re.findall(r"\b((?:storm|alignment_gfx950)_\w+)\s*\(", hdr))
+    assert len(names) > 20
+    for n in sorted(names):
+        assert hasattr(so, n), n
+
+
+def test_default_params_mirror_reference():
+    from storm_amd import lib
+    p = lib.params(0)
+    assert (p.match, p.mismatch, p.gap_open, p.gap_extend) == (5, -4, -9, -4)     # alignment.h:24-27
+    assert (p.allowed_indels, p.t, p.z, p.qual_base) == (3, 115, 100, 33)       # alignment.h:28,51-52
+    assert p.stdev_distance == 100 and p.sense_mask == 3
+
+
+def test_errors_are_returned_not_exited(tmp_path):
+    from storm_amd import lib
+    import pytest
+    with pytest.raises(lib.StormError):
+        lib.map_files(str(tmp_path / "missing.fa"), str(tmp_path / "missing.fq"), str(tmp_path / "o.sam"),
+                      max_hits=0)
