@@ -1,0 +1,271 @@
+"""Benchmark of the SToRM mapping hot path on one MI355X.
+
+Default workload (one "step" = one launch of the banded filter over the whole batch):
+  10M candidate pairs = 5M reads of 75 nt x 2 windows (true origin - 7, uniform random decoy) on a
+  1 Mbp i.i.d. genome, -i 7 (W = 8 lanes, 16 diagonals per the reference's geometry), saturating
+  u8 scores, z = 100.  Inputs are resident in HBM before the timed region.
+Reported: filter GCUPS (cells = candidates x L x 2W), the filter through the per-call drop-in
+entry point (one launch per read, as run.c calls the SIMD table) on a sample, whole -M mapping
+runs in reads/s (plain per-batch and batched), and the reference's OpenMP SIMD filter timed on
+this host's cores as the CPU baseline.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-map] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "mapped reads/s + filter GCUPS, batched vs plain GPU; p50/p99 request latency"
+OPS_PER_CELL = 13            # compute chain of the reference's SIMD cell (alignment_simd.c:3810-3880)
+VALU_PEAK_INT32 = 39.3       # T VALU lane-instructions/s: 256 CU x 64 lanes x 2.4 GHz
+VALU_PEAK_PK16 = 78.6        # T 16-bit ops/s with packed VALU instructions (2 per lane-instruction)
+# VALU lane-instructions k_filter<8> issues per cell at L=75 (SQ_INSTS_VALU x 64 / cells, rocprofv3 --pmc,
+# profiles/k_filter_W8_L75_10Mcand.md); the kernel's own work measure for the VALU-issue roofline
+VALU_INSTR_PER_CELL_W8_L75 = 5.447
+# HBM bytes per candidate of the same dispatch (FETCH_SIZE + WRITE_SIZE, KiB x 1024 / candidates)
+HBM_BYTES_PER_CAND_W8_L75 = 24.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_filter_workload(n_reads: int, L: int, W: int, d: int, seed: int = 2, genome_len: int = 1_000_000):
+    """Packed reads (n_reads x stride bytes) and 2 window starts per read, built in chunks."""
+    from storm_amd import synth
+    g = synth.random_genome(genome_len, 1)
+    rng = synth.rng(seed)
+    stride = (L + 3) // 4 + 8
+    reads = np.zeros((n_reads, stride), np.uint8)
+    pos = np.zeros((n_reads, 2), np.int32)
+    ch = 250_000
+    Lp = ((L + 3) // 4) * 4
+    for s in range(0, n_reads, ch):
+        n = min(ch, n_reads - s)
+        origin = rng.integers(d + 1, genome_len - (L + 2 * W + 2), size=n)
+        idx = origin[:, None] + np.arange(L)[None, :]
+        r = g[idx]
+        m = rng.random(r.shape) < 0.02
+        r = np.where(m, (r + rng.integers(1, 4, size=r.shape, dtype=np.uint8)) & 3, r).astype(np.uint8)
+        rr = np.zeros((n, Lp), np.uint8)
+        rr[:, :L] = r
+        rr = rr.reshape(n, Lp // 4, 4)
+        reads[s:s + n, :Lp // 4] = rr[:, :, 0] | (rr[:, :, 1] << 2) | (rr[:, :, 2] << 4) | (rr[:, :, 3] << 6)
+        pos[s:s + n, 0] = origin - d
+        pos[s:s + n, 1] = rng.integers(0, genome_len - (L + 2 * W + 2), size=n)
+    return g, reads, pos, stride
+
+
+def gpu_filter(lib, g, reads, pos, stride, L, W, steps, warmup):
+    from storm_amd import synth
+    import torch
+    P = C.c_void_p
+    gp = synth.pack_codes(g)
+    n_reads = reads.shape[0]
+    rl = np.full(n_reads, L, np.int32)
+    cr = np.repeat(np.arange(n_reads, dtype=np.int32), 2)
+    cp = np.ascontiguousarray(pos.reshape(-1))
+    h = P()
+    lib.check(lib.load().storm_filter_plan_create(0, P(gp.ctypes.data), len(g), P(reads.ctypes.data), stride,
+                                                  P(rl.ctypes.data), n_reads, P(cr.ctypes.data), P(cp.ctypes.data),
+                                                  len(cr), W, 5, -4, -9, -4, 100, C.byref(h)))
+    so = lib.load()
+    ms = C.c_double()
+    lib.check(so.storm_filter_run(h, warmup, C.byref(ms)))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lib.check(so.storm_filter_run(h, steps, C.byref(ms)))
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    scores = np.zeros(len(cr), np.uint8)
+    passf = np.zeros(len(cr), np.uint8)
+    lib.check(so.storm_filter_fetch(h, P(scores.ctypes.data), P(passf.ctypes.data)))
+    so.storm_filter_plan_destroy(h)
+    return ms.value, wall, scores, passf, cr, cp
+
+
+def percall_filter(lib, g, reads, pos, stride, L, W, n_calls):
+    """The drop-in row: alignment_gfx950__init_pair + __align_pair, one call per read (run.c:660)."""
+    from storm_amd import synth
+    P = C.c_void_p
+    so = lib.load()
+    pad = 1024
+    gp = np.zeros(pad + (len(g) + 3) // 4 + pad, np.uint8)
+    gp[pad:pad + (len(g) + 3) // 4] = synth.pack_codes(g)
+    lib.check(so.alignment_gfx950__set_genome(P(gp.ctypes.data), gp.nbytes, pad))
+    so.alignment_gfx950__init_pair(5, 4, 9, 4, 100, L)
+    bits = np.zeros(n_calls, np.uint32)
+    t0 = time.perf_counter()
+    for k in range(n_calls):
+        win = np.ascontiguousarray(pos[k])
+        bits[k] = so.alignment_gfx950__align_pair(P(gp.ctypes.data + pad), P(win.ctypes.data), P(reads[k].ctypes.data))
+    dt = time.perf_counter() - t0
+    so.alignment_gfx950__clean()
+    return dt, bits
+
+
+def cpu_baseline(g, reads, pos, stride, L, d, n_sample, target_s=10.0):
+    """The reference's SSE2 filter (its default Makefile build) with OpenMP on this host."""
+    from storm_amd import synth
+    path = os.path.join(ROOT, "oracle", "_ref", "libref_filter.so")
+    if not os.path.exists(path):
+        return None
+    lib = C.CDLL(path)
+    P = C.c_void_p
+    lib.ref_filter_time.restype = C.c_double
+    lib.ref_filter_time.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, P, C.c_long, P, P, P, C.c_long, P, C.c_int,
+                                    C.c_int]
+    cores = min(16, os.cpu_count() or 1)
+    pad = 1024
+    gp = np.zeros(pad + (len(g) + 3) // 4 + pad, np.uint8)
+    gp[pad:pad + (len(g) + 3) // 4] = synth.pack_codes(g)
+    n = min(n_sample, reads.shape[0])
+    rd = np.ascontiguousarray(reads[:n])
+    seg = np.arange(0, 2 * n + 1, 2, dtype=np.int64)
+    ps = np.ascontiguousarray(pos[:n].reshape(-1))
+    bits = np.zeros(2 * n, np.uint8)
+    out = {}
+    for isa, name in ((0, "sse2"), (1, "avx2")):
+        t = lib.ref_filter_time(isa, d, L, 100, P(gp.ctypes.data + pad), n, P(seg.ctypes.data), P(ps.ctypes.data),
+                                P(rd.ctypes.data), stride, P(bits.ctypes.data), cores, 1)
+        if t <= 0:
+            continue
+        reps = max(1, int(target_s / max(t, 1e-3)))
+        if name == "avx2":
+            reps = max(1, reps // 2)
+        t = lib.ref_filter_time(isa, d, L, 100, P(gp.ctypes.data + pad), n, P(seg.ctypes.data), P(ps.ctypes.data),
+                                P(rd.ctypes.data), stride, P(bits.ctypes.data), cores, reps)
+        out[name] = (2 * n * reps, t, cores)
+    return out
+
+
+def mapping_runs(lib):
+    """Whole -M runs: 1 Mbp genome, 100k Illumina reads of 50 nt (2 % substitutions, 0.1 % indels,
+    Q20-40), -i 3 -z 100 -t 100 -M 1, reference default seeds."""
+    from storm_amd import synth
+    d = tempfile.mkdtemp(prefix="storm_bench_")
+    g = synth.random_genome(1_000_000, 1)
+    synth.write_fasta(os.path.join(d, "g.fa"), "chr1", g)
+    r = synth.simulate_nucleotide_reads(g, 100_000, 50, 101, sub=0.02, indel=0.001, qlo=20, qhi=40)
+    synth.write_fastq(os.path.join(d, "r.fq"), r)
+    res = {}
+    for mode, name in ((0, "plain"), (1, "batched")):
+        best = None
+        for _ in range(2):
+            st = lib.map_files(os.path.join(d, "g.fa"), os.path.join(d, "r.fq"), os.path.join(d, f"{name}.sam"),
+                               mode=mode, allowed_indels=3, z=100, t=100, max_hits=1)
+            if best is None or st["seconds_map"] < best["seconds_map"]:
+                best = st
+        res[name] = {"reads_per_s": best["n_reads"] / best["seconds_map"], "mapped": best["n_mapped"],
+                     "reads": best["n_reads"], "seconds_map": best["seconds_map"],
+                     "gpu_ms": {k: round(best["gpu_ms_" + k], 3) for k in ("seed", "filter", "align", "replay")}}
+    same = open(os.path.join(d, "plain.sam")).read() == open(os.path.join(d, "batched.sam")).read()
+    res["plain_equals_batched"] = same
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--reads", type=int, default=5_000_000, help="reads of the filter workload (2 windows each)")
+    ap.add_argument("--no-map", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--percall", type=int, default=20000)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the path shards reads across GPUs by design "
+              f"but this build measures one GPU.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from storm_amd import lib
+    assert torch.cuda.is_available()
+    torch.zeros(1, device="cuda:0")
+    lib.load()
+    L, W, d = 75, 8, 7
+    t0 = time.perf_counter()
+    g, reads, pos, stride = make_filter_workload(a.reads, L, W, d)
+    log(f"workload: {a.reads} reads x 2 windows built in {time.perf_counter() - t0:.1f}s")
+    n_cand = 2 * a.reads
+    cells = n_cand * L * 2 * W
+    ms, wall, scores, passf, cr, cp = gpu_filter(lib, g, reads, pos, stride, L, W, a.steps, a.warmup)
+    gcups = cells / (ms * 1e-3) / 1e9
+    log(f"filter: {ms:.3f} ms/launch (events), {wall * 1e3:.3f} ms/step (wall), {gcups:.1f} GCUPS, "
+        f"pass rate {passf.mean():.3f}")
+    # spot check against the CPU restatement (the full parity tests live in tests/)
+    spot = None
+    orc = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+    if os.path.exists(orc):
+        from storm_amd import synth
+        ol = C.CDLL(orc)
+        ol.orc_filter_score.restype = C.c_int
+        ol.orc_filter_score.argtypes = [C.c_void_p, C.c_long, C.c_void_p, C.c_int, C.c_int] + [C.c_int] * 4
+        pad = 64
+        gp = np.zeros(pad + (len(g) + 3) // 4 + pad, np.uint8)
+        gp[pad:pad + (len(g) + 3) // 4] = synth.pack_codes(g)
+        sel = np.linspace(0, n_cand - 1, 2000).astype(np.int64)
+        want = np.array([min(255, ol.orc_filter_score(C.c_void_p(gp.ctypes.data + pad), int(cp[c]),
+                                                      C.c_void_p(reads[int(cr[c])].ctypes.data), L, W, 5, -4, -9, -4))
+                         for c in sel])
+        spot = bool(np.array_equal(scores[sel].astype(np.int64), want))
+        log(f"spot check vs oracle on 2000 candidates: {'identical' if spot else 'MISMATCH'}")
+    pc_dt, _ = percall_filter(lib, g, reads, pos, stride, L, W, min(a.percall, a.reads))
+    pc_gcups = 2 * min(a.percall, a.reads) * L * 2 * W / pc_dt / 1e9
+    log(f"per-call drop-in row: {pc_dt:.2f}s for {min(a.percall, a.reads)} calls, {pc_gcups:.3f} GCUPS")
+    mapping = None if a.no_map else mapping_runs(lib)
+    if mapping:
+        log(f"mapping 100k x 50nt on 1 Mbp: plain {mapping['plain']['reads_per_s']:.0f} reads/s, "
+            f"batched {mapping['batched']['reads_per_s']:.0f} reads/s")
+    cpu = None
+    if not a.no_cpu:
+        cb = cpu_baseline(g, reads, pos, stride, L, d, n_sample=500_000)
+        if cb and "sse2" in cb:
+            nc, t, cores = cb["sse2"]
+            cpu = {"value": round(nc * L * 2 * W / t / 1e9, 3), "unit": "GCUPS", "cores": cores, "kind": "reference",
+                   "sample": f"reference SSE2 filter (alignment_sse2__align_pair, default Makefile flags) over "
+                             f"{nc} candidate evaluations of the same workload (first 500k reads x 2 windows, "
+                             f"repeated), OpenMP, {t:.1f}s"}
+            if "avx2" in cb:
+                nc2, t2, _ = cb["avx2"]
+                cpu["avx2_gcups"] = round(nc2 * L * 2 * W / t2 / 1e9, 3)
+            log(f"cpu baseline: {cpu['value']} GCUPS on {cores} threads")
+    achieved = gcups * VALU_INSTR_PER_CELL_W8_L75 / 1e3          # T lane-instructions/s
+    algorithmic = gcups * OPS_PER_CELL / 1e3                       # T reference-ops/s
+    line = {
+        "metric": METRIC, "value": round(gcups, 2), "unit": "GCUPS", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(wall * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": f"banded u8 filter, {n_cand} candidates ({a.reads} reads x 2 windows: true origin, "
+                               f"random decoy), L={L}, -i 7 (W={W}, {2 * W} diagonals), z=100, 1 Mbp i.i.d. genome",
+                   "cells_per_step": cells, "kernel": "k_filter<8>"},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": VALU_PEAK_INT32,
+                     "unit": "T VALU lane-instr/s", "frac": round(achieved / VALU_PEAK_INT32, 4),
+                     "traffic": int(HBM_BYTES_PER_CAND_W8_L75 * n_cand),
+                     "work_per_cell": VALU_INSTR_PER_CELL_W8_L75,
+                     "algorithmic": {"ops_per_cell": OPS_PER_CELL, "achieved_tops": round(algorithmic, 2),
+                                     "peak_packed16_tops": VALU_PEAK_PK16,
+                                     "frac": round(algorithmic / VALU_PEAK_PK16, 4)}},
+        "cpu_baseline": cpu,
+        "kernel_ms": round(ms, 4),
+        "percall_gcups": round(pc_gcups, 4),
+        "batched_over_percall": round(gcups / pc_gcups, 1),
+        "spot_check_identical": spot,
+        "mapping": mapping,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()
