@@ -1,0 +1,109 @@
+"""The CPU restatement against the reference binaries built from the reference's own sources
+(oracle/_ref, SSE2 default build) on freshly generated inputs covering the options of the -M path.
+Skipped where oracle/_ref has not been built."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, sam_body
+
+REF = os.path.join(ROOT, "oracle", "_ref")
+SEEDS_N = "####--#-#--##--####;#####--###-#-###;###-##-#######"
+SEEDS_C = "####----##--##-####;####-###--#----####;####-####-####"
+
+pytestmark = pytest.mark.skipif(not os.path.exists(os.path.join(REF, "storm-nucleotide")),
+                                reason="reference binaries not built (oracle/_ref)")
+
+
+def _genome(path, kind, seed):
+    from storm_amd import synth
+    g = synth.rng(seed)
+    if kind == "multi":
+        seqs = [synth.random_genome(n, seed + k) for k, n in enumerate([20000, 5000, 30000])]
+    elif kind == "repeat":
+        seqs = [synth.repeat_genome(80000, seed, 0.2, 0.15)]
+    else:
+        seqs = [synth.random_genome(40000, seed)]
+    with open(path, "wb") as f:
+        for k, s in enumerate(seqs):
+            letters = bytearray(synth.BASES[s].tobytes())
+            if kind in ("multi", "plain"):
+                for _ in range(5):
+                    a = int(g.integers(100, len(letters) - 300))
+                    b = a + int(g.integers(1, 200))
+                    letters[a:b] = b"N" * (b - a)
+            if kind == "multi":
+                a = int(g.integers(0, len(letters) - 500))
+                letters[a:a + 400] = letters[a:a + 400].lower()
+            f.write(b">seq%d desc\n" % k)
+            for s0 in range(0, len(letters), 60):
+                f.write(bytes(letters[s0:s0 + 60]) + b"\n")
+    return seqs[-1]
+
+
+CASES = [
+    ("n", "multi", 1200, 48, 3, ["-M", "4"], 11),
+    ("n", "repeat", 1200, 60, 7, ["-M", "4", "-l", "3"], 12),
+    ("n", "repeat", 1200, 60, 15, ["-M", "4", "-z", "80"], 13),
+    ("n", "plain", 800, 64, 20, ["-M", "2", "-z", "60", "-t", "90"], 14),
+    ("n", "plain", 800, 50, 5, ["-M", "3", "-u", "5", "-A"], 15),
+    ("n", "plain", 800, 50, 2, ["-M", "1", "-F"], 16),
+    ("n", "plain", 800, 50, 4, ["-M", "1", "-R", "-m", "4", "-x", "-3", "-d", "-7", "-e", "-2"], 17),
+    ("n", "plain", 800, 40, 3, ["-M", "2", "-s", "###-##--#-##-#-###"], 18),
+    ("n", "plain", 600, 45, 3, ["-M", "4", "-s", "#####-####:0 3 9 20;##-####-###"], 19),
+    ("c", "multi", 1200, 48, 3, ["-M", "4", "-z", "85"], 21),
+    ("c", "repeat", 1200, 60, 7, ["-M", "4", "-l", "3"], 22),
+    ("c", "plain", 800, 64, 12, ["-M", "2", "-z", "70", "-t", "90"], 23),
+    ("c", "plain", 800, 50, 3, ["-M", "3", "-u", "4", "-A"], 24),
+    ("c", "plain", 800, 35, 2, ["-M", "1"], 25),
+]
+
+
+def make_inputs(tmp, mode, kind, nreads, L, seed, fasta_reads=False):
+    from storm_amd import synth
+    gpath = os.path.join(tmp, "g.fa")
+    full = _genome(gpath, kind, seed)
+    if mode == "n":
+        r = synth.simulate_nucleotide_reads(full, nreads, L, seed + 7, sub=0.03, indel=0.01)
+        if fasta_reads:
+            path = os.path.join(tmp, "r.fa")
+            with open(path, "wb") as f:
+                for k, s in enumerate(r.seqs):
+                    f.write(b">q%d\n" % k + synth.BASES[s].tobytes() + b"\n")
+            return gpath, ["-r", path]
+        path = os.path.join(tmp, "r.fq")
+        synth.write_fastq(path, r)
+        return gpath, ["-r", path]
+    r = synth.simulate_color_reads(full, nreads, L, seed + 7, snp=0.02, color_err=0.04)
+    path, qpath = os.path.join(tmp, "r.csfasta"), os.path.join(tmp, "r.qual")
+    synth.write_csfasta(path, None if fasta_reads else qpath, r)
+    return gpath, ["-r", path] + ([] if fasta_reads else ["-q", qpath])
+
+
+def run_both(tmp, mode, gpath, rargs, args):
+    ref = os.path.join(REF, "storm-nucleotide" if mode == "n" else "storm-color")
+    orc = os.path.join(ROOT, "oracle", "_build", "oracle-nucleotide" if mode == "n" else "oracle-color")
+    if "-s" not in args:
+        args = args + ["-s", SEEDS_N if mode == "n" else SEEDS_C]
+    common = ["-g", gpath] + rargs + args
+    subprocess.run([ref] + common + ["-o", os.path.join(tmp, "ref.sam"), "-v", "0"], check=True,
+                   env=dict(os.environ, MALLOC_PERTURB_="255"), stderr=subprocess.DEVNULL)
+    subprocess.run([orc] + common + ["-o", os.path.join(tmp, "orc.sam")], check=True, stderr=subprocess.DEVNULL)
+    return sam_body(os.path.join(tmp, "ref.sam")), sam_body(os.path.join(tmp, "orc.sam"))
+
+
+@pytest.mark.parametrize("mode,kind,nreads,L,i,extra,seed", CASES, ids=[f"{c[0]}-{c[1]}-i{c[4]}-{c[6]}" for c in CASES])
+def test_oracle_equals_reference_binary(mode, kind, nreads, L, i, extra, seed, tmp_path, oracle_built):
+    gpath, rargs = make_inputs(str(tmp_path), mode, kind, nreads, L, seed)
+    a, b = run_both(str(tmp_path), mode, gpath, rargs, ["-i", str(i)] + extra)
+    assert len(a) > 5
+    assert a == b
+
+
+@pytest.mark.parametrize("mode", ["n", "c"])
+def test_oracle_equals_reference_binary_without_qualities(mode, tmp_path, oracle_built):
+    gpath, rargs = make_inputs(str(tmp_path), mode, "plain", 600, 50, 31, fasta_reads=True)
+    a, b = run_both(str(tmp_path), mode, gpath, rargs, ["-i", "3", "-M", "2"])
+    assert a == b
