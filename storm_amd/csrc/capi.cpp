@@ -144,6 +144,40 @@ struct PassScratch {   // reused across passes
   DevMem mark_rec, mark_slot, mark_count, gath;
 };
 
+// reads resident on the device for one map call (global read ids = device order)
+struct DevReads {
+  DevMem codes, qual, first_qual, len, planes, qplanes;
+  int64_t n = 0;
+  int has_qual = 0;
+  ReadsDev view() const {
+    return ReadsDev{planes.as<uint32_t>(), qplanes.as<uint32_t>(), len.as<int32_t>(), n, has_qual};
+  }
+};
+
+// batched path: buffers sized to the largest batch seen, one captured graph per reference
+struct FusedTune {
+  int group = 16;
+  int grid = 0;
+  float ms = 0;
+};
+
+struct FusedState {
+  DevReads reads;
+  DevMem slots, cells, overflow, n_dev, counter, stats, scratch, tbbuf, gids, gtb, gcount;
+  int64_t cap_reads = 0, scratch_stride = 0;
+  int grid = 0;
+  std::vector<hipGraphExec_t> graphs;   // per reference (nullptr = not captured)
+  std::vector<std::vector<const void*>> graph_sig;   // buffers and settings a graph was captured with
+  std::map<int64_t, FusedTune> tune;    // key: (W, I, color, length bucket)
+  bool tuned_any = false;
+  ~FusedState() {
+    for (auto g : graphs) if (g) (void)hipGraphExecDestroy(g);
+  }
+  void drop_graphs() {
+    for (auto& g : graphs) { if (g) (void)hipGraphExecDestroy(g); g = nullptr; }
+  }
+};
+
 }  // namespace storm
 
 struct storm_mapper {
@@ -157,6 +191,7 @@ struct storm_mapper {
   storm::PassScratch ps;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double index_seconds = 0;
+  storm::FusedState fs;
 };
 
 namespace storm {
@@ -210,16 +245,6 @@ static SeedSetDev seed_set(const RefState& rs) {
   }
   return s;
 }
-
-// reads resident on the device for one map call (global read ids = device order)
-struct DevReads {
-  DevMem codes, qual, first_qual, len, planes, qplanes;
-  int64_t n = 0;
-  int has_qual = 0;
-  ReadsDev view() const {
-    return ReadsDev{planes.as<uint32_t>(), qplanes.as<uint32_t>(), len.as<int32_t>(), n, has_qual};
-  }
-};
 
 static int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order, DevReads& d) {
   const int64_t n = v->n_reads;
@@ -475,8 +500,8 @@ static int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int 
   HIPOK(P.mark_slot.ensure(rb * 4));
   HIPOK(P.mark_count.ensure(4));
   HIPOK(hipMemsetAsync(P.mark_count.p, 0, 4, st));
-  HIPOK(launch_mark_referenced(slots + 4 * r0, 4 * (r1 - r0), pass_id, P.mark_rec.as<int32_t>(), P.mark_slot.as<int32_t>(),
-                               P.mark_count.as<int32_t>(), st));
+  HIPOK(launch_mark_referenced(slots + 4 * r0, 4 * (r1 - r0), pass_id, m->p.max_hits, P.mark_rec.as<int32_t>(),
+                               P.mark_slot.as<int32_t>(), P.mark_count.as<int32_t>(), st));
   (void)n_slots;
   int32_t n_mark = 0;
   HIPOK(hipMemcpyAsync(&n_mark, P.mark_count.p, 4, hipMemcpyDeviceToHost, st));
@@ -504,8 +529,45 @@ static int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int 
 }
 
 // whole map call: reads of the view in the given order; results per read in view order
-static int map_view(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
-                    storm_map_stats* st_out) {
+
+// Host slots in view order from the device slots (device order).  Tracebacks of ranks < -M come
+// from the staged path's store (pass >= 0) or the fused path's gathered cells (PASS_FUSED).
+static int assemble_slots(storm_mapper* m, const std::vector<Slot>& hs, const std::vector<int64_t>& order,
+                          const TbStore& store, const std::unordered_map<int64_t, const uint8_t*>* cells,
+                          std::vector<HostSlot>& out_slots, int64_t& mapped) {
+  const int64_t n = (int64_t)order.size();
+  out_slots.assign((size_t)(4 * n), HostSlot{});
+  mapped = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t j = order[(size_t)i];
+    if (hs[(size_t)(4 * i)].ref_id != -1) mapped++;
+    for (int k = 0; k < 4; ++k) {
+      const Slot& s = hs[(size_t)(4 * i + k)];
+      HostSlot& h = out_slots[(size_t)(4 * j + k)];
+      h.score = s.score; h.ref_id = s.ref_id; h.ref_start = s.ref_start; h.ref_end = s.ref_end;
+      h.read_start = s.read_start; h.read_end = s.read_end; h.sense = s.sense;
+      h.tb_len = s.tb_len;
+      if (s.ref_id == -1 || k >= m->p.max_hits) continue;
+      if (s.pass == PASS_FUSED) {
+        if (!cells) return fail(STORM_ERR_DEVICE, "traceback of a fused hit was not gathered");
+        auto it = cells->find(4 * i + k);
+        if (it == cells->end()) return fail(STORM_ERR_DEVICE, "traceback of a fused hit was not gathered");
+        h.tb.assign(it->second, it->second + TBB);
+      } else {
+        auto it = store.find(((int64_t)s.pass << 32) | (uint32_t)s.rec);
+        if (it == store.end()) return fail(STORM_ERR_DEVICE, "traceback of a hit was not gathered");
+        h.tb = it->second;
+      }
+    }
+  }
+  return STORM_OK;
+}
+
+static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
+                           storm_map_stats* st_out);
+
+static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
+                           storm_map_stats* st_out) {
   const int64_t n = v->n_reads;
   if (n <= 0) return fail(STORM_ERR_INVALID, "no reads");
   // plain mode: equal-length batches (the reference maps one read length per run); the device
@@ -548,24 +610,9 @@ static int map_view(storm_mapper* m, const storm_reads_view* v, int mode, std::v
   std::vector<Slot> hs((size_t)(4 * n));
   HIPOK(hipMemcpyAsync(hs.data(), slots.p, hs.size() * sizeof(Slot), hipMemcpyDeviceToHost, m->st));
   HIPOK(hipStreamSynchronize(m->st));
-  out_slots.assign((size_t)(4 * n), HostSlot{});
   int64_t mapped = 0;
-  for (int64_t i = 0; i < n; ++i) {
-    const int64_t j = order[(size_t)i];
-    if (hs[(size_t)(4 * i)].ref_id != -1) mapped++;
-    for (int k = 0; k < 4; ++k) {
-      const Slot& s = hs[(size_t)(4 * i + k)];
-      HostSlot& h = out_slots[(size_t)(4 * j + k)];
-      h.score = s.score; h.ref_id = s.ref_id; h.ref_start = s.ref_start; h.ref_end = s.ref_end;
-      h.read_start = s.read_start; h.read_end = s.read_end; h.sense = s.sense;
-      h.tb_len = s.tb_len;
-      if (s.ref_id != -1) {
-        auto it = store.find(((int64_t)s.pass << 32) | (uint32_t)s.rec);
-        if (it == store.end()) return fail(STORM_ERR_DEVICE, "traceback of a hit was not gathered");
-        h.tb = it->second;
-      }
-    }
-  }
+  rc = assemble_slots(m, hs, order, store, nullptr, out_slots, mapped);
+  if (rc) return rc;
   if (st_out) {
     st_out->n_reads = n;
     st_out->n_mapped = mapped;
@@ -582,7 +629,277 @@ static int map_view(storm_mapper* m, const storm_reads_view* v, int mode, std::v
   return STORM_OK;
 }
 
+// ------------------------------------------------------------------------------ batched path
+static int device_cus(int device) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 256;
+  return prop.multiProcessorCount;
+}
+
+static int ensure_fused_buffers(storm_mapper* m, int64_t n, int64_t stride, int grid) {
+  FusedState& F = m->fs;
+  if (n > F.cap_reads) {
+    const int64_t cap = std::max<int64_t>(n, F.cap_reads);
+    HIPOK(F.slots.ensure((size_t)(4 * cap) * sizeof(Slot)));
+    HIPOK(F.cells.ensure((size_t)(4 * cap) * TBB));
+    HIPOK(F.overflow.ensure((size_t)cap));
+    HIPOK(F.gids.ensure((size_t)(4 * cap) * 4));
+    HIPOK(F.gtb.ensure((size_t)(4 * cap) * TBB));
+    F.cap_reads = cap;
+  }
+  if (stride > F.scratch_stride || grid > F.grid) {
+    const int64_t st = std::max(stride, F.scratch_stride);
+    const int g = std::max(grid, F.grid);
+    HIPOK(F.scratch.ensure((size_t)g * 64 * st));
+    HIPOK(F.tbbuf.ensure((size_t)g * FUSED_PCAP * TBB));
+    F.scratch_stride = st;
+    F.grid = g;
+  }
+  HIPOK(F.n_dev.ensure(16));
+  HIPOK(F.counter.ensure(16));
+  HIPOK(F.stats.ensure(64));
+  HIPOK(F.gcount.ensure(16));
+  return STORM_OK;
+}
+
+static FusedArgs fused_args(storm_mapper* m, size_t ref, int group) {
+  FusedState& F = m->fs;
+  FusedArgs a{};
+  a.g = m->rstate[ref]->g.view();
+  a.r = F.reads.view();
+  a.seeds = seed_set(*m->rstate[ref]);
+  fill_score(m->p, a.sc);
+  a.n_reads = F.n_dev.as<int32_t>();
+  a.group = group;
+  a.task_counter = F.counter.as<int32_t>();
+  a.slots = F.slots.as<Slot>();
+  a.cells = F.cells.as<uint8_t>();
+  a.overflow = F.overflow.as<uint8_t>();
+  a.scratch = F.scratch.as<uint8_t>();
+  a.scratch_stride = F.scratch_stride;
+  a.tbbuf = F.tbbuf.as<uint8_t>();
+  a.ref_id = (int)ref;
+  a.sense_mask = m->p.sense_mask;
+  a.stats = F.stats.as<unsigned long long>();
+  return a;
+}
+
+// Autotuner: reads per task and persistent grid for this band class and length bucket, timed on
+// a sample of the batch (the sample's slots are reset afterwards).
+static int tune_fused(storm_mapper* m, int64_t n, int maxL, int max_grid, FusedTune& out) {
+  FusedState& F = m->fs;
+  ScoreDev sc{};
+  fill_score(m->p, sc);
+  const int64_t key = ((int64_t)sc.W << 40) | ((int64_t)sc.I << 32) | ((int64_t)m->p.color << 24) | (int64_t)((maxL + 31) / 32);
+  auto it = F.tune.find(key);
+  if (it != F.tune.end()) { out = it->second; return STORM_OK; }
+  const int64_t sample = std::min<int64_t>(n, 32768);
+  const int32_t ns = (int32_t)sample;
+  HIPOK(hipMemcpyAsync(F.n_dev.p, &ns, 4, hipMemcpyHostToDevice, m->st));
+  hipEvent_t e0, e1;
+  HIPOK(hipEventCreate(&e0));
+  HIPOK(hipEventCreate(&e1));
+  FusedTune best;
+  best.ms = 1e30f;
+  {   // warm-up launch (code object load, caches)
+    FusedArgs a = fused_args(m, 0, 16);
+    HIPOK(launch_fused_reset(a.task_counter, a.stats, m->st));
+    HIPOK(launch_fused(a, max_grid, m->st));
+    HIPOK(hipStreamSynchronize(m->st));
+  }
+  const int groups[] = {8, 16, 32, 64};
+  const int grids[] = {max_grid, max_grid / 2};
+  for (int gi = 0; gi < 2; ++gi) {
+    for (int G : groups) {
+      const int grid = std::max(1, grids[gi]);
+      FusedArgs a = fused_args(m, 0, G);
+      HIPOK(launch_slots_init(F.slots.as<Slot>(), 4 * sample, m->st));
+      HIPOK(launch_fused_reset(a.task_counter, a.stats, m->st));
+      HIPOK(hipEventRecord(e0, m->st));
+      HIPOK(launch_fused(a, grid, m->st));
+      HIPOK(hipEventRecord(e1, m->st));
+      HIPOK(hipEventSynchronize(e1));
+      float ms = 0;
+      HIPOK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best.ms) { best.ms = ms; best.group = G; best.grid = grid; }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPOK(launch_slots_init(F.slots.as<Slot>(), 4 * sample, m->st));
+  F.tune[key] = best;
+  out = best;
+  return STORM_OK;
+}
+
+// Reads flagged by the fused kernel (LDS capacity exceeded) go through the staged kernels for
+// this reference; their slots are gathered into a sub-batch and scattered back.
+static int staged_subset(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order,
+                         const std::vector<int64_t>& dev_ids, int ref, int& pass_id, TbStore& store, PassStats& ps) {
+  FusedState& F = m->fs;
+  const int64_t k = (int64_t)dev_ids.size();
+  std::vector<int64_t> sub((size_t)k);
+  for (int64_t i = 0; i < k; ++i) sub[(size_t)i] = order[(size_t)dev_ids[(size_t)i]];
+  DevReads dr;
+  int rc = upload_reads(m, v, sub, dr);
+  if (rc) return rc;
+  std::vector<Slot> hs((size_t)(4 * k));
+  for (int64_t i = 0; i < k; ++i)
+    HIPOK(hipMemcpyAsync(&hs[(size_t)(4 * i)], F.slots.as<Slot>() + 4 * dev_ids[(size_t)i], 4 * sizeof(Slot),
+                         hipMemcpyDeviceToHost, m->st));
+  HIPOK(hipStreamSynchronize(m->st));
+  DevMem sl;
+  HIPOK(sl.ensure(hs.size() * sizeof(Slot)));
+  HIPOK(hipMemcpyAsync(sl.p, hs.data(), hs.size() * sizeof(Slot), hipMemcpyHostToDevice, m->st));
+  rc = run_pass(m, dr, sl.as<Slot>(), ref, pass_id++, 0, k, store, ps);
+  if (rc) return rc;
+  HIPOK(hipMemcpyAsync(hs.data(), sl.p, hs.size() * sizeof(Slot), hipMemcpyDeviceToHost, m->st));
+  HIPOK(hipStreamSynchronize(m->st));
+  for (int64_t i = 0; i < k; ++i)
+    HIPOK(hipMemcpyAsync(F.slots.as<Slot>() + 4 * dev_ids[(size_t)i], &hs[(size_t)(4 * i)], 4 * sizeof(Slot),
+                         hipMemcpyHostToDevice, m->st));
+  HIPOK(hipStreamSynchronize(m->st));
+  return STORM_OK;
+}
+
+static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vector<HostSlot>& out_slots,
+                            storm_map_stats* st_out) {
+  FusedState& F = m->fs;
+  const int64_t n = v->n_reads;
+  if (n <= 0) return fail(STORM_ERR_INVALID, "no reads");
+  if (n > (int64_t)INT32_MAX / 4) return fail(STORM_ERR_INVALID, "batch too large");
+  // device order: by length (tasks then hold reads of similar length)
+  std::vector<int64_t> order((size_t)n);
+  for (int64_t i = 0; i < n; ++i) order[(size_t)i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return v->len[a] < v->len[b]; });
+  int maxL = 0;
+  for (int64_t i = 0; i < n; ++i) maxL = std::max(maxL, (int)v->len[i]);
+  const double t0 = now_s();
+  const double ti = m->index_seconds;
+  int rc = upload_reads(m, v, order, F.reads);
+  if (rc) return rc;
+  ScoreDev sc{};
+  fill_score(m->p, sc);
+  const int64_t stride = (((int64_t)maxL * (2 * sc.I + 1)) + 15) & ~15LL;
+  const int bpc = fused_blocks_per_cu(sc.W, sc.I, m->p.color);
+  if (bpc <= 0) return fail(STORM_ERR_DEVICE, "fused kernel: no occupancy for this band class");
+  const int max_grid = device_cus(m->device) * bpc;
+  rc = ensure_fused_buffers(m, n, stride, max_grid);
+  if (rc) return rc;
+  HIPOK(launch_slots_init(F.slots.as<Slot>(), 4 * n, m->st));
+  const int32_t n32 = (int32_t)n;
+  if (F.graphs.size() != m->refs.size()) {
+    F.drop_graphs();
+    F.graphs.assign(m->refs.size(), nullptr);
+    F.graph_sig.assign(m->refs.size(), {});
+  }
+  TbStore store;
+  PassStats ps;
+  int pass_id = 0;
+  unsigned long long totals[3] = {0, 0, 0};
+  std::vector<uint8_t> ovf((size_t)n);
+  for (size_t r = 0; r < m->refs.size(); ++r) {
+    rc = ensure_ref_resident(m, r);
+    if (rc) return rc;
+    FusedTune tune;
+    if (r == 0) {
+      rc = tune_fused(m, n, maxL, max_grid, tune);
+      if (rc) return rc;
+    } else {
+      tune_fused(m, n, maxL, max_grid, tune);
+    }
+    HIPOK(hipMemcpyAsync(F.n_dev.p, &n32, 4, hipMemcpyHostToDevice, m->st));
+    // one graph per reference: reset + fused kernel, replayed for every batch with the same buffers
+    FusedArgs a = fused_args(m, r, tune.group);
+    std::vector<const void*> sig = {a.g.lo, a.r.planes, a.r.qual, a.r.len, a.slots, a.cells, a.scratch, a.tbbuf,
+                                    a.seeds.s[0].first, (const void*)(intptr_t)tune.group, (const void*)(intptr_t)tune.grid,
+                                    (const void*)(intptr_t)a.scratch_stride};
+    if (!F.graphs[r] || F.graph_sig[r] != sig) {
+      if (F.graphs[r]) { (void)hipGraphExecDestroy(F.graphs[r]); F.graphs[r] = nullptr; }
+      hipGraph_t g;
+      HIPOK(hipStreamBeginCapture(m->st, hipStreamCaptureModeThreadLocal));
+      hipError_t e1 = launch_fused_reset(a.task_counter, a.stats, m->st);
+      hipError_t e2 = launch_fused(a, tune.grid, m->st);
+      HIPOK(hipStreamEndCapture(m->st, &g));
+      HIPOK(e1);
+      HIPOK(e2);
+      hipGraphExec_t ex;
+      HIPOK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(g);
+      F.graphs[r] = ex;
+      F.graph_sig[r] = sig;
+    }
+    hipEvent_t e0, e1;
+    HIPOK(hipEventCreate(&e0));
+    HIPOK(hipEventCreate(&e1));
+    HIPOK(hipEventRecord(e0, m->st));
+    HIPOK(hipGraphLaunch(F.graphs[r], m->st));
+    HIPOK(hipEventRecord(e1, m->st));
+    unsigned long long st3[3];
+    HIPOK(hipMemcpyAsync(st3, F.stats.p, sizeof st3, hipMemcpyDeviceToHost, m->st));
+    HIPOK(hipMemcpyAsync(ovf.data(), F.overflow.p, (size_t)n, hipMemcpyDeviceToHost, m->st));
+    HIPOK(hipStreamSynchronize(m->st));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ps.ms_filter += ms;   // the fused kernel carries every stage
+    for (int q = 0; q < 3; ++q) totals[q] += st3[q];
+    std::vector<int64_t> over;
+    for (int64_t i = 0; i < n; ++i) if (ovf[(size_t)i]) over.push_back(i);
+    if (!over.empty()) {
+      rc = staged_subset(m, v, order, over, (int)r, pass_id, store, ps);
+      if (rc) return rc;
+    }
+    if (m->refs.size() > 1) { release_ref(m, r); if (F.graphs[r]) { (void)hipGraphExecDestroy(F.graphs[r]); F.graphs[r] = nullptr; } }
+  }
+  // tracebacks of the fused slots that will be written (ranks < -M)
+  HIPOK(launch_fused_reset(F.gcount.as<int32_t>(), nullptr, m->st));
+  HIPOK(launch_gather_cells(F.slots.as<Slot>(), 4 * n, m->p.max_hits, F.cells.as<uint8_t>(), F.gids.as<int32_t>(),
+                            F.gtb.as<uint8_t>(), F.gcount.as<int32_t>(), m->st));
+  int32_t ng = 0;
+  std::vector<Slot> hs((size_t)(4 * n));
+  HIPOK(hipMemcpyAsync(hs.data(), F.slots.p, hs.size() * sizeof(Slot), hipMemcpyDeviceToHost, m->st));
+  HIPOK(hipMemcpyAsync(&ng, F.gcount.p, 4, hipMemcpyDeviceToHost, m->st));
+  HIPOK(hipStreamSynchronize(m->st));
+  std::vector<int32_t> gids((size_t)ng);
+  std::vector<uint8_t> gtb((size_t)ng * TBB);
+  if (ng > 0) {
+    HIPOK(hipMemcpyAsync(gids.data(), F.gids.p, gids.size() * 4, hipMemcpyDeviceToHost, m->st));
+    HIPOK(hipMemcpyAsync(gtb.data(), F.gtb.p, gtb.size(), hipMemcpyDeviceToHost, m->st));
+    HIPOK(hipStreamSynchronize(m->st));
+  }
+  std::unordered_map<int64_t, const uint8_t*> cells;
+  cells.reserve((size_t)ng * 2 + 1);
+  for (int32_t q = 0; q < ng; ++q) cells[gids[(size_t)q]] = &gtb[(size_t)q * TBB];
+  int64_t mapped = 0;
+  rc = assemble_slots(m, hs, order, store, &cells, out_slots, mapped);
+  if (rc) return rc;
+  if (st_out) {
+    st_out->n_reads = n;
+    st_out->n_mapped = mapped;
+    st_out->n_candidates = (int64_t)totals[0] + ps.cand;
+    st_out->n_filter_pass = (int64_t)totals[1] + ps.rec;
+    st_out->n_accepted = (int64_t)totals[2];
+    st_out->seconds_index = m->index_seconds - ti;
+    st_out->seconds_map = now_s() - t0 - st_out->seconds_index;
+    st_out->gpu_ms_seed = ps.ms_seed;
+    st_out->gpu_ms_filter = ps.ms_filter;
+    st_out->gpu_ms_align = ps.ms_align;
+    st_out->gpu_ms_replay = ps.ms_replay;
+  }
+  return STORM_OK;
+}
+
+static int map_view(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
+                    storm_map_stats* st_out) {
+  if (mode == STORM_MODE_BATCHED) return map_view_batched(m, v, out_slots, st_out);
+  return map_view_staged(m, v, mode, out_slots, st_out);
+}
+
+
 }  // namespace storm
+
 
 // =============================================================================== C ABI
 using namespace storm;

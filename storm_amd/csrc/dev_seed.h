@@ -1,0 +1,111 @@
+// dev_seed.h -- spaced-seed keys, CSR bucket gathers and the candidate keep rule for one read
+// strand held by one wavefront (used by k_fused; k_seed.hip holds the staged variant).  The
+// functions synchronise with __syncthreads(): they are called from one-wavefront workgroups.
+//
+// Semantics: run.c:570-698 (heap merge of the per-(seed, position) hit lists), seed.c:203-214
+// (keys), index.c:212-241 (buckets): the candidates of a read strand are the distinct diagonals
+// q - rp in ascending order, the smallest one dropped iff it is <= -I.
+#pragma once
+#include "storm_dev.h"
+
+namespace storm {
+
+__device__ __forceinline__ int seed_key(const SeedDev& s, const uint32_t* pl, int rp) {
+  int key = 0;
+  for (int k = 0; k < s.weight; ++k) key = (key << 2) | rcode(pl, rp + s.offs[k]);
+  return key;
+}
+
+__device__ __forceinline__ int seed_npos(const SeedDev& s, int L) {
+  if (s.n_pos > 0) return s.n_pos;
+  const int n = L - s.span + 1;
+  return n > 0 ? n : 0;
+}
+
+// wave-wide inclusive scan (64 lanes)
+__device__ __forceinline__ int wave_scan_incl_i32(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int n = __shfl_up(v, o, 64);
+    if (lane >= o) v += n;
+  }
+  return v;
+}
+
+// bitonic sort of buf[0..n) in place (one wavefront, buf in LDS, padded to a power of two)
+__device__ __forceinline__ void wave_bitonic_sort(int32_t* buf, int n) {
+  const int lane = threadIdx.x & 63;
+  int np2 = 1;
+  while (np2 < n) np2 <<= 1;
+  for (int e = n + lane; e < np2; e += 64) buf[e] = 0x7fffffff;
+  __syncthreads();
+  for (int kk = 2; kk <= np2; kk <<= 1) {
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      for (int e = lane; e < np2; e += 64) {
+        const int x = e ^ j;
+        if (x > e) {
+          const int32_t u = buf[e], v = buf[x];
+          const bool up = (e & kk) == 0;
+          if ((u > v) == up) { buf[e] = v; buf[x] = u; }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// All candidates of read strand rs, ascending, into out[0..), one wavefront.  hbuf is an LDS
+// buffer of `hcap` entries (hcap >= 64, power of two).  Returns the number of candidates, or -1
+// when the raw hits exceed hcap or the candidates exceed `cap` (the caller sends the read to the
+// staged path, which has no such limits).
+__device__ int wave_candidates(const ReadsDev& r, const SeedSetDev& seeds, int64_t rs, int I, int32_t* hbuf, int hcap,
+                               int32_t* out, int cap) {
+  const int lane = threadIdx.x & 63;
+  const int L = r.len[rs >> 1];
+  const uint32_t* pl = read_planes(r, rs);
+  int run = 0;
+  bool over = false;
+  for (int s = 0; s < seeds.n; ++s) {
+    const SeedDev& sd = seeds.s[s];
+    const int np = seed_npos(sd, L);
+    for (int k0 = 0; k0 < np; k0 += 64) {
+      const int k = k0 + lane;
+      int rp = 0, f0 = 0, c = 0;
+      if (k < np) {
+        rp = sd.n_pos > 0 ? sd.pos[k] : k;
+        const int key = seed_key(sd, pl, rp);
+        f0 = sd.first[key];
+        c = sd.first[key + 1] - f0;
+      }
+      const int incl = wave_scan_incl_i32(c);
+      const int off = run + incl - c;
+      if (off + c <= hcap) {
+        for (int e = 0; e < c; ++e) hbuf[off + e] = sd.map[f0 + e] - rp;
+      }
+      run += __shfl(incl, 63, 64);
+      if (run > hcap) over = true;
+    }
+  }
+  if (over) return -1;
+  if (run == 0) return 0;
+  __syncthreads();
+  wave_bitonic_sort(hbuf, run);
+  int placed = 0;
+  for (int e0 = 0; e0 < run; e0 += 64) {
+    const int e = e0 + lane;
+    int keep = 0;
+    int32_t v = 0;
+    if (e < run) {
+      v = hbuf[e];
+      keep = (e == 0) ? (v > -I) : (v > hbuf[e - 1]);
+    }
+    const int incl = wave_scan_incl_i32(keep);
+    if (keep && placed + incl - 1 < cap) out[placed + incl - 1] = v;
+    placed += __shfl(incl, 63, 64);
+  }
+  __syncthreads();
+  return placed > cap ? -1 : placed;
+}
+
+}  // namespace storm

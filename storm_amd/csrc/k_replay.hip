@@ -46,8 +46,10 @@ __global__ __launch_bounds__(128) void k_replay(ReplayArgs a) {
         int i = k;
         while (i >= 0 && score > s[i].score) --i;
         if (i != 3) {
+          const int32_t cell = s[k].cell;
           for (int j = k; j > i + 1; --j) s[j] = s[j - 1];
           Slot n;
+          n.cell = cell;
           n.score = score; n.ref_id = a.ref_id; n.ref_start = ref_start; n.ref_end = ref_end;
           n.read_start = read_start; n.read_end = read_end; n.sense = strand ? -1 : 1;
           n.pass = a.pass; n.rec = (int32_t)r; n.tb_len = x.tb_len;
@@ -70,15 +72,17 @@ __global__ __launch_bounds__(256) void k_slots_init(Slot* slots, int64_t n) {
   Slot x;
   x.score = MIN_SCORE; x.ref_id = -1; x.ref_start = -1; x.ref_end = -1;
   x.read_start = -1; x.read_end = -1; x.sense = 0; x.pass = -1; x.rec = -1; x.tb_len = 0;
+  x.cell = (int32_t)(t & 3);
   slots[t] = x;
 }
 
 // records of pass `pass` that a slot still references: their ids, compacted with an atomic
-__global__ __launch_bounds__(256) void k_mark_referenced(const Slot* slots, int64_t n_slots, int pass,
+// (only ranks < max_rank: a slot never moves back up, so later ranks are never written out)
+__global__ __launch_bounds__(256) void k_mark_referenced(const Slot* slots, int64_t n_slots, int pass, int max_rank,
                                                          int32_t* out_rec, int32_t* out_slot, int32_t* counter) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_slots) return;
-  if (slots[t].pass == pass && slots[t].rec >= 0) {
+  if ((int)(t & 3) < max_rank && slots[t].pass == pass && slots[t].rec >= 0) {
     const int o = atomicAdd(counter, 1);
     out_rec[o] = slots[t].rec;
     out_slot[o] = (int32_t)t;
@@ -103,11 +107,34 @@ hipError_t launch_slots_init(Slot* slots, int64_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_slots_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, slots, n);
   return hipGetLastError();
 }
-hipError_t launch_mark_referenced(const Slot* slots, int64_t n_slots, int pass, int32_t* out_rec, int32_t* out_slot,
-                                  int32_t* counter, hipStream_t st) {
+hipError_t launch_mark_referenced(const Slot* slots, int64_t n_slots, int pass, int max_rank, int32_t* out_rec,
+                                  int32_t* out_slot, int32_t* counter, hipStream_t st) {
   if (n_slots == 0) return hipSuccess;
   hipLaunchKernelGGL(k_mark_referenced, dim3((unsigned)((n_slots + 255) / 256)), dim3(256), 0, st, slots, n_slots, pass,
-                     out_rec, out_slot, counter);
+                     max_rank, out_rec, out_slot, counter);
+  return hipGetLastError();
+}
+
+// tracebacks of the fused path's slots (ranks < max_rank) into a compact list
+__global__ __launch_bounds__(256) void k_gather_cells(const Slot* slots, int64_t n_slots, int max_rank, const uint8_t* cells,
+                                                      int32_t* out_slot, uint8_t* out_tb, int32_t* counter) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_slots) return;
+  const Slot s = slots[t];
+  if ((int)(t & 3) < max_rank && s.ref_id != -1 && s.pass == PASS_FUSED) {
+    const int o = atomicAdd(counter, 1);
+    out_slot[o] = (int32_t)t;
+    const uint32_t* src = (const uint32_t*)(cells + ((t & ~3LL) + s.cell) * (int64_t)TBB);
+    uint32_t* dst = (uint32_t*)(out_tb + (int64_t)o * TBB);
+    for (int w = 0; w < TBB / 4; ++w) dst[w] = src[w];
+  }
+}
+
+hipError_t launch_gather_cells(const Slot* slots, int64_t n_slots, int max_rank, const uint8_t* cells, int32_t* out_slot,
+                               uint8_t* out_tb, int32_t* counter, hipStream_t st) {
+  if (n_slots == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_cells, dim3((unsigned)((n_slots + 255) / 256)), dim3(256), 0, st, slots, n_slots, max_rank,
+                     cells, out_slot, out_tb, counter);
   return hipGetLastError();
 }
 hipError_t launch_gather_tb(const uint8_t* tb, const int32_t* recs, int n, uint8_t* out, hipStream_t st) {

@@ -18,7 +18,10 @@ constexpr int TBB = (MAXL + MAX_I + 2) / 2;   // traceback bytes per record, 2 c
 struct Slot {
   int32_t score, ref_id, ref_start, ref_end;
   int32_t read_start, read_end, sense, pass, rec, tb_len;
+  int32_t cell;   // traceback cell of this read (0..3) in the fused path's store; the four slots of a
+                  // read always hold a permutation of the four cells
 };
+constexpr int PASS_FUSED = -2;   // Slot.pass of entries whose traceback lives in the cell store
 
 struct FilterArgs {
   GenomeDev g;
@@ -113,6 +116,31 @@ struct ReplayArgs {
   Slot* slots;               // 4 per read, global read id
 };
 
+// persistent fused kernel (k_fused.hip)
+struct FusedArgs {
+  GenomeDev g;
+  ReadsDev r;
+  SeedSetDev seeds;
+  ScoreDev sc;
+  const int32_t* n_reads;    // reads of the batch (device value, so a captured graph serves any size)
+  int group;                 // G reads per task
+  int32_t* task_counter;     // cleared before the launch
+  Slot* slots;               // 4 per read
+  uint8_t* cells;            // 4 * TBB bytes per read: the traceback cells
+  uint8_t* overflow;         // per read
+  uint8_t* scratch;          // per workgroup: 64 * scratch_stride direction bytes
+  int64_t scratch_stride;
+  uint8_t* tbbuf;            // per workgroup: F_PCAP * TBB traceback bytes of the current task
+  int ref_id;
+  int sense_mask;
+  unsigned long long* stats; // [0] candidates, [1] filter passes, [2] accepted
+};
+
+constexpr int FUSED_PCAP = 192;   // DP records per task held in LDS (k_fused.hip F_PCAP)
+
+hipError_t launch_fused(const FusedArgs& a, int grid, hipStream_t st);
+hipError_t launch_fused_reset(int32_t* counter, unsigned long long* stats, hipStream_t st);
+int fused_blocks_per_cu(int W, int I, int color);
 hipError_t launch_filter(const FilterArgs& a, int W, hipStream_t st);
 hipError_t launch_align(const AlignArgs& a, hipStream_t st);
 hipError_t launch_prep_reads(const PrepArgs& a, hipStream_t st);
@@ -123,8 +151,10 @@ hipError_t launch_compact_candidates(const CompactArgs& a, hipStream_t st);
 hipError_t launch_make_records(const RecArgs& a, hipStream_t st);
 hipError_t launch_replay(const ReplayArgs& a, hipStream_t st);
 hipError_t launch_slots_init(Slot* slots, int64_t n, hipStream_t st);
-hipError_t launch_mark_referenced(const Slot* slots, int64_t n_slots, int pass, int32_t* out_rec, int32_t* out_slot,
-                                  int32_t* counter, hipStream_t st);
+hipError_t launch_mark_referenced(const Slot* slots, int64_t n_slots, int pass, int max_rank, int32_t* out_rec,
+                                  int32_t* out_slot, int32_t* counter, hipStream_t st);
+hipError_t launch_gather_cells(const Slot* slots, int64_t n_slots, int max_rank, const uint8_t* cells, int32_t* out_slot,
+                               uint8_t* out_tb, int32_t* counter, hipStream_t st);
 hipError_t launch_gather_tb(const uint8_t* tb, const int32_t* recs, int n, uint8_t* out, hipStream_t st);
 hipError_t launch_overflow_ranges(const int64_t* seg_off, const uint8_t* overflow, int64_t n_rs, int64_t* begin,
                                   int64_t* end, hipStream_t st);
