@@ -118,6 +118,7 @@ void storm_filter_plan_destroy(storm_filter_plan *plan);
 typedef struct storm_map_stats {
   int64_t n_reads, n_mapped;
   int64_t n_candidates, n_filter_pass, n_accepted;
+  int64_t n_lookups, n_hits;      /* seed-key lookups (first[key], first[key+1]) and raw bucket hits */
   double seconds_load, seconds_index, seconds_map, seconds_sam;
   double gpu_ms_seed, gpu_ms_filter, gpu_ms_align, gpu_ms_replay;
 } storm_map_stats;
@@ -132,7 +133,8 @@ int storm_map_files(int device, const char *genome_path, const char *reads_path,
  *                  quality levels (read_quality.h:52-58) or NULL
  *   first_base / first_qual : color mode only (reads_data.h:22-27), else NULL
  * The reads are taken in the order given (the caller sorts them by tag if it wants the
- * reference's order, reads_data.c:59-67).  The SAM text is written to sam_path.
+ * reference's order, reads_data.c:59-67).  The SAM text is written to sam_path.  names may be
+ * NULL: read i is then named "r<i>".
  */
 typedef struct storm_reads_view {
   int64_t n_reads;

@@ -298,7 +298,7 @@ static int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::v
 }
 
 struct PassStats {
-  int64_t cand = 0, rec = 0;
+  int64_t cand = 0, rec = 0, hits = 0;
   float ms_seed = 0, ms_filter = 0, ms_align = 0, ms_replay = 0;
 };
 
@@ -354,6 +354,7 @@ static int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int 
   int64_t total = 0;
   HIPOK(hipMemcpyAsync(&total, P.seg_off.as<int64_t>() + n_rs, 8, hipMemcpyDeviceToHost, st));
   HIPOK(hipStreamSynchronize(st));
+  stats.hits += total;
   HIPOK(P.hits.ensure((size_t)std::max<int64_t>(total, 1) * 4));
   HIPOK(P.kept.ensure((size_t)std::max<int64_t>(total, 1) * 4));
   sa.seg_off = P.seg_off.as<int64_t>();
@@ -532,6 +533,9 @@ static int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int 
 
 // Host slots in view order from the device slots (device order).  Tracebacks of ranks < -M come
 // from the staged path's store (pass >= 0) or the fused path's gathered cells (PASS_FUSED).
+// seed-key lookups of a read set: per strand in the sense mask, per seed, its read positions
+static int64_t count_lookups(const storm_mapper* m, const storm_reads_view* v);
+
 static int assemble_slots(storm_mapper* m, const std::vector<Slot>& hs, const std::vector<int64_t>& order,
                           const TbStore& store, const std::unordered_map<int64_t, const uint8_t*>* cells,
                           std::vector<HostSlot>& out_slots, int64_t& mapped) {
@@ -619,6 +623,7 @@ static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode,
     st_out->n_candidates = ps.cand;
     st_out->n_filter_pass = ps.rec;
     st_out->n_accepted = 0;
+    st_out->n_hits = ps.hits;
     st_out->seconds_index = m->index_seconds - ti;
     st_out->seconds_map = now_s() - t0 - st_out->seconds_index;
     st_out->gpu_ms_seed = ps.ms_seed;
@@ -796,7 +801,7 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
   TbStore store;
   PassStats ps;
   int pass_id = 0;
-  unsigned long long totals[3] = {0, 0, 0};
+  unsigned long long totals[4] = {0, 0, 0, 0};
   std::vector<uint8_t> ovf((size_t)n);
   for (size_t r = 0; r < m->refs.size(); ++r) {
     rc = ensure_ref_resident(m, r);
@@ -835,7 +840,7 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
     HIPOK(hipEventRecord(e0, m->st));
     HIPOK(hipGraphLaunch(F.graphs[r], m->st));
     HIPOK(hipEventRecord(e1, m->st));
-    unsigned long long st3[3];
+    unsigned long long st3[4];
     HIPOK(hipMemcpyAsync(st3, F.stats.p, sizeof st3, hipMemcpyDeviceToHost, m->st));
     HIPOK(hipMemcpyAsync(ovf.data(), F.overflow.p, (size_t)n, hipMemcpyDeviceToHost, m->st));
     HIPOK(hipStreamSynchronize(m->st));
@@ -844,7 +849,7 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     ps.ms_filter += ms;   // the fused kernel carries every stage
-    for (int q = 0; q < 3; ++q) totals[q] += st3[q];
+    for (int q = 0; q < 4; ++q) totals[q] += st3[q];
     std::vector<int64_t> over;
     for (int64_t i = 0; i < n; ++i) if (ovf[(size_t)i]) over.push_back(i);
     if (!over.empty()) {
@@ -881,6 +886,7 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
     st_out->n_candidates = (int64_t)totals[0] + ps.cand;
     st_out->n_filter_pass = (int64_t)totals[1] + ps.rec;
     st_out->n_accepted = (int64_t)totals[2];
+    st_out->n_hits = (int64_t)totals[3] + ps.hits;
     st_out->seconds_index = m->index_seconds - ti;
     st_out->seconds_map = now_s() - t0 - st_out->seconds_index;
     st_out->gpu_ms_seed = ps.ms_seed;
@@ -891,10 +897,21 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
   return STORM_OK;
 }
 
+static int64_t count_lookups(const storm_mapper* m, const storm_reads_view* v) {
+  const int strands = ((m->p.sense_mask & 1) ? 1 : 0) + ((m->p.sense_mask & 2) ? 1 : 0);
+  int64_t n = 0;
+  for (int64_t i = 0; i < v->n_reads; ++i)
+    for (const SeedSpec& s : m->seeds)
+      n += s.pos.empty() ? std::max<int64_t>(0, v->len[i] - s.span + 1) : (int64_t)s.pos.size();
+  return n * strands * (int64_t)m->refs.size();
+}
+
 static int map_view(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
                     storm_map_stats* st_out) {
-  if (mode == STORM_MODE_BATCHED) return map_view_batched(m, v, out_slots, st_out);
-  return map_view_staged(m, v, mode, out_slots, st_out);
+  int rc = (mode == STORM_MODE_BATCHED) ? map_view_batched(m, v, out_slots, st_out)
+                                         : map_view_staged(m, v, mode, out_slots, st_out);
+  if (rc == STORM_OK && st_out) st_out->n_lookups = count_lookups(m, v);
+  return rc;
 }
 
 
@@ -995,7 +1012,7 @@ static int build_host_reads(const storm_reads_view* v, int color, HostReads& hr)
     if (hr.first_qual.empty()) hr.first_qual.assign((size_t)hr.n, 0);
   }
   for (int64_t i = 0; i < hr.n; ++i) {
-    hr.names[(size_t)i] = v->names ? v->names[i] : std::to_string(i);
+    hr.names[(size_t)i] = v->names ? std::string(v->names[i]) : ("r" + std::to_string(i));
     hr.max_len = std::max(hr.max_len, (int)v->len[i]);
   }
   return STORM_OK;

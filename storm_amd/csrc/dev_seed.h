@@ -60,7 +60,7 @@ __device__ __forceinline__ void wave_bitonic_sort(int32_t* buf, int n) {
 // when the raw hits exceed hcap or the candidates exceed `cap` (the caller sends the read to the
 // staged path, which has no such limits).
 __device__ int wave_candidates(const ReadsDev& r, const SeedSetDev& seeds, int64_t rs, int I, int32_t* hbuf, int hcap,
-                               int32_t* out, int cap) {
+                               int32_t* out, int cap, int* raw_hits) {
   const int lane = threadIdx.x & 63;
   const int L = r.len[rs >> 1];
   const uint32_t* pl = read_planes(r, rs);
@@ -87,6 +87,7 @@ __device__ int wave_candidates(const ReadsDev& r, const SeedSetDev& seeds, int64
       if (run > hcap) over = true;
     }
   }
+  *raw_hits = run;
   if (over) return -1;
   if (run == 0) return 0;
   __syncthreads();

@@ -30,6 +30,7 @@ __global__ __launch_bounds__(64) void k_fused(FusedArgs a) {
   __shared__ int16_t crec[F_CCAP];
   __shared__ int16_t cstart[2 * F_GMAX + 1];
   __shared__ uint8_t rovf[F_GMAX];
+  __shared__ int32_t rawh[2 * F_GMAX];
   __shared__ AlignRec recs[F_PCAP];
   __shared__ int16_t rec2cand[F_PCAP];
   __shared__ int32_t task_sh;
@@ -41,7 +42,7 @@ __global__ __launch_bounds__(64) void k_fused(FusedArgs a) {
   const FilterConst fk = filter_const(a.sc.fmatch, a.sc.fmismatch, a.sc.fopen, a.sc.fext, a.sc.fthr);
   uint8_t* scratch = a.scratch + (int64_t)blockIdx.x * 64 * a.scratch_stride + (int64_t)lane * a.scratch_stride;
   uint8_t* tbbuf = a.tbbuf + (int64_t)blockIdx.x * F_PCAP * TBB;
-  unsigned long long st_cand = 0, st_pass = 0, st_acc = 0;
+  unsigned long long st_cand = 0, st_pass = 0, st_acc = 0, st_hits = 0;
 
   for (;;) {
     if (lane == 0) task_sh = atomicAdd(a.task_counter, 1);
@@ -57,11 +58,13 @@ __global__ __launch_bounds__(64) void k_fused(FusedArgs a) {
     // 1. candidates of every read strand of the task (stages a + b)
     int nc = 0;
     for (int j = 0; j < 2 * nr; ++j) {
-      if (lane == 0) cstart[j] = (int16_t)nc;
+      if (lane == 0) { cstart[j] = (int16_t)nc; rawh[j] = 0; }
       const bool wanted = ((a.sense_mask >> (j & 1)) & 1) && !rovf[j >> 1];
       if (!wanted) continue;
       const int64_t rs = 2 * base + j;
-      const int cnt = wave_candidates(a.r, a.seeds, rs, I, hbuf, F_HCAP, cdiag + nc, F_CCAP - nc);
+      int raw = 0;
+      const int cnt = wave_candidates(a.r, a.seeds, rs, I, hbuf, F_HCAP, cdiag + nc, F_CCAP - nc, &raw);
+      if (lane == 0) rawh[j] = raw;
       if (cnt < 0) {
         if (lane == 0) rovf[j >> 1] = 1;
         __syncthreads();
@@ -128,6 +131,7 @@ __global__ __launch_bounds__(64) void k_fused(FusedArgs a) {
       const int64_t read = base + lr;
       if (rovf[lr]) { a.overflow[read] = 1; continue; }
       a.overflow[read] = 0;
+      st_hits += (unsigned long long)(rawh[2 * lr] + rawh[2 * lr + 1]);
       Slot s[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) s[k] = a.slots[read * 4 + k];
@@ -189,13 +193,14 @@ __global__ __launch_bounds__(64) void k_fused(FusedArgs a) {
   if (a.stats) {
     if (lane == 0) { atomicAdd(&a.stats[0], st_cand); atomicAdd(&a.stats[1], st_pass); }
     atomicAdd(&a.stats[2], st_acc);
+    atomicAdd(&a.stats[3], st_hits);
   }
 }
 
 // clears the task counter and the stats inside a captured graph (no memset nodes)
 __global__ void k_fused_reset(int32_t* counter, unsigned long long* stats) {
   if (threadIdx.x == 0) { *counter = 0; }
-  if (stats && threadIdx.x < 3) stats[threadIdx.x] = 0;
+  if (stats && threadIdx.x < 4) stats[threadIdx.x] = 0;
 }
 
 hipError_t launch_fused_reset(int32_t* counter, unsigned long long* stats, hipStream_t st) {

@@ -133,7 +133,7 @@ struct FusedArgs {
   uint8_t* tbbuf;            // per workgroup: F_PCAP * TBB traceback bytes of the current task
   int ref_id;
   int sense_mask;
-  unsigned long long* stats; // [0] candidates, [1] filter passes, [2] accepted
+  unsigned long long* stats; // [0] candidates, [1] filter passes, [2] accepted, [3] raw hits
 };
 
 constexpr int FUSED_PCAP = 192;   // DP records per task held in LDS (k_fused.hip F_PCAP)

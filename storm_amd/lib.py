@@ -30,7 +30,8 @@ class Params(C.Structure):
 
 
 class MapStats(C.Structure):
-    _fields_ = [(n, C.c_int64) for n in ("n_reads", "n_mapped", "n_candidates", "n_filter_pass", "n_accepted")] + \
+    _fields_ = [(n, C.c_int64) for n in ("n_reads", "n_mapped", "n_candidates", "n_filter_pass", "n_accepted",
+                                          "n_lookups", "n_hits")] + \
                [(n, C.c_double) for n in ("seconds_load", "seconds_index", "seconds_map", "seconds_sam",
                                           "gpu_ms_seed", "gpu_ms_filter", "gpu_ms_align", "gpu_ms_replay")]
 
@@ -156,3 +157,80 @@ def map_files(genome: str, reads: str, sam: str, qual: str | None = None, mode: 
     check(lib.storm_map_files(device, genome.encode(), reads.encode(), qual.encode() if qual else None, C.byref(p),
                               mode, sam.encode(), len(argv), C.cast(arr, C.c_void_p), C.byref(st)))
     return st.as_dict()
+
+
+class Mapper:
+    """storm_mapper_*: genome and seed indexes resident in HBM, reads mapped from host arrays.
+
+    `reads` is any object with the fields of synth.PackedReads: codes (n, stride) u8 packed
+    codes, qual (n, stride) u8 packed 2-bit levels or None, len (n,) i32, and for color reads
+    first_base / first_qual (n,) u8.
+    """
+
+    def __init__(self, genome: str, device: int = 0, **kw):
+        lib = load()
+        self.params = params(**kw)
+        h = C.c_void_p()
+        check(lib.storm_mapper_create(device, genome.encode(), C.byref(self.params), C.byref(h)))
+        self.h = h
+
+    @staticmethod
+    def view(reads, lo: int = 0, hi: int | None = None, names=None) -> tuple[ReadsView, tuple]:
+        hi = len(reads.len) if hi is None else hi
+        codes = np.ascontiguousarray(reads.codes[lo:hi])
+        qual = None if reads.qual is None else np.ascontiguousarray(reads.qual[lo:hi])
+        lens = np.ascontiguousarray(reads.len[lo:hi], dtype=np.int32)
+        fb = None if reads.first_base is None else np.ascontiguousarray(reads.first_base[lo:hi])
+        fq = None if reads.first_qual is None else np.ascontiguousarray(reads.first_qual[lo:hi])
+        v = ReadsView()
+        v.n_reads = hi - lo
+        keep = [codes, qual, lens, fb, fq]
+        if names is not None:
+            arr = (C.c_char_p * (hi - lo))(*[n.encode() for n in names[lo:hi]])
+            keep.append(arr)
+            v.names = C.cast(arr, C.POINTER(C.c_char_p))
+        v.codes = codes.ctypes.data
+        v.code_stride = codes.shape[1]
+        v.qual = None if qual is None else qual.ctypes.data
+        v.qual_stride = 0 if qual is None else qual.shape[1]
+        v.len = lens.ctypes.data
+        v.first_base = None if fb is None else fb.ctypes.data
+        v.first_qual = None if fq is None else fq.ctypes.data
+        v.db_name = None
+        return v, tuple(keep)
+
+    def map_sam(self, reads, sam: str, mode: int = STORM_MODE_BATCHED, lo: int = 0, hi: int | None = None,
+                names=None, argv=None) -> dict:
+        v, keep = self.view(reads, lo, hi, names)
+        st = MapStats()
+        argv = argv or ["storm-gpu"]
+        arr = (C.c_char_p * len(argv))(*[a.encode() for a in argv])
+        check(load().storm_mapper_map(self.h, C.byref(v), mode, sam.encode(), len(argv), C.cast(arr, C.c_void_p),
+                                      C.byref(st)))
+        del keep
+        return st.as_dict()
+
+    def map_hits(self, reads, mode: int = STORM_MODE_BATCHED, lo: int = 0, hi: int | None = None):
+        """Top-4 hits per read: dict of (n, 4) int32 arrays score/ref_id/ref_start/sense, and stats."""
+        v, keep = self.view(reads, lo, hi)
+        n = v.n_reads
+        out = {k: np.zeros((n, 4), np.int32) for k in ("score", "ref_id", "ref_start", "sense")}
+        st = MapStats()
+        check(load().storm_mapper_map_hits(self.h, C.byref(v), mode, _ptr(out["score"]), _ptr(out["ref_id"]),
+                                           _ptr(out["ref_start"]), _ptr(out["sense"]), C.byref(st)))
+        del keep
+        return out, st.as_dict()
+
+    def free_bytes(self) -> int:
+        return int(load().storm_mapper_free_bytes(self.h))
+
+    def close(self):
+        if self.h:
+            load().storm_mapper_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

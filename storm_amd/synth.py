@@ -224,3 +224,141 @@ def filter_candidates(genome: np.ndarray, n_reads: int, L: int, W: int, d: int, 
 def ensure_dir(p: str) -> str:
     os.makedirs(p, exist_ok=True)
     return p
+
+
+def quality_levels(q: np.ndarray) -> np.ndarray:
+    """read_quality.c:10-16: q <= 2 -> 0, <= 7 -> 1, <= 10 -> 2, else 3."""
+    return np.where(q <= 2, 0, np.where(q <= 7, 1, np.where(q <= 10, 2, 3))).astype(np.uint8)
+
+
+def pack_rows(codes: np.ndarray, nbytes: int) -> np.ndarray:
+    """Row-wise 2-bit packing of an (n, Lmax) code matrix into (n, nbytes)."""
+    n, L = codes.shape
+    Lp = ((L + 3) // 4) * 4
+    c = np.zeros((n, Lp), np.uint8)
+    c[:, :L] = codes
+    c = c.reshape(n, Lp // 4, 4)
+    packed = c[:, :, 0] | (c[:, :, 1] << 2) | (c[:, :, 2] << 4) | (c[:, :, 3] << 6)
+    out = np.zeros((n, nbytes), np.uint8)
+    out[:, :min(nbytes, packed.shape[1])] = packed[:, :nbytes]
+    return out
+
+
+@dataclass
+class PackedReads:
+    """Reads packed for storm_reads_view: codes/levels 2-bit, one row per read."""
+    codes: np.ndarray       # (n, stride) uint8
+    qual: np.ndarray        # (n, stride) uint8 levels
+    len: np.ndarray         # (n,) int32
+    first_base: np.ndarray | None = None
+    first_qual: np.ndarray | None = None
+    origin: np.ndarray | None = None
+    strand: np.ndarray | None = None
+
+
+def fast_nucleotide_reads(genome: np.ndarray, n: int, L, seed: int, sub: float, indel: float, qlo: int, qhi: int,
+                          chunk: int = 500_000) -> PackedReads:
+    """Vectorised Illumina-like reads: uniform position and strand, substitutions, and for a
+    fraction n*L*indel of the reads one 1-3 bp insertion or deletion at a uniform offset."""
+    g = rng(seed)
+    G = len(genome)
+    lens = np.full(n, L, np.int32) if np.isscalar(L) else g.integers(L[0], L[1] + 1, size=n).astype(np.int32)
+    Lmax = int(lens.max())
+    stride = ((Lmax + 3) // 4 + 7) & ~7
+    codes = np.zeros((n, stride), np.uint8)
+    qual = np.zeros((n, stride), np.uint8)
+    origin = g.integers(0, G - Lmax - 8, size=n)
+    strand = g.integers(0, 2, size=n).astype(np.uint8)
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        m = e - s
+        idx = origin[s:e, None] + np.arange(Lmax + 4)[None, :]
+        r = genome[idx]
+        has = g.random(m) < (lens[s:e] * indel)
+        for k in np.nonzero(has)[0]:
+            Lk = int(lens[s + k])
+            off = int(g.integers(1, Lk - 1))
+            ln = int(g.integers(1, 4))
+            row = r[k].copy()
+            if g.random() < 0.5:   # insertion in the read
+                row = np.concatenate([row[:off], g.integers(0, 4, size=ln, dtype=np.uint8), row[off:]])[:Lmax + 4]
+            else:                  # deletion from the read
+                row = np.concatenate([row[:off], row[off + ln:], g.integers(0, 4, size=ln, dtype=np.uint8)])[:Lmax + 4]
+            r[k] = row
+        r = r[:, :Lmax]
+        mut = g.random(r.shape) < sub
+        r = np.where(mut, (r + g.integers(1, 4, size=r.shape, dtype=np.uint8)) & 3, r).astype(np.uint8)
+        rev = strand[s:e] == 1
+        if rev.any():
+            rr = r[rev]
+            Ls = lens[s:e][rev]
+            out = np.zeros_like(rr)
+            for Lv in np.unique(Ls):
+                sel = Ls == Lv
+                out[sel, :Lv] = 3 - rr[sel, :Lv][:, ::-1]
+            r[rev] = out
+        col = np.arange(Lmax)[None, :]
+        r = np.where(col < lens[s:e, None], r, 0).astype(np.uint8)
+        codes[s:e] = pack_rows(r, stride)
+        q = g.integers(qlo, qhi + 1, size=(m, Lmax))
+        qual[s:e] = pack_rows(np.where(col < lens[s:e, None], quality_levels(q), 0).astype(np.uint8), stride)
+    return PackedReads(codes, qual, lens, None, None, origin, strand)
+
+
+def fast_color_reads(genome: np.ndarray, n: int, L, seed: int, snp: float, color_err: float, qlo: int, qhi: int,
+                     chunk: int = 500_000) -> PackedReads:
+    """Vectorised SOLiD-like reads: L colors covering L+1 bases; SNPs in base space, then color
+    errors; qualities uniform in [qlo, qhi] for the primer-side color and the L colors."""
+    g = rng(seed)
+    G = len(genome)
+    lens = np.full(n, L, np.int32) if np.isscalar(L) else g.integers(L[0], L[1] + 1, size=n).astype(np.int32)
+    Lmax = int(lens.max())
+    stride = ((Lmax + 3) // 4 + 7) & ~7
+    codes = np.zeros((n, stride), np.uint8)
+    qual = np.zeros((n, stride), np.uint8)
+    first_base = np.zeros(n, np.uint8)
+    first_qual = np.zeros(n, np.uint8)
+    origin = g.integers(0, G - Lmax - 4, size=n)
+    strand = g.integers(0, 2, size=n).astype(np.uint8)
+    col = np.arange(Lmax)[None, :]
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        m = e - s
+        idx = origin[s:e, None] + np.arange(Lmax + 1)[None, :]
+        b = genome[idx]
+        mut = g.random(b.shape) < snp
+        b = np.where(mut, (b + g.integers(1, 4, size=b.shape, dtype=np.uint8)) & 3, b).astype(np.uint8)
+        rev = strand[s:e] == 1
+        if rev.any():
+            bb = b[rev]
+            Ls = lens[s:e][rev]
+            out = np.zeros_like(bb)
+            for Lv in np.unique(Ls):
+                sel = Ls == Lv
+                out[sel, :Lv + 1] = 3 - bb[sel, :Lv + 1][:, ::-1]
+            b[rev] = out
+        c = (b[:, 1:] ^ b[:, :-1]).astype(np.uint8)
+        err = g.random(c.shape) < color_err
+        c = np.where(err, c ^ g.integers(1, 4, size=c.shape, dtype=np.uint8), c).astype(np.uint8)
+        c = np.where(col < lens[s:e, None], c, 0).astype(np.uint8)
+        codes[s:e] = pack_rows(c, stride)
+        first_base[s:e] = b[:, 0]
+        q = g.integers(qlo, qhi + 1, size=(m, Lmax + 1))
+        lv = quality_levels(q)
+        first_qual[s:e] = lv[:, 0]
+        qual[s:e] = pack_rows(np.where(col < lens[s:e, None], lv[:, 1:], 0).astype(np.uint8), stride)
+    return PackedReads(codes, qual, lens, first_base, first_qual, origin, strand)
+
+
+def write_fasta_fast(path: str, name: str, codes: np.ndarray, width: int = 60) -> None:
+    letters = BASES[codes]
+    n = len(letters)
+    full = n // width
+    with open(path, "wb") as f:
+        f.write(b">" + name.encode() + b"\n")
+        if full:
+            block = np.concatenate([letters[:full * width].reshape(full, width),
+                                    np.full((full, 1), ord("\n"), np.uint8)], axis=1)
+            f.write(block.tobytes())
+        if n % width:
+            f.write(letters[full * width:].tobytes() + b"\n")
