@@ -17,6 +17,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import subprocess
 import sys
 import tempfile
 import time
@@ -150,29 +151,172 @@ def cpu_baseline(g, reads, pos, stride, L, d, n_sample, target_s=10.0):
     return out
 
 
-def mapping_runs(lib):
-    """Whole -M runs: 1 Mbp genome, 100k Illumina reads of 50 nt (2 % substitutions, 0.1 % indels,
-    Q20-40), -i 3 -z 100 -t 100 -M 1, reference default seeds."""
+SEEDS_N = "####--#-#--##--####;#####--###-#-###;###-##-#######"   # seed.h:29 (nucleotide defaults)
+SEEDS_C = "####----##--##-####;####-###--#----####;####-####-####"   # seed.h:31 (color defaults)
+
+
+def host_cores() -> int:
+    try:
+        return max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        return max(1, min(16, os.cpu_count() or 1))
+
+
+def sam_body(path):
+    with open(path) as f:
+        return [l for l in f if not l.startswith("@PG")]
+
+
+def ref_binary(color: int):
+    p = os.path.join(ROOT, "oracle", "_ref", "storm-color" if color else "storm-nucleotide")
+    return p if os.path.exists(p) else None
+
+
+def cli_args(kw) -> list:
+    out = ["-i", str(kw["allowed_indels"]), "-z", str(kw["z"]), "-t", str(kw["t"]), "-M", str(kw["max_hits"])]
+    return out
+
+
+def write_read_files(d, pr, idx, color, tag):
     from storm_amd import synth
-    d = tempfile.mkdtemp(prefix="storm_bench_")
-    g = synth.random_genome(1_000_000, 1)
-    synth.write_fasta(os.path.join(d, "g.fa"), "chr1", g)
-    r = synth.simulate_nucleotide_reads(g, 100_000, 50, 101, sub=0.02, indel=0.001, qlo=20, qhi=40)
-    synth.write_fastq(os.path.join(d, "r.fq"), r)
-    res = {}
-    for mode, name in ((0, "plain"), (1, "batched")):
-        best = None
-        for _ in range(2):
-            st = lib.map_files(os.path.join(d, "g.fa"), os.path.join(d, "r.fq"), os.path.join(d, f"{name}.sam"),
-                               mode=mode, allowed_indels=3, z=100, t=100, max_hits=1)
-            if best is None or st["seconds_map"] < best["seconds_map"]:
-                best = st
-        res[name] = {"reads_per_s": best["n_reads"] / best["seconds_map"], "mapped": best["n_mapped"],
-                     "reads": best["n_reads"], "seconds_map": best["seconds_map"],
-                     "gpu_ms": {k: round(best["gpu_ms_" + k], 3) for k in ("seed", "filter", "align", "replay")}}
-    same = open(os.path.join(d, "plain.sam")).read() == open(os.path.join(d, "batched.sam")).read()
-    res["plain_equals_batched"] = same
+    sr = synth.packed_to_simreads(pr, idx)
+    if color:
+        path, qpath = os.path.join(d, f"{tag}.csfasta"), os.path.join(d, f"{tag}.qual")
+        synth.write_csfasta(path, qpath, sr)
+        return ["-r", path, "-q", qpath], (path, qpath)
+    path = os.path.join(d, f"{tag}.fq")
+    synth.write_fastq(path, sr)
+    return ["-r", path], (path, None)
+
+
+def run_reference(color, gpath, rargs, kw, sam, cores):
+    exe = ref_binary(color)
+    cmd = [exe, "-g", gpath] + rargs + cli_args(kw) + ["-s", SEEDS_C if color else SEEDS_N, "-N", str(cores),
+                                                     "-o", sam, "-v", "0"]
+    t0 = time.perf_counter()
+    # the reference ORs color reads into an uninitialised buffer; a fixed perturbation byte makes it deterministic
+    subprocess.run(cmd, check=True, env=dict(os.environ, MALLOC_PERTURB_="255"), stderr=subprocess.DEVNULL,
+                   stdout=subprocess.DEVNULL)
+    return time.perf_counter() - t0
+
+
+def cpu_mapping_baseline(lib, d, gpath, pr, kw, color, n_sample):
+    """The reference binary (oracle/_ref, default Makefile flags, OpenMP) on a sample of the
+    workload's reads, timed on this host: one run on 16 reads gives the fixed cost (genome load,
+    index build), one on the sample; reads/s = sample / (difference).  The GPU maps the same files
+    and its SAM is compared with the reference's."""
+    if ref_binary(color) is None:
+        return None
+    cores = host_cores()
+    n_sample = min(n_sample, len(pr.len))
+    r_tiny, _ = write_read_files(d, pr, np.arange(16), color, "tiny")
+    r_samp, files = write_read_files(d, pr, np.arange(n_sample), color, "sample")
+    t_fixed = run_reference(color, gpath, r_tiny, kw, os.path.join(d, "ref_tiny.sam"), cores)
+    t_all = run_reference(color, gpath, r_samp, kw, os.path.join(d, "ref_sample.sam"), cores)
+    gsam = os.path.join(d, "gpu_sample.sam")
+    lib.map_files(gpath, files[0], gsam, qual=files[1], mode=1, color=color, **kw)
+    same = sam_body(gsam) == sam_body(os.path.join(d, "ref_sample.sam"))
+    dt = max(t_all - t_fixed, 1e-6)
+    return {"value": round(n_sample / dt, 1), "unit": "reads/s", "cores": cores, "kind": "reference",
+            "sample": f"{n_sample} reads of the workload; reference {os.path.basename(ref_binary(color))} -N {cores}: "
+                      f"{t_all:.1f}s total minus {t_fixed:.1f}s fixed cost (load + index, measured on 16 reads)",
+            "gpu_sam_identical_to_reference": bool(same)}
+
+
+def map_config(lib, name, gpath, pr, kw, color, n_sample_cpu, sam_reads, skip_plain=False):
+    """Batched (persistent fused kernel in a hipGraph) and plain (staged kernels, one batch per
+    read length) on the same resident mapper; hits returned to host memory; whole-call wall time."""
+    t0 = time.perf_counter()
+    m = lib.Mapper(gpath, color=color, **kw)
+    t_create = time.perf_counter() - t0
+    n = len(pr.len)
+    w = min(n, 50_000)
+    m.map_hits(pr, mode=1, lo=0, hi=w)   # warm-up: code objects, autotune, graph capture
+    if not skip_plain:
+        m.map_hits(pr, mode=0, lo=0, hi=w)
+    res = {"reads": n, "genome_load_index_s": round(t_create, 2)}
+    t0 = time.perf_counter()
+    hb, sb = m.map_hits(pr, mode=1)
+    tb = time.perf_counter() - t0
+    res["batched"] = {"reads_per_s": round(n / tb, 1), "seconds": round(tb, 3), "mapped": sb["n_mapped"],
+                      "gpu_ms_fused": round(sb["gpu_ms_filter"], 2),
+                      "staged_fallback_gpu_ms": round(sb["gpu_ms_seed"] + sb["gpu_ms_align"] + sb["gpu_ms_replay"], 2),
+                      "fallback_reads": sb["n_fallback"],
+                      "candidates": sb["n_candidates"], "filter_passes": sb["n_filter_pass"],
+                      "lookups": sb["n_lookups"], "raw_hits": sb["n_hits"]}
+    log(f"{name}: batched {n} reads in {tb:.2f}s = {n / tb:.0f} reads/s, mapped {sb['n_mapped']}, "
+        f"candidates {sb['n_candidates']}, fused kernel {sb['gpu_ms_filter']:.1f} ms")
+    if not skip_plain:
+        t0 = time.perf_counter()
+        hp, sp = m.map_hits(pr, mode=0)
+        tp = time.perf_counter() - t0
+        same = all(np.array_equal(hb[k], hp[k]) for k in hb)
+        lk_bytes = 8 * sp["n_lookups"]
+        res["plain"] = {"reads_per_s": round(n / tp, 1), "seconds": round(tp, 3), "mapped": sp["n_mapped"],
+                        "gpu_ms": {k: round(sp["gpu_ms_" + k], 2) for k in ("seed", "lookup", "filter", "align",
+                                                                            "replay")},
+                        "index_lookup_GBps": round(lk_bytes / (sp["gpu_ms_lookup"] * 1e-3) / 1e9, 1)
+                        if sp["gpu_ms_lookup"] > 0 else None}
+        res["batched_over_plain"] = round(tp / tb, 2)
+        res["plain_equals_batched"] = bool(same)
+        log(f"{name}: plain {n / tp:.0f} reads/s (batched/plain = {tp / tb:.2f}x), identical hits: {same}")
+    if sam_reads:
+        k = min(n, sam_reads)
+        st = m.map_sam(pr, "/dev/null", mode=1, lo=0, hi=k)
+        res["sam_writer_reads_per_s"] = round(k / max(st["seconds_sam"], 1e-9), 1)
+    m.close()
+    if n_sample_cpu:
+        cb = cpu_mapping_baseline(lib, os.path.dirname(gpath), gpath, pr, kw, color, n_sample_cpu)
+        res["cpu_baseline"] = cb
+        if cb:
+            log(f"{name}: reference CPU {cb['value']} reads/s on {cb['cores']} cores; GPU SAM identical on the sample: "
+                f"{cb['gpu_sam_identical_to_reference']}")
     return res
+
+
+def config1(lib, cpu=True):
+    """1 Mbp i.i.d. genome (seed 1), 100k Illumina reads of 50 nt (2 % substitutions, 0.1 % 1-3 bp
+    indels, Q20-40), default seeds, -i 3 -z 100 -t 100 -M 1."""
+    from storm_amd import synth
+    d = tempfile.mkdtemp(prefix="storm_c1_")
+    g = synth.random_genome(1_000_000, 1)
+    gpath = os.path.join(d, "g.fa")
+    synth.write_fasta_fast(gpath, "chr1", g)
+    pr = synth.fast_nucleotide_reads(g, 100_000, 50, 101, sub=0.02, indel=0.001, qlo=20, qhi=40)
+    kw = dict(allowed_indels=3, z=100, t=100, max_hits=1)
+    return map_config(lib, "config1", gpath, pr, kw, 0, 100_000 if cpu else 0, 100_000)
+
+
+def config3(lib, cpu=True, n_reads=5_000_000):
+    """50 Mbp genome with GC drawn in 40-60 % per 10 kb window; 5M SOLiD color reads of 34-74
+    colors after the primer (35-75 bp reads), 3 % color errors, 1 % SNPs, Q10-35; the three
+    default weight-12 color seeds, -i 3 -z 85 -t 100 -M 1."""
+    from storm_amd import synth
+    d = tempfile.mkdtemp(prefix="storm_c3_")
+    g = synth.gc_window_genome(50_000_000, 3)
+    gpath = os.path.join(d, "g.fa")
+    synth.write_fasta_fast(gpath, "chr1", g)
+    t0 = time.perf_counter()
+    pr = synth.fast_color_reads(g, n_reads, (34, 74), 303, snp=0.01, color_err=0.03, qlo=10, qhi=35)
+    log(f"config3: {n_reads} reads generated in {time.perf_counter() - t0:.1f}s")
+    kw = dict(allowed_indels=3, z=85, t=100, max_hits=1)
+    return map_config(lib, "config3", gpath, pr, kw, 1, 20_000 if cpu else 0, 500_000)
+
+
+def config4(lib, cpu=True, n_reads=20_000_000, skip_plain=False):
+    """100 Mbp genome: i.i.d. background, 10 % segmental duplications (1-20 kb copies, 1-5 %
+    divergence), 5 % tandem repeats (units of 2-50 bp); 20M reads of 100 nt, 1.5 % substitutions,
+    0.2 % indels, Q15-40; the three default weight-12 seeds, -i 7 -z 100 -t 115 -M 4."""
+    from storm_amd import synth
+    d = tempfile.mkdtemp(prefix="storm_c4_")
+    g = synth.repeat_genome(100_000_000, 4)
+    gpath = os.path.join(d, "g.fa")
+    synth.write_fasta_fast(gpath, "chr1", g)
+    t0 = time.perf_counter()
+    pr = synth.fast_nucleotide_reads(g, n_reads, 100, 404, sub=0.015, indel=0.002, qlo=15, qhi=40)
+    log(f"config4: {n_reads} reads generated in {time.perf_counter() - t0:.1f}s")
+    kw = dict(allowed_indels=7, z=100, t=115, max_hits=4)
+    return map_config(lib, "config4", gpath, pr, kw, 0, 2_000 if cpu else 0, 200_000, skip_plain=skip_plain)
 
 
 def main():
@@ -184,6 +328,11 @@ def main():
     ap.add_argument("--no-map", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--percall", type=int, default=20000)
+    ap.add_argument("--config", default="default", choices=["default", "1", "3", "4"],
+                    help="default: filter headline + config-1 mapping; 1/3/4: that mapping workload alone")
+    ap.add_argument("--reads-c3", type=int, default=5_000_000)
+    ap.add_argument("--reads-c4", type=int, default=20_000_000)
+    ap.add_argument("--no-plain", action="store_true", help="mapping workloads: batched path only")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the path shards reads across GPUs by design "
@@ -194,6 +343,23 @@ def main():
     assert torch.cuda.is_available()
     torch.zeros(1, device="cuda:0")
     lib.load()
+    if a.config != "default":
+        cpu = not a.no_cpu
+        if a.config == "1":
+            res = config1(lib, cpu)
+        elif a.config == "3":
+            res = config3(lib, cpu, a.reads_c3)
+        else:
+            res = config4(lib, cpu, a.reads_c4, skip_plain=a.no_plain)
+        best = res["batched"]
+        line = {"metric": METRIC, "value": best["reads_per_s"], "unit": "reads/s", "n_gpus": 1, "steps": 1,
+                "warmup": 1, "ms_per_step": round(best["seconds"] * 1e3, 2), "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "u8/int32", "data": "synthetic",
+                "config": {"workload": f"mapping config {a.config}: " + " ".join(
+                    {"1": config1, "3": config3, "4": config4}[a.config].__doc__.split())},
+                "mapping": res}
+        print(json.dumps(line), flush=True)
+        return
     L, W, d = 75, 8, 7
     t0 = time.perf_counter()
     g, reads, pos, stride = make_filter_workload(a.reads, L, W, d)
@@ -224,10 +390,7 @@ def main():
     pc_dt, _ = percall_filter(lib, g, reads, pos, stride, L, W, min(a.percall, a.reads))
     pc_gcups = 2 * min(a.percall, a.reads) * L * 2 * W / pc_dt / 1e9
     log(f"per-call drop-in row: {pc_dt:.2f}s for {min(a.percall, a.reads)} calls, {pc_gcups:.3f} GCUPS")
-    mapping = None if a.no_map else mapping_runs(lib)
-    if mapping:
-        log(f"mapping 100k x 50nt on 1 Mbp: plain {mapping['plain']['reads_per_s']:.0f} reads/s, "
-            f"batched {mapping['batched']['reads_per_s']:.0f} reads/s")
+    mapping = None if a.no_map else config1(lib, cpu=not a.no_cpu)
     cpu = None
     if not a.no_cpu:
         cb = cpu_baseline(g, reads, pos, stride, L, d, n_sample=500_000)

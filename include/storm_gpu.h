@@ -119,8 +119,10 @@ typedef struct storm_map_stats {
   int64_t n_reads, n_mapped;
   int64_t n_candidates, n_filter_pass, n_accepted;
   int64_t n_lookups, n_hits;      /* seed-key lookups (first[key], first[key+1]) and raw bucket hits */
+  int64_t n_fallback;             /* batched mode: reads the fused kernel handed to the staged kernels */
   double seconds_load, seconds_index, seconds_map, seconds_sam;
   double gpu_ms_seed, gpu_ms_filter, gpu_ms_align, gpu_ms_replay;
+  double gpu_ms_lookup;           /* plain mode: the seed-key lookup kernel alone (part of gpu_ms_seed) */
 } storm_map_stats;
 
 int storm_map_files(int device, const char *genome_path, const char *reads_path, const char *qual_path,

@@ -163,7 +163,7 @@ struct FusedTune {
 
 struct FusedState {
   DevReads reads;
-  DevMem slots, cells, overflow, n_dev, counter, stats, scratch, tbbuf, gids, gtb, gcount;
+  DevMem slots, cells, overflow, n_dev, counter, stats, scratch, tbbuf, work, gids, gtb, gcount;
   int64_t cap_reads = 0, scratch_stride = 0;
   int grid = 0;
   std::vector<hipGraphExec_t> graphs;   // per reference (nullptr = not captured)
@@ -192,6 +192,7 @@ struct storm_mapper {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double index_seconds = 0;
   storm::FusedState fs;
+  bool want_tb = true;   // false while a call returns hits only: tracebacks stay on the device
 };
 
 namespace storm {
@@ -246,8 +247,10 @@ static SeedSetDev seed_set(const RefState& rs) {
   return s;
 }
 
+// Uploads the reads order[0..) of the view (all of them, or the subset the fused kernel handed
+// to the staged kernels), in that order.
 static int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order, DevReads& d) {
-  const int64_t n = v->n_reads;
+  const int64_t n = (int64_t)order.size();
   d.n = n;
   d.has_qual = v->qual != nullptr;
   const int64_t cs = v->code_stride, qs = v->qual ? v->qual_stride : 0;
@@ -299,7 +302,7 @@ static int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::v
 
 struct PassStats {
   int64_t cand = 0, rec = 0, hits = 0;
-  float ms_seed = 0, ms_filter = 0, ms_align = 0, ms_replay = 0;
+  float ms_seed = 0, ms_filter = 0, ms_align = 0, ms_replay = 0, ms_lookup = 0;
 };
 
 // traceback bytes of records still referenced by a slot, kept on the host per (pass, rec)
@@ -323,10 +326,9 @@ static int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int 
   if (n_rs == 0) return STORM_OK;
   ScoreDev sc{};
   fill_score(m->p, sc);
-  hipEvent_t e[6];
+  hipEvent_t e[7];
   for (auto& x : e) HIPOK(hipEventCreate(&x));
-  struct EvGuard { hipEvent_t* e; ~EvGuard() { for (int i = 0; i < 6; ++i) (void)hipEventDestroy(e[i]); } } eg{e};
-  HIPOK(hipEventRecord(e[0], st));
+  struct EvGuard { hipEvent_t* e; ~EvGuard() { for (int i = 0; i < 7; ++i) (void)hipEventDestroy(e[i]); } } eg{e};
 
   SeedArgs sa{};
   sa.r = dr.view();
@@ -343,7 +345,9 @@ static int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int 
   HIPOK(hipMemsetAsync(P.seg_count.p, 0, (size_t)(n_rs + 1) * 8, st));
   HIPOK(hipMemsetAsync(P.n_overflow.p, 0, 4, st));
   sa.seg_count = P.seg_count.as<int64_t>();
-  HIPOK(launch_seed_count(sa, st));
+  HIPOK(hipEventRecord(e[0], st));
+  HIPOK(launch_seed_count(sa, st));   // the index lookups alone: first[key], first[key + 1]
+  HIPOK(hipEventRecord(e[6], st));
   size_t tmpb = 0;
   HIPOK(scan_exclusive_i64(nullptr, nullptr, n_rs + 1, nullptr, tmpb, st));
   size_t tmpb2 = 0;
@@ -496,6 +500,16 @@ static int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int 
   HIPOK(launch_replay(pa, st));
   HIPOK(hipEventRecord(e[4], st));
   // tracebacks still referenced by a slot go to the host store
+  if (!m->want_tb) {
+    HIPOK(hipEventRecord(e[5], st));
+    HIPOK(hipEventSynchronize(e[5]));
+    stats.ms_seed += elapsed(m, e[0], e[1]);
+    stats.ms_lookup += elapsed(m, e[0], e[6]);
+    stats.ms_filter += elapsed(m, e[1], e[2]);
+    stats.ms_align += elapsed(m, e[2], e[3]);
+    stats.ms_replay += elapsed(m, e[3], e[4]);
+    return STORM_OK;
+  }
   const int64_t n_slots = 4 * dr.n;
   HIPOK(P.mark_rec.ensure(rb * 4));
   HIPOK(P.mark_slot.ensure(rb * 4));
@@ -523,6 +537,7 @@ static int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int 
   HIPOK(hipEventRecord(e[5], st));
   HIPOK(hipEventSynchronize(e[5]));
   stats.ms_seed += elapsed(m, e[0], e[1]);
+  stats.ms_lookup += elapsed(m, e[0], e[6]);
   stats.ms_filter += elapsed(m, e[1], e[2]);
   stats.ms_align += elapsed(m, e[2], e[3]);
   stats.ms_replay += elapsed(m, e[3], e[4]);
@@ -551,7 +566,7 @@ static int assemble_slots(storm_mapper* m, const std::vector<Slot>& hs, const st
       h.score = s.score; h.ref_id = s.ref_id; h.ref_start = s.ref_start; h.ref_end = s.ref_end;
       h.read_start = s.read_start; h.read_end = s.read_end; h.sense = s.sense;
       h.tb_len = s.tb_len;
-      if (s.ref_id == -1 || k >= m->p.max_hits) continue;
+      if (s.ref_id == -1 || k >= m->p.max_hits || !m->want_tb) continue;
       if (s.pass == PASS_FUSED) {
         if (!cells) return fail(STORM_ERR_DEVICE, "traceback of a fused hit was not gathered");
         auto it = cells->find(4 * i + k);
@@ -630,6 +645,7 @@ static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode,
     st_out->gpu_ms_filter = ps.ms_filter;
     st_out->gpu_ms_align = ps.ms_align;
     st_out->gpu_ms_replay = ps.ms_replay;
+    st_out->gpu_ms_lookup = ps.ms_lookup;
   }
   return STORM_OK;
 }
@@ -657,6 +673,7 @@ static int ensure_fused_buffers(storm_mapper* m, int64_t n, int64_t stride, int 
     const int g = std::max(grid, F.grid);
     HIPOK(F.scratch.ensure((size_t)g * 64 * st));
     HIPOK(F.tbbuf.ensure((size_t)g * FUSED_PCAP * TBB));
+    HIPOK(F.work.ensure((size_t)g * FUSED_WORK_BYTES));
     F.scratch_stride = st;
     F.grid = g;
   }
@@ -683,6 +700,7 @@ static FusedArgs fused_args(storm_mapper* m, size_t ref, int group) {
   a.scratch = F.scratch.as<uint8_t>();
   a.scratch_stride = F.scratch_stride;
   a.tbbuf = F.tbbuf.as<uint8_t>();
+  a.work = F.work.as<uint8_t>();
   a.ref_id = (int)ref;
   a.sense_mask = m->p.sense_mask;
   a.stats = F.stats.as<unsigned long long>();
@@ -802,6 +820,7 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
   PassStats ps;
   int pass_id = 0;
   unsigned long long totals[4] = {0, 0, 0, 0};
+  int64_t n_fallback = 0;
   std::vector<uint8_t> ovf((size_t)n);
   for (size_t r = 0; r < m->refs.size(); ++r) {
     rc = ensure_ref_resident(m, r);
@@ -816,7 +835,7 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
     HIPOK(hipMemcpyAsync(F.n_dev.p, &n32, 4, hipMemcpyHostToDevice, m->st));
     // one graph per reference: reset + fused kernel, replayed for every batch with the same buffers
     FusedArgs a = fused_args(m, r, tune.group);
-    std::vector<const void*> sig = {a.g.lo, a.r.planes, a.r.qual, a.r.len, a.slots, a.cells, a.scratch, a.tbbuf,
+    std::vector<const void*> sig = {a.g.lo, a.r.planes, a.r.qual, a.r.len, a.slots, a.cells, a.scratch, a.tbbuf, a.work,
                                     a.seeds.s[0].first, (const void*)(intptr_t)tune.group, (const void*)(intptr_t)tune.grid,
                                     (const void*)(intptr_t)a.scratch_stride};
     if (!F.graphs[r] || F.graph_sig[r] != sig) {
@@ -852,6 +871,7 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
     for (int q = 0; q < 4; ++q) totals[q] += st3[q];
     std::vector<int64_t> over;
     for (int64_t i = 0; i < n; ++i) if (ovf[(size_t)i]) over.push_back(i);
+    n_fallback += (int64_t)over.size();
     if (!over.empty()) {
       rc = staged_subset(m, v, order, over, (int)r, pass_id, store, ps);
       if (rc) return rc;
@@ -859,13 +879,15 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
     if (m->refs.size() > 1) { release_ref(m, r); if (F.graphs[r]) { (void)hipGraphExecDestroy(F.graphs[r]); F.graphs[r] = nullptr; } }
   }
   // tracebacks of the fused slots that will be written (ranks < -M)
-  HIPOK(launch_fused_reset(F.gcount.as<int32_t>(), nullptr, m->st));
-  HIPOK(launch_gather_cells(F.slots.as<Slot>(), 4 * n, m->p.max_hits, F.cells.as<uint8_t>(), F.gids.as<int32_t>(),
-                            F.gtb.as<uint8_t>(), F.gcount.as<int32_t>(), m->st));
   int32_t ng = 0;
   std::vector<Slot> hs((size_t)(4 * n));
+  if (m->want_tb) {
+    HIPOK(launch_fused_reset(F.gcount.as<int32_t>(), nullptr, m->st));
+    HIPOK(launch_gather_cells(F.slots.as<Slot>(), 4 * n, m->p.max_hits, F.cells.as<uint8_t>(), F.gids.as<int32_t>(),
+                              F.gtb.as<uint8_t>(), F.gcount.as<int32_t>(), m->st));
+    HIPOK(hipMemcpyAsync(&ng, F.gcount.p, 4, hipMemcpyDeviceToHost, m->st));
+  }
   HIPOK(hipMemcpyAsync(hs.data(), F.slots.p, hs.size() * sizeof(Slot), hipMemcpyDeviceToHost, m->st));
-  HIPOK(hipMemcpyAsync(&ng, F.gcount.p, 4, hipMemcpyDeviceToHost, m->st));
   HIPOK(hipStreamSynchronize(m->st));
   std::vector<int32_t> gids((size_t)ng);
   std::vector<uint8_t> gtb((size_t)ng * TBB);
@@ -887,12 +909,14 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
     st_out->n_filter_pass = (int64_t)totals[1] + ps.rec;
     st_out->n_accepted = (int64_t)totals[2];
     st_out->n_hits = (int64_t)totals[3] + ps.hits;
+    st_out->n_fallback = n_fallback;
     st_out->seconds_index = m->index_seconds - ti;
     st_out->seconds_map = now_s() - t0 - st_out->seconds_index;
     st_out->gpu_ms_seed = ps.ms_seed;
     st_out->gpu_ms_filter = ps.ms_filter;
     st_out->gpu_ms_align = ps.ms_align;
     st_out->gpu_ms_replay = ps.ms_replay;
+    st_out->gpu_ms_lookup = ps.ms_lookup;
   }
   return STORM_OK;
 }
@@ -1043,7 +1067,9 @@ int storm_mapper_map_hits(storm_mapper* m, const storm_reads_view* reads, int mo
   HIPOK(hipSetDevice(m->device));
   std::vector<HostSlot> slots;
   storm_map_stats local{};
+  m->want_tb = false;
   int rc = map_view(m, reads, mode, slots, &local);
+  m->want_tb = true;
   if (rc) return rc;
   for (size_t k = 0; k < slots.size(); ++k) {
     if (score) score[k] = slots[k].score;

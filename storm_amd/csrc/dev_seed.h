@@ -56,11 +56,14 @@ __device__ __forceinline__ void wave_bitonic_sort(int32_t* buf, int n) {
 }
 
 // All candidates of read strand rs, ascending, into out[0..), one wavefront.  hbuf is an LDS
-// buffer of `hcap` entries (hcap >= 64, power of two).  Returns the number of candidates, or -1
-// when the raw hits exceed hcap or the candidates exceed `cap` (the caller sends the read to the
-// staged path, which has no such limits).
+// buffer of `hcap` entries (hcap >= 64, power of two), lk an LDS buffer of 192 ints.  The raw hits
+// of 64 lookups are gathered cooperatively: hit e of the batch goes to lane e mod 64, which finds
+// its bucket by binary search over the lookups' inclusive counts, so consecutive lanes read
+// consecutive entries of a bucket.  Returns the number of candidates, or -1 when the raw hits
+// exceed hcap or the candidates exceed `cap` (the caller sends the read to the staged path, which
+// has no such limits).  *raw_hits receives the bucket entries read (the lookups' total count).
 __device__ int wave_candidates(const ReadsDev& r, const SeedSetDev& seeds, int64_t rs, int I, int32_t* hbuf, int hcap,
-                               int32_t* out, int cap, int* raw_hits) {
+                               int32_t* lk, int32_t* out, int cap, int* raw_hits) {
   const int lane = threadIdx.x & 63;
   const int L = r.len[rs >> 1];
   const uint32_t* pl = read_planes(r, rs);
@@ -79,18 +82,29 @@ __device__ int wave_candidates(const ReadsDev& r, const SeedSetDev& seeds, int64
         c = sd.first[key + 1] - f0;
       }
       const int incl = wave_scan_incl_i32(c);
-      const int off = run + incl - c;
-      if (off + c <= hcap) {
-        for (int e = 0; e < c; ++e) hbuf[off + e] = sd.map[f0 + e] - rp;
+      const int tot = __shfl(incl, 63, 64);
+      if (!over && run + tot <= hcap) {
+        lk[lane] = f0 - (incl - c);   // bucket entry of batch hit e is lk[q] + e
+        lk[64 + lane] = rp;
+        lk[128 + lane] = incl;
+        __syncthreads();
+        for (int e = lane; e < tot; e += 64) {
+          int q = 0;   // smallest lookup q with incl[q] > e
+#pragma unroll
+          for (int step = 32; step > 0; step >>= 1)
+            if (lk[128 + q + step - 1] <= e) q += step;
+          hbuf[run + e] = sd.map[lk[q] + e] - lk[64 + q];
+        }
+        __syncthreads();
+      } else {
+        over = true;
       }
-      run += __shfl(incl, 63, 64);
-      if (run > hcap) over = true;
+      run += tot;
     }
   }
   *raw_hits = run;
   if (over) return -1;
   if (run == 0) return 0;
-  __syncthreads();
   wave_bitonic_sort(hbuf, run);
   int placed = 0;
   for (int e0 = 0; e0 < run; e0 += 64) {

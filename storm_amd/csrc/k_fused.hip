@@ -3,11 +3,16 @@
 // Same semantics as the staged kernels (k_seed, k_filter, k_align, k_replay; run.c:555-747,
 // hit_map.c:50-144), in one launch.  Workgroups are single wavefronts that pull tasks from a
 // device counter until the batch is exhausted (every wave reaches the exit).  A task is a group
-// of G consecutive reads (both strands): its candidates, filter flags and DP records stay in
-// LDS, so a group of reads keeps all 64 lanes busy even though one read strand has only a few
-// candidates.  Reads of any length share a task; one launch serves one band class (W, I).
-// A read whose raw hits, candidates or filter passes exceed the LDS capacities is flagged in
-// `overflow` and left untouched; the host sends it through the staged path.
+// of G consecutive reads (both strands).  Their candidates accumulate in the workgroup's own
+// slice of a global work buffer (diagonal, read strand, filter flag, record index), which stays
+// in L2; when the next read might not fit, the accumulated candidates are flushed: filtered two
+// per lane, the passes aligned one per lane, and the hit maps of the completed reads replayed
+// one lane per read.  So a group of reads keeps all 64 lanes busy when one read strand has only a
+// few candidates, and a read with thousands of candidates (a large genome) fills the lanes by
+// itself.  Reads of any length share a task; one launch serves one band class (W, I).
+// A read whose raw hits exceed the LDS sort buffer, or whose flush has more passes than the
+// record buffer, is flagged in `overflow` and left untouched; the host sends it through the
+// staged path.
 #include "dev_align.h"
 #include "dev_filter.h"
 #include "dev_seed.h"
@@ -15,25 +20,21 @@
 
 namespace storm {
 
-constexpr int F_HCAP = 1024;   // raw hits of one read strand
-constexpr int F_CCAP = 768;    // candidates of one task
-constexpr int F_PCAP = FUSED_PCAP;   // filter passes (DP records) of one task
-constexpr int F_GMAX = 64;     // reads per task
-
+constexpr int F_HCAP = 2048;         // raw hits of one read strand (LDS sort buffer)
+constexpr int F_CCAP = FUSED_CCAP;   // candidates per flush (work buffer)
+constexpr int F_PCAP = FUSED_PCAP;   // filter passes (DP records) per flush
+constexpr int F_GMAX = 64;           // reads per task
+static_assert(F_CCAP >= 2 * F_HCAP, "a flush must hold both strands of one read");
 
 template <int W, int IM, bool COLOR>
-__global__ __launch_bounds__(64) void k_fused(FusedArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 4 : 2))) void k_fused(FusedArgs a) {
   __shared__ int32_t hbuf[F_HCAP];
-  __shared__ int32_t cdiag[F_CCAP];
-  __shared__ uint8_t crs[F_CCAP];
-  __shared__ uint8_t cpass[F_CCAP];
-  __shared__ int16_t crec[F_CCAP];
-  __shared__ int16_t cstart[2 * F_GMAX + 1];
-  __shared__ uint8_t rovf[F_GMAX];
+  __shared__ int32_t lk[192];
+  __shared__ int32_t cstart[2 * F_GMAX + 1];
   __shared__ int32_t rawh[2 * F_GMAX];
-  __shared__ AlignRec recs[F_PCAP];
-  __shared__ int16_t rec2cand[F_PCAP];
+  __shared__ uint8_t rovf[F_GMAX];
   __shared__ int32_t task_sh;
+  __shared__ unsigned long long stl[4];   // candidates, filter passes, accepted, raw hits
 
   const int lane = threadIdx.x;
   const int n = *a.n_reads;
@@ -42,41 +43,21 @@ __global__ __launch_bounds__(64) void k_fused(FusedArgs a) {
   const FilterConst fk = filter_const(a.sc.fmatch, a.sc.fmismatch, a.sc.fopen, a.sc.fext, a.sc.fthr);
   uint8_t* scratch = a.scratch + (int64_t)blockIdx.x * 64 * a.scratch_stride + (int64_t)lane * a.scratch_stride;
   uint8_t* tbbuf = a.tbbuf + (int64_t)blockIdx.x * F_PCAP * TBB;
-  unsigned long long st_cand = 0, st_pass = 0, st_acc = 0, st_hits = 0;
+  uint8_t* wb = a.work + (int64_t)blockIdx.x * FUSED_WORK_BYTES;
+  int32_t* cdiag = (int32_t*)wb;
+  int32_t* crec = cdiag + F_CCAP;
+  AlignRec* recs = (AlignRec*)(crec + F_CCAP);
+  int32_t* rec2cand = (int32_t*)(recs + F_PCAP);
+  uint8_t* crs = (uint8_t*)(rec2cand + F_PCAP);
+  uint8_t* cpass = crs + F_CCAP;
+  if (lane < 4) stl[lane] = 0;
 
-  for (;;) {
-    if (lane == 0) task_sh = atomicAdd(a.task_counter, 1);
+  int64_t base = 0;
+  // filter, DP and hit-map replay of the nc candidates of reads [f0, f1) of the task
+  auto flush = [&](int f0, int f1, int nc) {
+    if (lane == 0) cstart[2 * f1] = nc;
     __syncthreads();
-    const int task = task_sh;
-    __syncthreads();
-    const int64_t base = (int64_t)task * G;
-    if (base >= n) break;
-    const int nr = (n - base) < G ? (int)(n - base) : G;
-    for (int j = lane; j < nr; j += 64) rovf[j] = 0;
-    __syncthreads();
-
-    // 1. candidates of every read strand of the task (stages a + b)
-    int nc = 0;
-    for (int j = 0; j < 2 * nr; ++j) {
-      if (lane == 0) { cstart[j] = (int16_t)nc; rawh[j] = 0; }
-      const bool wanted = ((a.sense_mask >> (j & 1)) & 1) && !rovf[j >> 1];
-      if (!wanted) continue;
-      const int64_t rs = 2 * base + j;
-      int raw = 0;
-      const int cnt = wave_candidates(a.r, a.seeds, rs, I, hbuf, F_HCAP, cdiag + nc, F_CCAP - nc, &raw);
-      if (lane == 0) rawh[j] = raw;
-      if (cnt < 0) {
-        if (lane == 0) rovf[j >> 1] = 1;
-        __syncthreads();
-        continue;
-      }
-      for (int e = lane; e < cnt; e += 64) crs[nc + e] = (uint8_t)j;
-      nc += cnt;
-    }
-    if (lane == 0) cstart[2 * nr] = (int16_t)nc;
-    __syncthreads();
-
-    // 2. filter (stage c): two candidates per lane
+    // filter (stage c): two candidates per lane
     for (int b0 = 0; b0 < nc; b0 += 128) {
       const int c0 = b0 + 2 * lane;
       if (c0 < nc) {
@@ -91,8 +72,7 @@ __global__ __launch_bounds__(64) void k_fused(FusedArgs a) {
       }
     }
     __syncthreads();
-
-    // 3. DP + traceback (stage d) for the passes, in candidate order
+    // record index of every pass, in candidate order
     int np = 0;
     for (int b0 = 0; b0 < nc; b0 += 64) {
       const int c = b0 + lane;
@@ -101,37 +81,34 @@ __global__ __launch_bounds__(64) void k_fused(FusedArgs a) {
       const int incl = wave_scan_incl_i32(p);
       if (c < nc) {
         const int k = p ? np + incl - 1 : -1;
-        crec[c] = (int16_t)k;
-        if (k >= 0 && k < F_PCAP) rec2cand[k] = (int16_t)c;
+        crec[c] = k;
+        if (k >= 0 && k < F_PCAP) rec2cand[k] = c;
       }
       np += __shfl(incl, 63, 64);
     }
     __syncthreads();
-    if (np > F_PCAP) {   // too many passes for LDS: every read of the task that has one goes staged
+    if (np > F_PCAP) {   // too many passes for the record buffer: their reads go staged
       for (int c = lane; c < nc; c += 64)
         if (crec[c] >= F_PCAP) rovf[crs[c] >> 1] = 1;
       __syncthreads();
     }
-    st_cand += (lane == 0) ? (unsigned long long)nc : 0ull;
-    st_pass += (lane == 0) ? (unsigned long long)np : 0ull;
-    {
-      const int npl = np < F_PCAP ? np : F_PCAP;
-      // lane l computes records l, l + 64, ...
-      for (int k = lane; k < npl; k += 64) {
-        const int c = rec2cand[k];
-        if (rovf[crs[c] >> 1]) continue;
-        const int64_t rs = 2 * base + crs[c];
-        recs[k] = dp_one<IM, COLOR>(a.g, a.r, a.sc, (int)rs, cdiag[c], scratch, tbbuf + (int64_t)k * TBB);
-      }
+    if (lane == 0) { stl[0] += (unsigned long long)nc; stl[1] += (unsigned long long)np; }
+    // DP + traceback (stage d): lane l computes records l, l + 64, ...
+    const int npl = np < F_PCAP ? np : F_PCAP;
+    for (int k = lane; k < npl; k += 64) {
+      const int c = rec2cand[k];
+      if (rovf[crs[c] >> 1]) continue;
+      const int64_t rs = 2 * base + crs[c];
+      recs[k] = dp_one<IM, COLOR>(a.g, a.r, a.sc, (int)rs, cdiag[c], scratch, tbbuf + (int64_t)k * TBB);
     }
     __syncthreads();
-
-    // 4. hit map replay (run.c:660-697, hit_map.c:50-144): one lane per read
-    for (int lr = lane; lr < nr; lr += 64) {
+    // hit map replay (run.c:660-697, hit_map.c:50-144): one lane per read
+    int st_acc = 0, st_hits = 0;
+    for (int lr = f0 + lane; lr < f1; lr += 64) {
       const int64_t read = base + lr;
       if (rovf[lr]) { a.overflow[read] = 1; continue; }
       a.overflow[read] = 0;
-      st_hits += (unsigned long long)(rawh[2 * lr] + rawh[2 * lr + 1]);
+      st_hits += rawh[2 * lr] + rawh[2 * lr + 1];
       Slot s[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k) s[k] = a.slots[read * 4 + k];
@@ -150,29 +127,48 @@ __global__ __launch_bounds__(64) void k_fused(FusedArgs a) {
           const int score = x.score;
           const int ref_start = x.win + x.best_j0, ref_end = x.win + x.best_j;
           const int read_start = x.best_i0, read_end = L - 1;
-          int k;
-          bool drop = false;
-          for (k = 0; k < 4 && s[k].ref_id != -1; ++k) {
-            if (s[k].ref_id == a.ref_id &&
-                ((s[k].ref_start == ref_start && s[k].read_start == read_start) ||
-                 (s[k].ref_end == ref_end && s[k].read_end == read_end))) {
-              if (score > s[k].score) break;
-              drop = true;
-              break;
+          // hit_map__update with compile-time slot indices only (the slots stay in registers)
+          int k = 4;
+          bool found = false, drop = false;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            if (!found) {
+              if (s[q].ref_id == -1) {
+                k = q;
+                found = true;
+              } else if (s[q].ref_id == a.ref_id &&
+                         ((s[q].ref_start == ref_start && s[q].read_start == read_start) ||
+                          (s[q].ref_end == ref_end && s[q].read_end == read_end))) {
+                k = q;
+                found = true;
+                drop = !(score > s[q].score);
+              }
             }
           }
           if (!drop) {
             if (k == 4) k = 3;
-            int i = k;
-            while (i >= 0 && score > s[i].score) --i;
+            int i = k;   // i = k; while (i >= 0 && score > s[i].score) --i;
+            bool go = true;
+#pragma unroll
+            for (int q = 3; q >= 0; --q) {
+              if (go && q <= k) {
+                if (score > s[q].score) i = q - 1;
+                else go = false;
+              }
+            }
             if (i != 3) {
-              const int32_t cell = s[k].cell;
-              for (int jj = k; jj > i + 1; --jj) s[jj] = s[jj - 1];
+              const int p = i + 1;
+              int32_t cell = 0;
+#pragma unroll
+              for (int q = 0; q < 4; ++q) if (q == k) cell = s[q].cell;
+#pragma unroll
+              for (int q = 3; q >= 1; --q) if (q > p && q <= k) s[q] = s[q - 1];
               Slot nw;
               nw.score = score; nw.ref_id = a.ref_id; nw.ref_start = ref_start; nw.ref_end = ref_end;
               nw.read_start = read_start; nw.read_end = read_end; nw.sense = strand ? -1 : 1;
               nw.pass = PASS_FUSED; nw.rec = cell; nw.tb_len = x.tb_len; nw.cell = cell;
-              s[i + 1] = nw;
+#pragma unroll
+              for (int q = 0; q < 4; ++q) if (q == p) s[q] = nw;
               dirty = true;
               // the traceback goes into the freed cell
               const uint32_t* src = (const uint32_t*)(tbbuf + (int64_t)kr * TBB);
@@ -188,13 +184,55 @@ __global__ __launch_bounds__(64) void k_fused(FusedArgs a) {
         for (int k = 0; k < 4; ++k) a.slots[read * 4 + k] = s[k];
       }
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      st_acc += __shfl_xor(st_acc, o, 64);
+      st_hits += __shfl_xor(st_hits, o, 64);
+    }
+    if (lane == 0) { stl[2] += (unsigned long long)st_acc; stl[3] += (unsigned long long)st_hits; }
     __syncthreads();
+  };
+
+  for (;;) {
+    if (lane == 0) task_sh = atomicAdd(a.task_counter, 1);
+    __syncthreads();
+    const int task = task_sh;
+    __syncthreads();
+    base = (int64_t)task * G;
+    if (base >= n) break;
+    const int nr = (n - base) < G ? (int)(n - base) : G;
+    for (int j = lane; j < nr; j += 64) rovf[j] = 0;
+    __syncthreads();
+
+    // candidates of every read strand (stages a + b), flushed whenever the next read might not
+    // fit and after the last read (one call site keeps the flush inlined once)
+    int nc = 0, f0 = 0;
+    for (int lr = 0;; ++lr) {
+      if (lr == nr || nc + 2 * F_HCAP > F_CCAP) {
+        flush(f0, lr, nc);
+        nc = 0;
+        f0 = lr;
+        if (lr == nr) break;
+      }
+      for (int strand = 0; strand < 2; ++strand) {
+        const int j = 2 * lr + strand;
+        if (lane == 0) { cstart[j] = nc; rawh[j] = 0; }
+        const bool wanted = ((a.sense_mask >> strand) & 1) && !rovf[lr];
+        if (!wanted) continue;
+        int raw = 0;
+        const int cnt = wave_candidates(a.r, a.seeds, 2 * base + j, I, hbuf, F_HCAP, lk, cdiag + nc, F_CCAP - nc, &raw);
+        if (lane == 0) rawh[j] = raw;
+        if (cnt < 0) {
+          if (lane == 0) rovf[lr] = 1;
+          __syncthreads();
+          continue;
+        }
+        for (int e = lane; e < cnt; e += 64) crs[nc + e] = (uint8_t)j;
+        nc += cnt;
+      }
+    }
   }
-  if (a.stats) {
-    if (lane == 0) { atomicAdd(&a.stats[0], st_cand); atomicAdd(&a.stats[1], st_pass); }
-    atomicAdd(&a.stats[2], st_acc);
-    atomicAdd(&a.stats[3], st_hits);
-  }
+  if (a.stats && lane < 4) atomicAdd(&a.stats[lane], stl[lane]);
 }
 
 // clears the task counter and the stats inside a captured graph (no memset nodes)

@@ -130,13 +130,19 @@ struct FusedArgs {
   uint8_t* overflow;         // per read
   uint8_t* scratch;          // per workgroup: 64 * scratch_stride direction bytes
   int64_t scratch_stride;
-  uint8_t* tbbuf;            // per workgroup: F_PCAP * TBB traceback bytes of the current task
+  uint8_t* tbbuf;            // per workgroup: FUSED_PCAP * TBB traceback bytes of the current flush
+  uint8_t* work;             // per workgroup: FUSED_WORK_BYTES of candidates and records
   int ref_id;
   int sense_mask;
   unsigned long long* stats; // [0] candidates, [1] filter passes, [2] accepted, [3] raw hits
 };
 
-constexpr int FUSED_PCAP = 192;   // DP records per task held in LDS (k_fused.hip F_PCAP)
+constexpr int FUSED_PCAP = 1024;   // DP records per flush of k_fused
+constexpr int FUSED_CCAP = 8192;   // candidates per flush of k_fused
+// per-workgroup work buffer of k_fused: diagonal + record index (int32) per candidate, AlignRec
+// + candidate index per record, read strand + filter flag (u8) per candidate
+constexpr int64_t FUSED_WORK_BYTES =
+    ((8LL * FUSED_CCAP + ((int64_t)sizeof(AlignRec) + 4) * FUSED_PCAP + 2LL * FUSED_CCAP) + 255) & ~255LL;
 
 hipError_t launch_fused(const FusedArgs& a, int grid, hipStream_t st);
 hipError_t launch_fused_reset(int32_t* counter, unsigned long long* stats, hipStream_t st);

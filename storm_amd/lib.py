@@ -31,9 +31,10 @@ class Params(C.Structure):
 
 class MapStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in ("n_reads", "n_mapped", "n_candidates", "n_filter_pass", "n_accepted",
-                                          "n_lookups", "n_hits")] + \
+                                          "n_lookups", "n_hits", "n_fallback")] + \
                [(n, C.c_double) for n in ("seconds_load", "seconds_index", "seconds_map", "seconds_sam",
-                                          "gpu_ms_seed", "gpu_ms_filter", "gpu_ms_align", "gpu_ms_replay")]
+                                          "gpu_ms_seed", "gpu_ms_filter", "gpu_ms_align", "gpu_ms_replay",
+                                          "gpu_ms_lookup")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -256,10 +256,33 @@ class PackedReads:
     strand: np.ndarray | None = None
 
 
+def _chunk_rngs(seed: int, n_chunks: int):
+    return [np.random.Generator(np.random.PCG64(ss)) for ss in np.random.SeedSequence(seed).spawn(n_chunks)]
+
+
+def _reverse_rows(x: np.ndarray, lens: np.ndarray, extra: int, complement: bool) -> np.ndarray:
+    """Row-wise reversal of the first lens[i] + extra codes (complemented when asked)."""
+    out = np.zeros_like(x)
+    for Lv in np.unique(lens):
+        sel = lens == Lv
+        seg = x[sel, :Lv + extra][:, ::-1]
+        out[sel, :Lv + extra] = (3 - seg) if complement else seg
+    return out
+
+
+def _run_chunks(fn, n: int, chunk: int, seed: int, workers: int):
+    from concurrent.futures import ThreadPoolExecutor
+    starts = list(range(0, n, chunk))
+    gens = _chunk_rngs(seed, len(starts))
+    with ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
+        list(ex.map(lambda k: fn(starts[k], min(n, starts[k] + chunk), gens[k]), range(len(starts))))
+
+
 def fast_nucleotide_reads(genome: np.ndarray, n: int, L, seed: int, sub: float, indel: float, qlo: int, qhi: int,
-                          chunk: int = 500_000) -> PackedReads:
+                          chunk: int = 250_000, workers: int = 8) -> PackedReads:
     """Vectorised Illumina-like reads: uniform position and strand, substitutions, and for a
-    fraction n*L*indel of the reads one 1-3 bp insertion or deletion at a uniform offset."""
+    fraction n*L*indel of the reads one 1-3 bp insertion or deletion at a uniform offset.
+    Chunks are generated by a thread pool, each chunk from its own spawned generator."""
     g = rng(seed)
     G = len(genome)
     lens = np.full(n, L, np.int32) if np.isscalar(L) else g.integers(L[0], L[1] + 1, size=n).astype(np.int32)
@@ -269,44 +292,42 @@ def fast_nucleotide_reads(genome: np.ndarray, n: int, L, seed: int, sub: float, 
     qual = np.zeros((n, stride), np.uint8)
     origin = g.integers(0, G - Lmax - 8, size=n)
     strand = g.integers(0, 2, size=n).astype(np.uint8)
-    for s in range(0, n, chunk):
-        e = min(n, s + chunk)
+    col = np.arange(Lmax)[None, :]
+
+    def one(s, e, g):
         m = e - s
-        idx = origin[s:e, None] + np.arange(Lmax + 4)[None, :]
-        r = genome[idx]
-        has = g.random(m) < (lens[s:e] * indel)
-        for k in np.nonzero(has)[0]:
-            Lk = int(lens[s + k])
-            off = int(g.integers(1, Lk - 1))
-            ln = int(g.integers(1, 4))
-            row = r[k].copy()
-            if g.random() < 0.5:   # insertion in the read
-                row = np.concatenate([row[:off], g.integers(0, 4, size=ln, dtype=np.uint8), row[off:]])[:Lmax + 4]
-            else:                  # deletion from the read
-                row = np.concatenate([row[:off], row[off + ln:], g.integers(0, 4, size=ln, dtype=np.uint8)])[:Lmax + 4]
-            r[k] = row
+        r = genome[origin[s:e, None] + np.arange(Lmax + 4)[None, :]]
+        has = np.nonzero(g.random(m) < (lens[s:e] * indel))[0]
+        if len(has):
+            Lk = lens[s:e][has].astype(np.int64)
+            off = g.integers(1, Lk - 1)[:, None]
+            ln = g.integers(1, 4, size=len(has))[:, None]
+            ins = (g.random(len(has)) < 0.5)[:, None]
+            src = np.where(col < off, col, np.where(ins, col - ln, col + ln))
+            rows = np.take_along_axis(r[has], np.clip(src, 0, Lmax + 3), axis=1)
+            fill = ins & (col >= off) & (col < off + ln)
+            rows = np.where(fill, g.integers(0, 4, size=rows.shape, dtype=np.uint8), rows)
+            r[has, :Lmax] = rows
         r = r[:, :Lmax]
-        mut = g.random(r.shape) < sub
+        mut = g.random(r.shape, dtype=np.float32) < sub
         r = np.where(mut, (r + g.integers(1, 4, size=r.shape, dtype=np.uint8)) & 3, r).astype(np.uint8)
         rev = strand[s:e] == 1
         if rev.any():
-            rr = r[rev]
-            Ls = lens[s:e][rev]
-            out = np.zeros_like(rr)
-            for Lv in np.unique(Ls):
-                sel = Ls == Lv
-                out[sel, :Lv] = 3 - rr[sel, :Lv][:, ::-1]
-            r[rev] = out
-        col = np.arange(Lmax)[None, :]
+            r[rev] = _reverse_rows(r[rev], lens[s:e][rev], 0, True)
         r = np.where(col < lens[s:e, None], r, 0).astype(np.uint8)
         codes[s:e] = pack_rows(r, stride)
-        q = g.integers(qlo, qhi + 1, size=(m, Lmax))
-        qual[s:e] = pack_rows(np.where(col < lens[s:e, None], quality_levels(q), 0).astype(np.uint8), stride)
+        if qlo > 10:   # every quality maps to level 3 (read_quality.c:10-16)
+            lv = np.full((m, Lmax), 3, np.uint8)
+        else:
+            lv = quality_levels(g.integers(qlo, qhi + 1, size=(m, Lmax)))
+        qual[s:e] = pack_rows(np.where(col < lens[s:e, None], lv, 0).astype(np.uint8), stride)
+
+    _run_chunks(one, n, chunk, seed, workers)
     return PackedReads(codes, qual, lens, None, None, origin, strand)
 
 
 def fast_color_reads(genome: np.ndarray, n: int, L, seed: int, snp: float, color_err: float, qlo: int, qhi: int,
-                     chunk: int = 500_000) -> PackedReads:
+                     chunk: int = 250_000, workers: int = 8) -> PackedReads:
     """Vectorised SOLiD-like reads: L colors covering L+1 bases; SNPs in base space, then color
     errors; qualities uniform in [qlo, qhi] for the primer-side color and the L colors."""
     g = rng(seed)
@@ -321,32 +342,26 @@ def fast_color_reads(genome: np.ndarray, n: int, L, seed: int, snp: float, color
     origin = g.integers(0, G - Lmax - 4, size=n)
     strand = g.integers(0, 2, size=n).astype(np.uint8)
     col = np.arange(Lmax)[None, :]
-    for s in range(0, n, chunk):
-        e = min(n, s + chunk)
+
+    def one(s, e, g):
         m = e - s
-        idx = origin[s:e, None] + np.arange(Lmax + 1)[None, :]
-        b = genome[idx]
-        mut = g.random(b.shape) < snp
+        b = genome[origin[s:e, None] + np.arange(Lmax + 1)[None, :]]
+        mut = g.random(b.shape, dtype=np.float32) < snp
         b = np.where(mut, (b + g.integers(1, 4, size=b.shape, dtype=np.uint8)) & 3, b).astype(np.uint8)
         rev = strand[s:e] == 1
         if rev.any():
-            bb = b[rev]
-            Ls = lens[s:e][rev]
-            out = np.zeros_like(bb)
-            for Lv in np.unique(Ls):
-                sel = Ls == Lv
-                out[sel, :Lv + 1] = 3 - bb[sel, :Lv + 1][:, ::-1]
-            b[rev] = out
+            b[rev] = _reverse_rows(b[rev], lens[s:e][rev], 1, True)
         c = (b[:, 1:] ^ b[:, :-1]).astype(np.uint8)
-        err = g.random(c.shape) < color_err
+        err = g.random(c.shape, dtype=np.float32) < color_err
         c = np.where(err, c ^ g.integers(1, 4, size=c.shape, dtype=np.uint8), c).astype(np.uint8)
         c = np.where(col < lens[s:e, None], c, 0).astype(np.uint8)
         codes[s:e] = pack_rows(c, stride)
         first_base[s:e] = b[:, 0]
-        q = g.integers(qlo, qhi + 1, size=(m, Lmax + 1))
-        lv = quality_levels(q)
+        lv = quality_levels(g.integers(qlo, qhi + 1, size=(m, Lmax + 1)))
         first_qual[s:e] = lv[:, 0]
         qual[s:e] = pack_rows(np.where(col < lens[s:e, None], lv[:, 1:], 0).astype(np.uint8), stride)
+
+    _run_chunks(one, n, chunk, seed, workers)
     return PackedReads(codes, qual, lens, first_base, first_qual, origin, strand)
 
 
@@ -362,3 +377,32 @@ def write_fasta_fast(path: str, name: str, codes: np.ndarray, width: int = 60) -
             f.write(block.tobytes())
         if n % width:
             f.write(letters[full * width:].tobytes() + b"\n")
+
+
+def unpack_rows(packed: np.ndarray, L: int) -> np.ndarray:
+    """Inverse of pack_rows: (n, nbytes) -> (n, L) codes."""
+    p = packed[:, :(L + 3) // 4]
+    out = np.stack([p & 3, (p >> 2) & 3, (p >> 4) & 3, (p >> 6) & 3], axis=2).reshape(len(p), -1)
+    return out[:, :L].astype(np.uint8)
+
+
+LEVEL_QUALITY = np.array([2, 7, 10, 40], np.int16)   # one quality value per level (read_quality.c:10-16)
+
+
+def packed_to_simreads(pr: PackedReads, idx) -> SimReads:
+    """The reads `idx` of a PackedReads as SimReads (for the FASTQ / csfasta writers); each
+    quality is written as a value of its level, which is all the mapping path reads of it."""
+    idx = np.asarray(idx)
+    Lmax = int(pr.len[idx].max())
+    codes = unpack_rows(pr.codes[idx], Lmax)
+    lv = unpack_rows(pr.qual[idx], Lmax)
+    seqs, quals = [], []
+    for k, i in enumerate(idx):
+        L = int(pr.len[i])
+        seqs.append(codes[k, :L].copy())
+        q = LEVEL_QUALITY[lv[k, :L]]
+        if pr.first_qual is not None:
+            q = np.concatenate([[LEVEL_QUALITY[pr.first_qual[i]]], q]).astype(np.int16)
+        quals.append(q)
+    fb = None if pr.first_base is None else pr.first_base[idx].copy()
+    return SimReads(seqs, quals, pr.origin[idx], pr.strand[idx], fb)
