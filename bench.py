@@ -319,6 +319,130 @@ def config4(lib, cpu=True, n_reads=20_000_000, skip_plain=False):
     return map_config(lib, "config4", gpath, pr, kw, 0, 2_000 if cpu else 0, 200_000, skip_plain=skip_plain)
 
 
+def serve_trace(g, n_req: int, seed: int = 5, lo: int = 1_000, hi: int = 1_000_000):
+    """Requests of the serving workload: sizes log-uniform in [lo, hi] reads, read lengths uniform
+    in 36-150, half nucleotide and half color reads, -i 3 or 7, drawn from `seed`."""
+    from storm_amd import synth
+    r = synth.rng(seed)
+    sizes = np.exp(r.uniform(np.log(lo), np.log(hi), size=n_req)).astype(np.int64)
+    colors = r.integers(0, 2, size=n_req)
+    bands = np.where(r.integers(0, 2, size=n_req) == 0, 3, 7)
+    reqs = []
+    for k in range(n_req):
+        n, c, i = int(sizes[k]), int(colors[k]), int(bands[k])
+        if c:   # L colors after the primer cover L + 1 bases: 35-149 colors for 36-150 bp reads
+            pr = synth.fast_color_reads(g, n, (35, 149), 5000 + k, snp=0.01, color_err=0.03, qlo=10, qhi=35)
+        else:
+            pr = synth.fast_nucleotide_reads(g, n, (36, 150), 5000 + k, sub=0.015, indel=0.002, qlo=15, qhi=40)
+        reqs.append((pr, dict(color=c, allowed_indels=i, z=85 if c else 100, t=100, max_hits=1)))
+    return reqs
+
+
+def serve_run(server, reqs, arrivals):
+    """Open loop: request k is submitted at arrivals[k] seconds (Poisson process), a waiter thread
+    per request collects it.  Returns per-request latencies and the wall time from the first
+    submission to the last result."""
+    import threading
+    lat = [0.0] * len(reqs)
+    info = [None] * len(reqs)
+    errs = []
+
+    def waiter(k, rid):
+        try:
+            _, res = server.wait(rid)
+            lat[k] = res["seconds_latency"]
+            info[k] = res
+        except Exception as e:   # noqa: BLE001 - reported after the run
+            errs.append(f"request {k}: {e}")
+
+    th = []
+    t0 = time.perf_counter()
+    for k, (pr, kw) in enumerate(reqs):
+        dt = arrivals[k] - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        rid = server.submit(pr, **kw)
+        w = threading.Thread(target=waiter, args=(k, rid))
+        w.start()
+        th.append(w)
+    for w in th:
+        w.join()
+    wall = time.perf_counter() - t0
+    if errs:
+        raise RuntimeError("; ".join(errs[:3]))
+    return np.array(lat), wall, info
+
+
+def config5(lib, n_req=40, load=0.8, lanes=3, max_batch=1 << 18):
+    """Serving: the config-3 genome (50 Mbp, GC 40-60 % per 10 kb window) with nucleotide and color
+    indexes resident; Poisson arrivals of requests of 1k-1M reads (log-uniform), lengths 36-150,
+    50/50 nucleotide/color, -i 3 or 7 per request, -M 1; admission by free HBM; continuous
+    batching vs the plain per-request path on the same arrival trace."""
+    from storm_amd import synth
+    d = tempfile.mkdtemp(prefix="storm_c5_")
+    g = synth.gc_window_genome(50_000_000, 3)
+    gpath = os.path.join(d, "g.fa")
+    synth.write_fasta_fast(gpath, "chr1", g)
+    t0 = time.perf_counter()
+    reqs = serve_trace(g, n_req)
+    sizes = np.array([len(pr.len) for pr, _ in reqs])
+    total = int(sizes.sum())
+    log(f"config5: {n_req} requests, {total} reads generated in {time.perf_counter() - t0:.1f}s")
+    res = {"requests": n_req, "reads": total, "request_reads_min_max": [int(sizes.min()), int(sizes.max())]}
+    t0 = time.perf_counter()
+    srv = lib.Server(gpath, lanes=lanes, max_batch_reads=max_batch)
+    res["server_create_s"] = round(time.perf_counter() - t0, 2)
+    # warm-up: one small request per class (code objects, autotuner, graph capture on every lane)
+    warm = [(synth.fast_nucleotide_reads(g, 20_000, (36, 150), 1, 0.015, 0.002, 15, 40), dict(color=0, allowed_indels=i,
+             z=100, t=100)) for i in (3, 7)] + \
+           [(synth.fast_color_reads(g, 20_000, (35, 149), 2, 0.01, 0.03, 10, 35), dict(color=1, allowed_indels=i, z=85,
+             t=100)) for i in (3, 7)]
+    for _ in range(lanes):
+        ids = [srv.submit(pr, **kw) for pr, kw in warm]
+        for rid in ids:
+            srv.wait(rid)
+    # capacity: every request submitted at once (closed loop)
+    lat_c, wall_c, _ = serve_run(srv, reqs, np.zeros(n_req))
+    cap = total / wall_c
+    log(f"config5: saturated batched capacity {cap:.0f} reads/s ({wall_c:.2f}s)")
+    rate = load * cap / sizes.mean()
+    arrivals = np.cumsum(synth.rng(55).exponential(1.0 / rate, size=n_req))
+    arrivals -= arrivals[0]
+    s0 = srv.stats()
+    lat_b, wall_b, info_b = serve_run(srv, reqs, arrivals)
+    s1 = srv.stats()
+    srv.close()
+    nb = s1["batches"] - s0["batches"]
+    res["arrival_rate_req_per_s"] = round(rate, 3)
+    res["offered_reads_per_s"] = round(rate * sizes.mean(), 1)
+    res["batched"] = {"reads_per_s": round(total / wall_b, 1), "wall_s": round(wall_b, 3),
+                      "p50_latency_s": round(float(np.percentile(lat_b, 50)), 4),
+                      "p99_latency_s": round(float(np.percentile(lat_b, 99)), 4),
+                      "saturated_reads_per_s": round(cap, 1),
+                      "batches": nb, "mean_reads_per_batch": round((s1["batch_reads"] - s0["batch_reads"]) / max(nb, 1), 1),
+                      "mean_requests_per_batch": round((s1["batch_requests"] - s0["batch_requests"]) / max(nb, 1), 2),
+                      "max_lanes_busy": s1["max_lanes_busy"], "admission_waits": s1["admission_waits"],
+                      "arena_bytes": s1["arena_bytes"], "arena_peak_bytes": s1["arena_peak_bytes"],
+                      "mapped": int(sum(i["n_mapped"] for i in info_b))}
+    log(f"config5: batched {total / wall_b:.0f} reads/s, p50 {np.percentile(lat_b, 50):.3f}s "
+        f"p99 {np.percentile(lat_b, 99):.3f}s")
+    t0 = time.perf_counter()
+    srv = lib.Server(gpath, mode=lib.STORM_MODE_PLAIN)
+    log(f"config5: plain server created in {time.perf_counter() - t0:.1f}s")
+    for pr, kw in warm:
+        srv.wait(srv.submit(pr, **kw))
+    lat_p, wall_p, info_p = serve_run(srv, reqs, arrivals)
+    srv.close()
+    res["plain"] = {"reads_per_s": round(total / wall_p, 1), "wall_s": round(wall_p, 3),
+                    "p50_latency_s": round(float(np.percentile(lat_p, 50)), 4),
+                    "p99_latency_s": round(float(np.percentile(lat_p, 99)), 4),
+                    "mapped": int(sum(i["n_mapped"] for i in info_p))}
+    res["batched_over_plain"] = round(wall_p / wall_b, 2)
+    log(f"config5: plain {total / wall_p:.0f} reads/s, p50 {np.percentile(lat_p, 50):.3f}s "
+        f"p99 {np.percentile(lat_p, 99):.3f}s; batched/plain {wall_p / wall_b:.2f}x")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -328,11 +452,14 @@ def main():
     ap.add_argument("--no-map", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--percall", type=int, default=20000)
-    ap.add_argument("--config", default="default", choices=["default", "1", "3", "4"],
+    ap.add_argument("--config", default="default", choices=["default", "1", "3", "4", "5"],
                     help="default: filter headline + config-1 mapping; 1/3/4: that mapping workload alone")
     ap.add_argument("--reads-c3", type=int, default=5_000_000)
     ap.add_argument("--reads-c4", type=int, default=20_000_000)
     ap.add_argument("--no-plain", action="store_true", help="mapping workloads: batched path only")
+    ap.add_argument("--requests", type=int, default=40, help="config 5: requests in the arrival trace")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device 3 times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the path shards reads across GPUs by design "
@@ -343,6 +470,24 @@ def main():
     assert torch.cuda.is_available()
     torch.zeros(1, device="cuda:0")
     lib.load()
+    if a.calib_copy:
+        src = torch.ones(a.calib_copy << 18, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration: copied {a.calib_copy} MiB device to device 3 times")
+        del src, dst
+    if a.config == "5":
+        res = config5(lib, n_req=a.requests)
+        best = res["batched"]
+        line = {"metric": METRIC, "value": best["reads_per_s"], "unit": "reads/s", "n_gpus": 1, "steps": 1,
+                "warmup": 1, "ms_per_step": round(best["wall_s"] * 1e3, 2), "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "u8/int32", "data": "synthetic",
+                "config": {"workload": "serving config 5: " + " ".join(config5.__doc__.split())},
+                "p50_latency_s": best["p50_latency_s"], "p99_latency_s": best["p99_latency_s"], "serving": res}
+        print(json.dumps(line), flush=True)
+        return
     if a.config != "default":
         cpu = not a.no_cpu
         if a.config == "1":

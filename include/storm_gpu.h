@@ -165,6 +165,75 @@ int storm_mapper_map_hits(storm_mapper *m, const storm_reads_view *reads, int mo
 int64_t storm_mapper_free_bytes(storm_mapper *m);
 void storm_mapper_destroy(storm_mapper *m);
 
+/* ------------------------------------------------------------------------------------------
+ * Group 2: serving loop.  No counterpart in the reference, which maps one read file per run
+ * (reads_against_references, run.c:764-1109); each request here is such a run's -M mapping of
+ * its reads (results per read as storm_mapper_map_hits returns them), and requests of different
+ * sizes, read lengths, encodings and bands share the device.
+ *
+ * The genome (one FASTA file) and its seed indexes stay resident in HBM in the encodings asked
+ * for (nucleotides for storm-nucleotide reads, colors for storm-color reads).  A request is
+ * admitted when its reads and results fit the device-memory budget (a share of the free HBM at
+ * creation, or a set budget); until then it waits, in arrival order.  Admitted requests are
+ * mapped by continuous batching: each of `lanes` streams repeatedly takes up to max_batch_reads
+ * unmapped reads of one parameter class (encoding, -i, -z, -t, scores, strands) from every
+ * admitted request of that class, in admission order, and runs them through the persistent fused
+ * kernel; lanes overlap on the device.  A read's result does not depend on the batch it lands in.
+ * mode STORM_MODE_PLAIN is the per-request baseline: one request at a time through the staged
+ * kernels, no admission control.  The index options (seeds, -l, -j) are the server's; a
+ * request's params supply the rest.  submit and wait may be called from any thread.
+ */
+typedef struct storm_server storm_server;
+
+typedef struct storm_server_config {
+  int mode;                   /* STORM_MODE_BATCHED (default) or STORM_MODE_PLAIN */
+  int lanes;                  /* batches in flight on the device, one stream each (default 3; plain: 1) */
+  int cross_request;          /* 1 (default): a batch takes reads of several requests; 0: one request */
+  int64_t max_batch_reads;    /* reads per batch (default 262144) */
+  int64_t hbm_budget_bytes;   /* device memory for admitted requests; 0 = a quarter of the free HBM at
+                                 creation; never more than free HBM - reserve */
+  int64_t hbm_reserve_bytes;  /* left free for batch buffers and other users (default 8 GiB) */
+  int encodings;              /* bit 0: nucleotide index, bit 1: color index (default 3) */
+  const char *seeds_nucleotide;   /* NULL = reference defaults (seed.h:29) */
+  const char *seeds_color;        /* NULL = reference defaults (seed.h:31) */
+  int stdev_distance;         /* -l (default 100) */
+  int ignore_lowercase;       /* -j */
+} storm_server_config;
+
+typedef struct storm_server_stats {
+  int64_t submitted, completed, failed, reads_done;
+  int64_t batches, batch_reads, batch_requests;   /* sums over batches */
+  int64_t max_requests_per_batch;
+  int64_t max_lanes_busy;       /* most batches in flight at once */
+  int64_t admission_waits;      /* requests that waited for device memory */
+  int64_t arena_bytes, arena_used_bytes, arena_peak_bytes;
+  int64_t pending, active;      /* requests waiting for admission / admitted and unfinished */
+  int64_t n_fallback, candidates;
+  double gpu_busy_seconds;      /* summed over batches (lanes overlap, so it can exceed wall time) */
+} storm_server_stats;
+
+typedef struct storm_request_result {
+  int64_t n_reads, n_mapped;
+  int status;
+  int batches;                  /* batches the request's reads were spread over */
+  double seconds_queued;        /* submit -> admitted */
+  double seconds_latency;       /* submit -> results ready */
+  double t_submit, t_done;      /* seconds since the server was created */
+} storm_request_result;
+
+void storm_server_config_default(storm_server_config *cfg);
+int storm_server_create(int device, const char *genome_path, const storm_server_config *cfg, storm_server **out);
+/* Copies the reads (storm_reads_view: codes, qual, len, first_qual; names are not used) and queues
+ * them; *id identifies the request.  Fails at once when the request alone exceeds the budget. */
+int storm_server_submit(storm_server *s, const storm_reads_view *reads, const storm_params *params, int64_t *id);
+/* Blocks until request `id` is done, then copies its slots (n_reads x 4, as storm_mapper_map_hits)
+ * into the non-NULL arrays and releases it.  Returns the request's status. */
+int storm_server_wait(storm_server *s, int64_t id, int32_t *score, int32_t *ref_id, int32_t *ref_start, int32_t *sense,
+                      storm_request_result *out);
+int storm_server_get_stats(storm_server *s, storm_server_stats *out);
+/* Finishes every submitted request, then stops the lanes and frees the device memory. */
+void storm_server_destroy(storm_server *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,14 +14,13 @@
 #include <unordered_map>
 #include <vector>
 
-#include "host_io.h"
-#include "storm_kernels.h"
+#include "mapper.h"
 
 namespace storm {
 
 static thread_local std::string g_err;
 
-static int fail(int code, const char* fmt, ...) {
+int fail(int code, const char* fmt, ...) {
   char buf[1024];
   va_list ap;
   va_start(ap, fmt);
@@ -31,53 +30,11 @@ static int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIPOK(expr)                                                                         \
-  do {                                                                                      \
-    hipError_t _e = (expr);                                                                 \
-    if (_e != hipSuccess) return fail(STORM_ERR_DEVICE, "%s: %s (%s:%d)", #expr,            \
-                                      hipGetErrorString(_e), __FILE__, __LINE__);           \
-  } while (0)
-
-// device allocation owned by a C++ object
-struct DevMem {
-  void* p = nullptr;
-  size_t n = 0;
-  DevMem() = default;
-  DevMem(const DevMem&) = delete;
-  DevMem& operator=(const DevMem&) = delete;
-  DevMem(DevMem&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
-  DevMem& operator=(DevMem&& o) noexcept {
-    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
-    return *this;
-  }
-  ~DevMem() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  hipError_t ensure(size_t bytes) {
-    if (bytes <= n && p) return hipSuccess;
-    release();
-    size_t b = bytes ? bytes : 16;
-    hipError_t e = hipMalloc(&p, b);
-    if (e == hipSuccess) n = b;
-    return e;
-  }
-  template <class T> T* as() const { return (T*)p; }
-};
-
-static double now_s() {
+double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
 // ------------------------------------------------------------------------------ genome planes
-struct DevGenome {
-  DevMem lo, hi, mask;
-  int64_t size = 0;
-  GenomeDev view() const { return GenomeDev{lo.as<uint32_t>(), hi.as<uint32_t>(), mask.as<uint32_t>(), size}; }
-};
-
 static int64_t plane_words(int64_t size) { return (GPAD + size + GPAD) / 32 + 4; }
 
 static int upload_genome(const uint8_t* code, const uint8_t* mask, int64_t size, DevGenome& g, hipStream_t st) {
@@ -101,9 +58,9 @@ static int upload_genome(const uint8_t* code, const uint8_t* mask, int64_t size,
   return STORM_OK;
 }
 
-static int filter_lanes(int d) { return d <= 1 ? 2 : d <= 3 ? 4 : d <= 7 ? 8 : 16; }   // run.c:109-127
+int filter_lanes(int d) { return d <= 1 ? 2 : d <= 3 ? 4 : d <= 7 ? 8 : 16; }   // run.c:109-127
 
-static void fill_score(const storm_params& p, ScoreDev& sc) {
+void fill_score(const storm_params& p, ScoreDev& sc) {
   sc.fmatch = std::abs(p.match) & 0xff;
   sc.fmismatch = std::abs(p.mismatch) & 0xff;
   sc.fopen = std::abs(p.gap_open) & 0xff;
@@ -125,107 +82,36 @@ static void fill_score(const storm_params& p, ScoreDev& sc) {
   sc.t = p.t;
 }
 
-// ------------------------------------------------------------------------------ mapper
-struct DevIndex {
-  DevMem first, map;
-  SeedSpec spec;
-};
 
-struct RefState {
-  DevGenome g;
-  std::vector<DevIndex> idx;
-  bool resident = false;
-};
-
-struct PassScratch {   // reused across passes
-  DevMem seg_count, seg_off, hits, kept, overflow, n_overflow, seg_begin, seg_end, sorted;
-  DevMem cand_off, cand_rs, cand_diag, cand_pos, pass, pass_scan;
-  DevMem rec_rs, rec_diag, rec_off, recs, tb, align_scratch, cub_tmp;
-  DevMem mark_rec, mark_slot, mark_count, gath;
-};
-
-// reads resident on the device for one map call (global read ids = device order)
-struct DevReads {
-  DevMem codes, qual, first_qual, len, planes, qplanes;
-  int64_t n = 0;
-  int has_qual = 0;
-  ReadsDev view() const {
-    return ReadsDev{planes.as<uint32_t>(), qplanes.as<uint32_t>(), len.as<int32_t>(), n, has_qual};
-  }
-};
-
-// batched path: buffers sized to the largest batch seen, one captured graph per reference
-struct FusedTune {
-  int group = 16;
-  int grid = 0;
-  float ms = 0;
-};
-
-struct FusedState {
-  DevReads reads;
-  DevMem slots, cells, overflow, n_dev, counter, stats, scratch, tbbuf, work, gids, gtb, gcount;
-  int64_t cap_reads = 0, scratch_stride = 0;
-  int grid = 0;
-  std::vector<hipGraphExec_t> graphs;   // per reference (nullptr = not captured)
-  std::vector<std::vector<const void*>> graph_sig;   // buffers and settings a graph was captured with
-  std::map<int64_t, FusedTune> tune;    // key: (W, I, color, length bucket)
-  bool tuned_any = false;
-  ~FusedState() {
-    for (auto g : graphs) if (g) (void)hipGraphExecDestroy(g);
-  }
-  void drop_graphs() {
-    for (auto& g : graphs) { if (g) (void)hipGraphExecDestroy(g); g = nullptr; }
-  }
-};
-
-}  // namespace storm
-
-struct storm_mapper {
-  int device = 0;
-  hipStream_t st = nullptr;
-  storm_params p{};
-  std::string seeds_str;
-  std::vector<storm::HostRef> refs;
-  std::vector<storm::SeedSpec> seeds;
-  std::vector<std::unique_ptr<storm::RefState>> rstate;
-  storm::PassScratch ps;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  double index_seconds = 0;
-  storm::FusedState fs;
-  bool want_tb = true;   // false while a call returns hits only: tracebacks stay on the device
-};
-
-namespace storm {
-
-static int ensure_ref_resident(storm_mapper* m, size_t r) {
-  RefState& rs = *m->rstate[r];
+int ensure_ref_resident(storm_mapper* m, size_t r) {
+  RefState& rs = *m->res->rstate[r];
   if (rs.resident) return STORM_OK;
-  const HostRef& ref = m->refs[r];
+  const HostRef& ref = m->res->refs[r];
   int rc = upload_genome(ref.code.data(), ref.mask.data(), ref.size, rs.g, m->st);
   if (rc) return rc;
   const double t0 = now_s();
   rs.idx.clear();
-  rs.idx.resize(m->seeds.size());
-  std::vector<HostIndex> hidx(m->seeds.size());
+  rs.idx.resize(m->res->seeds.size());
+  std::vector<HostIndex> hidx(m->res->seeds.size());
   std::vector<std::thread> th;
-  for (size_t s = 0; s < m->seeds.size(); ++s)
-    th.emplace_back([&, s]() { build_index(ref, m->seeds[s], m->p, hidx[s]); });
+  for (size_t s = 0; s < m->res->seeds.size(); ++s)
+    th.emplace_back([&, s]() { build_index(ref, m->res->seeds[s], m->p, hidx[s]); });
   for (auto& t : th) t.join();
-  for (size_t s = 0; s < m->seeds.size(); ++s) {
-    rs.idx[s].spec = m->seeds[s];
+  for (size_t s = 0; s < m->res->seeds.size(); ++s) {
+    rs.idx[s].spec = m->res->seeds[s];
     HIPOK(rs.idx[s].first.ensure(hidx[s].first.size() * 4));
     HIPOK(rs.idx[s].map.ensure(hidx[s].map.size() * 4 + 16));
     HIPOK(hipMemcpyAsync(rs.idx[s].first.p, hidx[s].first.data(), hidx[s].first.size() * 4, hipMemcpyHostToDevice, m->st));
     HIPOK(hipMemcpyAsync(rs.idx[s].map.p, hidx[s].map.data(), hidx[s].map.size() * 4, hipMemcpyHostToDevice, m->st));
   }
   HIPOK(hipStreamSynchronize(m->st));
-  m->index_seconds += now_s() - t0;
+  m->res->index_seconds += now_s() - t0;
   rs.resident = true;
   return STORM_OK;
 }
 
 static void release_ref(storm_mapper* m, size_t r) {
-  RefState& rs = *m->rstate[r];
+  RefState& rs = *m->res->rstate[r];
   rs.g.lo.release(); rs.g.hi.release(); rs.g.mask.release();
   rs.idx.clear();
   rs.resident = false;
@@ -249,7 +135,7 @@ static SeedSetDev seed_set(const RefState& rs) {
 
 // Uploads the reads order[0..) of the view (all of them, or the subset the fused kernel handed
 // to the staged kernels), in that order.
-static int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order, DevReads& d) {
+int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order, DevReads& d) {
   const int64_t n = (int64_t)order.size();
   d.n = n;
   d.has_qual = v->qual != nullptr;
@@ -283,14 +169,21 @@ static int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::v
     HIPOK(d.first_qual.ensure(fq.size()));
     HIPOK(hipMemcpyAsync(d.first_qual.p, fq.data(), fq.size(), hipMemcpyHostToDevice, m->st));
   }
+  return prep_device_reads(m, n, cs, qs, qs != 0, !fq.empty(), d);
+}
+
+int prep_device_reads(storm_mapper* m, int64_t n, int64_t code_stride, int64_t qual_stride, bool has_qual,
+                      bool has_first_qual, DevReads& d) {
+  d.n = n;
+  d.has_qual = has_qual ? 1 : 0;
   HIPOK(d.planes.ensure((size_t)(2 * n) * 2 * RW * 4));
   HIPOK(d.qplanes.ensure((size_t)(2 * n) * QW * 4));
   PrepArgs pa{};
   pa.codes = d.codes.as<uint8_t>();
-  pa.code_stride = (int)cs;
-  pa.qual = qs ? d.qual.as<uint8_t>() : nullptr;
-  pa.qual_stride = (int)qs;
-  pa.first_qual = fq.empty() ? nullptr : d.first_qual.as<uint8_t>();
+  pa.code_stride = (int)code_stride;
+  pa.qual = has_qual ? d.qual.as<uint8_t>() : nullptr;
+  pa.qual_stride = (int)qual_stride;
+  pa.first_qual = has_first_qual ? d.first_qual.as<uint8_t>() : nullptr;
   pa.len = d.len.as<int32_t>();
   pa.n = n;
   pa.color = m->p.color;
@@ -299,14 +192,6 @@ static int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::v
   HIPOK(launch_prep_reads(pa, m->st));
   return STORM_OK;
 }
-
-struct PassStats {
-  int64_t cand = 0, rec = 0, hits = 0;
-  float ms_seed = 0, ms_filter = 0, ms_align = 0, ms_replay = 0, ms_lookup = 0;
-};
-
-// traceback bytes of records still referenced by a slot, kept on the host per (pass, rec)
-using TbStore = std::unordered_map<int64_t, std::vector<uint8_t>>;
 
 static float elapsed(storm_mapper* m, hipEvent_t a, hipEvent_t b) {
   float ms = 0;
@@ -317,11 +202,11 @@ static float elapsed(storm_mapper* m, hipEvent_t a, hipEvent_t b) {
 // One mapping pass of reads [r0, r1) (device order) against one reference sequence: the stages
 // of process_read (run.c:555-747) as separate launches, with host synchronisations to size
 // the buffers of the next stage.
-static int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int pass_id, int64_t r0, int64_t r1,
+int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int pass_id, int64_t r0, int64_t r1,
                     TbStore& store, PassStats& stats) {
   PassScratch& P = m->ps;
   hipStream_t st = m->st;
-  RefState& rs = *m->rstate[(size_t)ref_id];
+  RefState& rs = *m->res->rstate[(size_t)ref_id];
   const int64_t n_rs = 2 * (r1 - r0);
   if (n_rs == 0) return STORM_OK;
   ScoreDev sc{};
@@ -612,8 +497,8 @@ static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode,
   TbStore store;
   PassStats ps;
   int pass_id = 0;
-  const double ti = m->index_seconds;
-  for (size_t r = 0; r < m->refs.size(); ++r) {
+  const double ti = m->res->index_seconds;
+  for (size_t r = 0; r < m->res->refs.size(); ++r) {
     rc = ensure_ref_resident(m, r);
     if (rc) return rc;
     for (auto& b : batches) {
@@ -624,7 +509,7 @@ static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode,
         if (rc) return rc;
       }
     }
-    if (m->refs.size() > 1) release_ref(m, r);
+    if (m->res->refs.size() > 1 && !m->res->pinned) release_ref(m, r);
   }
   std::vector<Slot> hs((size_t)(4 * n));
   HIPOK(hipMemcpyAsync(hs.data(), slots.p, hs.size() * sizeof(Slot), hipMemcpyDeviceToHost, m->st));
@@ -639,7 +524,7 @@ static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode,
     st_out->n_filter_pass = ps.rec;
     st_out->n_accepted = 0;
     st_out->n_hits = ps.hits;
-    st_out->seconds_index = m->index_seconds - ti;
+    st_out->seconds_index = m->res->index_seconds - ti;
     st_out->seconds_map = now_s() - t0 - st_out->seconds_index;
     st_out->gpu_ms_seed = ps.ms_seed;
     st_out->gpu_ms_filter = ps.ms_filter;
@@ -687,9 +572,9 @@ static int ensure_fused_buffers(storm_mapper* m, int64_t n, int64_t stride, int 
 static FusedArgs fused_args(storm_mapper* m, size_t ref, int group) {
   FusedState& F = m->fs;
   FusedArgs a{};
-  a.g = m->rstate[ref]->g.view();
+  a.g = m->res->rstate[ref]->g.view();
   a.r = F.reads.view();
-  a.seeds = seed_set(*m->rstate[ref]);
+  a.seeds = seed_set(*m->res->rstate[ref]);
   fill_score(m->p, a.sc);
   a.n_reads = F.n_dev.as<int32_t>();
   a.group = group;
@@ -757,7 +642,7 @@ static int tune_fused(storm_mapper* m, int64_t n, int maxL, int max_grid, FusedT
 
 // Reads flagged by the fused kernel (LDS capacity exceeded) go through the staged kernels for
 // this reference; their slots are gathered into a sub-batch and scattered back.
-static int staged_subset(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order,
+int staged_subset(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order,
                          const std::vector<int64_t>& dev_ids, int ref, int& pass_id, TbStore& store, PassStats& ps) {
   FusedState& F = m->fs;
   const int64_t k = (int64_t)dev_ids.size();
@@ -785,44 +670,30 @@ static int staged_subset(storm_mapper* m, const storm_reads_view* v, const std::
   return STORM_OK;
 }
 
-static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vector<HostSlot>& out_slots,
-                            storm_map_stats* st_out) {
+int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback, TbStore& store, PassStats& ps,
+               unsigned long long totals[4], int64_t& n_fallback) {
   FusedState& F = m->fs;
-  const int64_t n = v->n_reads;
-  if (n <= 0) return fail(STORM_ERR_INVALID, "no reads");
-  if (n > (int64_t)INT32_MAX / 4) return fail(STORM_ERR_INVALID, "batch too large");
-  // device order: by length (tasks then hold reads of similar length)
-  std::vector<int64_t> order((size_t)n);
-  for (int64_t i = 0; i < n; ++i) order[(size_t)i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return v->len[a] < v->len[b]; });
-  int maxL = 0;
-  for (int64_t i = 0; i < n; ++i) maxL = std::max(maxL, (int)v->len[i]);
-  const double t0 = now_s();
-  const double ti = m->index_seconds;
-  int rc = upload_reads(m, v, order, F.reads);
-  if (rc) return rc;
   ScoreDev sc{};
   fill_score(m->p, sc);
   const int64_t stride = (((int64_t)maxL * (2 * sc.I + 1)) + 15) & ~15LL;
   const int bpc = fused_blocks_per_cu(sc.W, sc.I, m->p.color);
   if (bpc <= 0) return fail(STORM_ERR_DEVICE, "fused kernel: no occupancy for this band class");
   const int max_grid = device_cus(m->device) * bpc;
-  rc = ensure_fused_buffers(m, n, stride, max_grid);
+  int rc = ensure_fused_buffers(m, n, stride, max_grid);
   if (rc) return rc;
   HIPOK(launch_slots_init(F.slots.as<Slot>(), 4 * n, m->st));
   const int32_t n32 = (int32_t)n;
-  if (F.graphs.size() != m->refs.size()) {
+  const size_t n_refs = m->res->refs.size();
+  if (F.graphs.size() != n_refs) {
     F.drop_graphs();
-    F.graphs.assign(m->refs.size(), nullptr);
-    F.graph_sig.assign(m->refs.size(), {});
+    F.graphs.assign(n_refs, nullptr);
+    F.graph_sig.assign(n_refs, {});
   }
-  TbStore store;
-  PassStats ps;
   int pass_id = 0;
-  unsigned long long totals[4] = {0, 0, 0, 0};
-  int64_t n_fallback = 0;
+  for (int q = 0; q < 4; ++q) totals[q] = 0;
+  n_fallback = 0;
   std::vector<uint8_t> ovf((size_t)n);
-  for (size_t r = 0; r < m->refs.size(); ++r) {
+  for (size_t r = 0; r < n_refs; ++r) {
     rc = ensure_ref_resident(m, r);
     if (rc) return rc;
     FusedTune tune;
@@ -833,11 +704,14 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
       tune_fused(m, n, maxL, max_grid, tune);
     }
     HIPOK(hipMemcpyAsync(F.n_dev.p, &n32, 4, hipMemcpyHostToDevice, m->st));
-    // one graph per reference: reset + fused kernel, replayed for every batch with the same buffers
+    // one graph per reference: reset + fused kernel, replayed for every batch with the same
+    // buffers and scoring (the kernel arguments are part of the captured graph)
     FusedArgs a = fused_args(m, r, tune.group);
     std::vector<const void*> sig = {a.g.lo, a.r.planes, a.r.qual, a.r.len, a.slots, a.cells, a.scratch, a.tbbuf, a.work,
                                     a.seeds.s[0].first, (const void*)(intptr_t)tune.group, (const void*)(intptr_t)tune.grid,
-                                    (const void*)(intptr_t)a.scratch_stride};
+                                    (const void*)(intptr_t)a.scratch_stride, (const void*)(intptr_t)a.sense_mask};
+    const int* scw = (const int*)&a.sc;
+    for (size_t q = 0; q < sizeof(ScoreDev) / sizeof(int); ++q) sig.push_back((const void*)(intptr_t)scw[q]);
     if (!F.graphs[r] || F.graph_sig[r] != sig) {
       if (F.graphs[r]) { (void)hipGraphExecDestroy(F.graphs[r]); F.graphs[r] = nullptr; }
       hipGraph_t g;
@@ -873,11 +747,42 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
     for (int64_t i = 0; i < n; ++i) if (ovf[(size_t)i]) over.push_back(i);
     n_fallback += (int64_t)over.size();
     if (!over.empty()) {
-      rc = staged_subset(m, v, order, over, (int)r, pass_id, store, ps);
+      rc = fallback(over, (int)r, pass_id, store, ps);
       if (rc) return rc;
     }
-    if (m->refs.size() > 1) { release_ref(m, r); if (F.graphs[r]) { (void)hipGraphExecDestroy(F.graphs[r]); F.graphs[r] = nullptr; } }
+    if (n_refs > 1 && !m->res->pinned) {
+      release_ref(m, r);
+      if (F.graphs[r]) { (void)hipGraphExecDestroy(F.graphs[r]); F.graphs[r] = nullptr; }
+    }
   }
+  return STORM_OK;
+}
+
+static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vector<HostSlot>& out_slots,
+                            storm_map_stats* st_out) {
+  FusedState& F = m->fs;
+  const int64_t n = v->n_reads;
+  if (n <= 0) return fail(STORM_ERR_INVALID, "no reads");
+  if (n > (int64_t)INT32_MAX / 4) return fail(STORM_ERR_INVALID, "batch too large");
+  // device order: by length (tasks then hold reads of similar length)
+  std::vector<int64_t> order((size_t)n);
+  for (int64_t i = 0; i < n; ++i) order[(size_t)i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return v->len[a] < v->len[b]; });
+  int maxL = 0;
+  for (int64_t i = 0; i < n; ++i) maxL = std::max(maxL, (int)v->len[i]);
+  const double t0 = now_s();
+  const double ti = m->res->index_seconds;
+  int rc = upload_reads(m, v, order, F.reads);
+  if (rc) return rc;
+  TbStore store;
+  PassStats ps;
+  unsigned long long totals[4] = {0, 0, 0, 0};
+  int64_t n_fallback = 0;
+  FallbackFn fb = [&](const std::vector<int64_t>& over, int ref, int& pass_id, TbStore& st, PassStats& pst) {
+    return staged_subset(m, v, order, over, ref, pass_id, st, pst);
+  };
+  rc = fused_core(m, n, maxL, fb, store, ps, totals, n_fallback);
+  if (rc) return rc;
   // tracebacks of the fused slots that will be written (ranks < -M)
   int32_t ng = 0;
   std::vector<Slot> hs((size_t)(4 * n));
@@ -910,7 +815,7 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
     st_out->n_accepted = (int64_t)totals[2];
     st_out->n_hits = (int64_t)totals[3] + ps.hits;
     st_out->n_fallback = n_fallback;
-    st_out->seconds_index = m->index_seconds - ti;
+    st_out->seconds_index = m->res->index_seconds - ti;
     st_out->seconds_map = now_s() - t0 - st_out->seconds_index;
     st_out->gpu_ms_seed = ps.ms_seed;
     st_out->gpu_ms_filter = ps.ms_filter;
@@ -921,16 +826,29 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
   return STORM_OK;
 }
 
+int load_resident(const char* genome_path, const storm_params& p, std::shared_ptr<Resident>& out) {
+  std::shared_ptr<Resident> r = std::make_shared<Resident>();
+  r->seeds_str = p.seeds ? p.seeds : default_seeds(p.color);
+  std::string err;
+  int rc = parse_seeds(r->seeds_str.c_str(), r->seeds, err);
+  if (rc) return fail(rc, "%s", err.c_str());
+  rc = load_reference(genome_path, p, r->refs, err);
+  if (rc) return fail(rc, "%s", err.c_str());
+  for (size_t k = 0; k < r->refs.size(); ++k) r->rstate.emplace_back(new RefState());
+  out = std::move(r);
+  return STORM_OK;
+}
+
 static int64_t count_lookups(const storm_mapper* m, const storm_reads_view* v) {
   const int strands = ((m->p.sense_mask & 1) ? 1 : 0) + ((m->p.sense_mask & 2) ? 1 : 0);
   int64_t n = 0;
   for (int64_t i = 0; i < v->n_reads; ++i)
-    for (const SeedSpec& s : m->seeds)
+    for (const SeedSpec& s : m->res->seeds)
       n += s.pos.empty() ? std::max<int64_t>(0, v->len[i] - s.span + 1) : (int64_t)s.pos.size();
-  return n * strands * (int64_t)m->refs.size();
+  return n * strands * (int64_t)m->res->refs.size();
 }
 
-static int map_view(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
+int map_view(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
                     storm_map_stats* st_out) {
   int rc = (mode == STORM_MODE_BATCHED) ? map_view_batched(m, v, out_slots, st_out)
                                          : map_view_staged(m, v, mode, out_slots, st_out);
@@ -941,6 +859,22 @@ static int map_view(storm_mapper* m, const storm_reads_view* v, int mode, std::v
 
 }  // namespace storm
 
+
+const char* storm::default_seeds(int color) {   // seed.h:29-32
+  return color ? "####----##--##-####;####-###--#----####;####-####-####"
+               : "####--#-#--##--####;#####--###-#-###;###-##-#######";
+}
+
+int storm::check_params(const storm_params* p) {
+  if (!p) return fail(STORM_ERR_INVALID, "no parameters");
+  if (p->allowed_indels < 0 || p->allowed_indels > MAX_I)
+    return fail(STORM_ERR_INVALID, "-i %d outside 0..%d", p->allowed_indels, MAX_I);
+  if (p->max_hits < 1 || p->max_hits > 4) return fail(STORM_ERR_INVALID, "-M must be in 1..4");
+  if (p->match <= 0 || p->mismatch > 0 || p->gap_open > 0 || p->gap_extend > 0)
+    return fail(STORM_ERR_INVALID, "scores: match > 0, mismatch/gap penalties <= 0 (main.c:262-291)");
+  if (p->sense_mask < 1 || p->sense_mask > 3) return fail(STORM_ERR_INVALID, "sense_mask must be 1, 2 or 3");
+  return STORM_OK;
+}
 
 // =============================================================================== C ABI
 using namespace storm;
@@ -962,22 +896,6 @@ void storm_params_default(storm_params* p, int color) {   // alignment.h:24-28,5
   p->seeds = nullptr;
 }
 
-static const char* default_seeds(int color) {   // seed.h:29-32
-  return color ? "####----##--##-####;####-###--#----####;####-####-####"
-               : "####--#-#--##--####;#####--###-#-###;###-##-#######";
-}
-
-static int check_params(const storm_params* p) {
-  if (!p) return fail(STORM_ERR_INVALID, "no parameters");
-  if (p->allowed_indels < 0 || p->allowed_indels > MAX_I)
-    return fail(STORM_ERR_INVALID, "-i %d outside 0..%d", p->allowed_indels, MAX_I);
-  if (p->max_hits < 1 || p->max_hits > 4) return fail(STORM_ERR_INVALID, "-M must be in 1..4");
-  if (p->match <= 0 || p->mismatch > 0 || p->gap_open > 0 || p->gap_extend > 0)
-    return fail(STORM_ERR_INVALID, "scores: match > 0, mismatch/gap penalties <= 0 (main.c:262-291)");
-  if (p->sense_mask < 1 || p->sense_mask > 3) return fail(STORM_ERR_INVALID, "sense_mask must be 1, 2 or 3");
-  return STORM_OK;
-}
-
 int storm_mapper_create(int device, const char* genome_path, const storm_params* params, storm_mapper** out) {
   *out = nullptr;
   int rc = check_params(params);
@@ -985,17 +903,12 @@ int storm_mapper_create(int device, const char* genome_path, const storm_params*
   std::unique_ptr<storm_mapper> m(new storm_mapper());
   m->device = device;
   m->p = *params;
-  m->seeds_str = params->seeds ? params->seeds : default_seeds(params->color);
-  m->p.seeds = m->seeds_str.c_str();
-  std::string err;
-  rc = parse_seeds(m->seeds_str.c_str(), m->seeds, err);
-  if (rc) return fail(rc, "%s", err.c_str());
+  rc = load_resident(genome_path, *params, m->res);
+  if (rc) return rc;
+  m->p.seeds = m->res->seeds_str.c_str();
   HIPOK(hipSetDevice(device));
   HIPOK(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
-  rc = load_reference(genome_path, m->p, m->refs, err);
-  if (rc) return fail(rc, "%s", err.c_str());
-  for (size_t r = 0; r < m->refs.size(); ++r) m->rstate.emplace_back(new RefState());
-  if (m->refs.size() == 1) {   // one sequence: keep genome + indexes resident in HBM
+  if (m->res->refs.size() == 1) {   // one sequence: keep genome + indexes resident in HBM
     rc = ensure_ref_resident(m.get(), 0);
     if (rc) return rc;
   }
@@ -1006,8 +919,9 @@ int storm_mapper_create(int device, const char* genome_path, const storm_params*
 void storm_mapper_destroy(storm_mapper* m) {
   if (!m) return;
   (void)hipSetDevice(m->device);
-  m->rstate.clear();
+  m->fs.drop_graphs();
   m->ps = PassScratch();
+  m->res.reset();
   if (m->st) (void)hipStreamDestroy(m->st);
   delete m;
 }
@@ -1054,7 +968,7 @@ int storm_mapper_map(storm_mapper* m, const storm_reads_view* reads, int mode, c
   HostReads hr;
   build_host_reads(reads, m->p.color, hr);
   std::string err;
-  rc = write_sam(sam_path, m->refs, hr, slots, m->p, argc, argv, err);
+  rc = write_sam(sam_path, m->res->refs, hr, slots, m->p, argc, argv, err);
   if (rc) return fail(rc, "%s", err.c_str());
   local.seconds_sam = now_s() - t0;
   if (stats) *stats = local;
@@ -1111,7 +1025,7 @@ int storm_map_files(int device, const char* genome_path, const char* reads_path,
   v.db_name = hr.db_name.c_str();
   storm_map_stats local{};
   rc = storm_mapper_map(m, &v, mode, sam_path, argc, argv, &local);
-  local.seconds_index += m->index_seconds;
+  local.seconds_index += m->res->index_seconds;
   storm_mapper_destroy(m);
   if (rc) return rc;
   local.seconds_load = t1 - t0;

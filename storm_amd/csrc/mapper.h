@@ -1,0 +1,187 @@
+// mapper.h -- internal interface of the mapping pipeline, shared by the C ABI (capi.cpp) and the
+// serving loop (server.cpp): device buffers, the resident genome planes and seed indexes, and the
+// stages of one mapping batch.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <functional>
+#include <map>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "host_io.h"
+#include "storm_kernels.h"
+
+namespace storm {
+
+// sets the calling thread's storm_last_error() message and returns `code`
+int fail(int code, const char* fmt, ...);
+
+#define HIPOK(expr)                                                                         \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess) return ::storm::fail(STORM_ERR_DEVICE, "%s: %s (%s:%d)", #expr,   \
+                                               hipGetErrorString(_e), __FILE__, __LINE__);  \
+  } while (0)
+
+// device allocation owned by a C++ object
+struct DevMem {
+  void* p = nullptr;
+  size_t n = 0;
+  DevMem() = default;
+  DevMem(const DevMem&) = delete;
+  DevMem& operator=(const DevMem&) = delete;
+  DevMem(DevMem&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevMem& operator=(DevMem&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevMem() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= n && p) return hipSuccess;
+    release();
+    size_t b = bytes ? bytes : 16;
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) n = b;
+    return e;
+  }
+  template <class T> T* as() const { return (T*)p; }
+};
+
+double now_s();
+
+// genome bit planes of one reference sequence
+struct DevGenome {
+  DevMem lo, hi, mask;
+  int64_t size = 0;
+  GenomeDev view() const { return GenomeDev{lo.as<uint32_t>(), hi.as<uint32_t>(), mask.as<uint32_t>(), size}; }
+};
+
+struct DevIndex {
+  DevMem first, map;
+  SeedSpec spec;
+};
+
+struct RefState {
+  DevGenome g;
+  std::vector<DevIndex> idx;
+  bool resident = false;
+};
+
+// Genome (one encoding: nucleotides or colors) and its seed indexes, shared by every mapper that
+// maps against it.  With `pinned` every sequence stays resident; otherwise a multi-sequence genome
+// is uploaded one sequence at a time (each pass releases the previous one).
+struct Resident {
+  std::vector<HostRef> refs;
+  std::vector<SeedSpec> seeds;
+  std::string seeds_str;
+  std::vector<std::unique_ptr<RefState>> rstate;
+  double index_seconds = 0;
+  bool pinned = false;
+};
+
+struct PassScratch {   // reused across passes
+  DevMem seg_count, seg_off, hits, kept, overflow, n_overflow, seg_begin, seg_end, sorted;
+  DevMem cand_off, cand_rs, cand_diag, cand_pos, pass, pass_scan;
+  DevMem rec_rs, rec_diag, rec_off, recs, tb, align_scratch, cub_tmp;
+  DevMem mark_rec, mark_slot, mark_count, gath;
+};
+
+// reads resident on the device for one map call (global read ids = device order)
+struct DevReads {
+  DevMem codes, qual, first_qual, len, planes, qplanes;
+  int64_t n = 0;
+  int has_qual = 0;
+  ReadsDev view() const {
+    return ReadsDev{planes.as<uint32_t>(), qplanes.as<uint32_t>(), len.as<int32_t>(), n, has_qual};
+  }
+};
+
+// batched path: buffers sized to the largest batch seen, one captured graph per reference
+struct FusedTune {
+  int group = 16;
+  int grid = 0;
+  float ms = 0;
+};
+
+struct FusedState {
+  DevReads reads;
+  DevMem slots, cells, overflow, n_dev, counter, stats, scratch, tbbuf, work, gids, gtb, gcount;
+  int64_t cap_reads = 0, scratch_stride = 0;
+  int grid = 0;
+  std::vector<hipGraphExec_t> graphs;   // per reference (nullptr = not captured)
+  std::vector<std::vector<const void*>> graph_sig;   // buffers and settings a graph was captured with
+  std::map<int64_t, FusedTune> tune;    // key: (W, I, color, length bucket)
+  bool tuned_any = false;
+  ~FusedState() {
+    for (auto g : graphs) if (g) (void)hipGraphExecDestroy(g);
+  }
+  void drop_graphs() {
+    for (auto& g : graphs) { if (g) (void)hipGraphExecDestroy(g); g = nullptr; }
+  }
+};
+
+struct PassStats {
+  int64_t cand = 0, rec = 0, hits = 0;
+  float ms_seed = 0, ms_filter = 0, ms_align = 0, ms_replay = 0, ms_lookup = 0;
+};
+
+// traceback bytes of records still referenced by a slot, kept on the host per (pass, rec)
+using TbStore = std::unordered_map<int64_t, std::vector<uint8_t>>;
+
+}  // namespace storm
+
+// One execution context: a stream, per-batch buffers and the parameters of the batches it maps,
+// over a shared resident genome.
+struct storm_mapper {
+  int device = 0;
+  hipStream_t st = nullptr;
+  storm_params p{};
+  std::shared_ptr<storm::Resident> res;
+  storm::PassScratch ps;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  storm::FusedState fs;
+  bool want_tb = true;   // false while a call returns hits only: tracebacks stay on the device
+};
+
+namespace storm {
+
+int filter_lanes(int d);
+void fill_score(const storm_params& p, ScoreDev& sc);
+const char* default_seeds(int color);
+int check_params(const storm_params* p);
+
+// Loads a FASTA genome in the encoding of p.color and parses the seeds (no device work).
+int load_resident(const char* genome_path, const storm_params& p, std::shared_ptr<Resident>& out);
+int ensure_ref_resident(storm_mapper* m, size_t r);
+
+// Packed reads already in d.codes / d.qual / d.first_qual / d.len (device): builds the strand planes.
+int prep_device_reads(storm_mapper* m, int64_t n, int64_t code_stride, int64_t qual_stride, bool has_qual,
+                      bool has_first_qual, DevReads& d);
+int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order, DevReads& d);
+int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int pass_id, int64_t r0, int64_t r1,
+             TbStore& store, PassStats& stats);
+int staged_subset(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order,
+                  const std::vector<int64_t>& dev_ids, int ref, int& pass_id, TbStore& store, PassStats& ps);
+
+// Reads that the fused kernel handed back (device ids of m->fs.reads) for reference `ref`; the
+// callback maps them with the staged kernels and leaves their slots in m->fs.slots.
+using FallbackFn = std::function<int(const std::vector<int64_t>& dev_ids, int ref, int& pass_id, TbStore& store,
+                                     PassStats& ps)>;
+// The batched path over the n reads already in m->fs.reads (device order, longest maxL): slots
+// initialised, then per reference the captured fused kernel and the fallback of its overflow
+// reads.  Slots end in m->fs.slots.  totals: candidates, filter passes, accepted, raw hits.
+int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback, TbStore& store, PassStats& ps,
+               unsigned long long totals[4], int64_t& n_fallback);
+
+int map_view(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
+             storm_map_stats* st_out);
+
+}  // namespace storm

@@ -144,6 +144,32 @@ constexpr int FUSED_CCAP = 8192;   // candidates per flush of k_fused
 constexpr int64_t FUSED_WORK_BYTES =
     ((8LL * FUSED_CCAP + ((int64_t)sizeof(AlignRec) + 4) * FUSED_PCAP + 2LL * FUSED_CCAP) + 255) & ~255LL;
 
+// serving loop (k_serve.hip): one admitted request's reads and results in the device arena
+struct ServeRegion {
+  const uint8_t* codes;      // n * code_stride bytes, 2-bit packed (codes.h:125-130)
+  const uint8_t* qual;       // n * qual_stride bytes of 2-bit levels, or null (every level 3)
+  const int32_t* len;
+  const uint8_t* first_qual; // color: level of the primer-side color, or null
+  int32_t code_stride, qual_stride;
+  int32_t* hits;             // out: per read, 4 slots x {score, ref_id, ref_start, sense}
+};
+struct ServeRow {            // one read of a batch: region (index into the batch's table), read
+  int32_t region, row;
+};
+constexpr int SERVE_CS = MAXL / 4;   // code and quality bytes per read in a batch
+struct ServeGatherArgs {
+  const ServeRegion* regions;
+  const ServeRow* rows;
+  int64_t n;
+  uint8_t* codes;            // n * SERVE_CS
+  uint8_t* qual;             // n * SERVE_CS
+  int32_t* len;
+  uint8_t* first_qual;
+};
+hipError_t launch_serve_gather(const ServeGatherArgs& a, hipStream_t st);
+hipError_t launch_serve_scatter(const Slot* slots, const ServeRow* rows, int64_t n, const ServeRegion* regions,
+                                hipStream_t st);
+
 hipError_t launch_fused(const FusedArgs& a, int grid, hipStream_t st);
 hipError_t launch_fused_reset(int32_t* counter, unsigned long long* stats, hipStream_t st);
 int fused_blocks_per_cu(int W, int I, int color);

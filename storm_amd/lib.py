@@ -47,6 +47,33 @@ class ReadsView(C.Structure):
                 ("db_name", C.c_char_p)]
 
 
+class ServerConfig(C.Structure):
+    """storm_server_config (include/storm_gpu.h)."""
+    _fields_ = [("mode", C.c_int), ("lanes", C.c_int), ("cross_request", C.c_int), ("max_batch_reads", C.c_int64),
+                ("hbm_budget_bytes", C.c_int64), ("hbm_reserve_bytes", C.c_int64), ("encodings", C.c_int),
+                ("seeds_nucleotide", C.c_char_p), ("seeds_color", C.c_char_p), ("stdev_distance", C.c_int),
+                ("ignore_lowercase", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in (
+        "submitted", "completed", "failed", "reads_done", "batches", "batch_reads", "batch_requests",
+        "max_requests_per_batch", "max_lanes_busy", "admission_waits", "arena_bytes", "arena_used_bytes",
+        "arena_peak_bytes", "pending", "active", "n_fallback", "candidates")] + [("gpu_busy_seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("n_reads", C.c_int64), ("n_mapped", C.c_int64), ("status", C.c_int), ("batches", C.c_int),
+                ("seconds_queued", C.c_double), ("seconds_latency", C.c_double), ("t_submit", C.c_double),
+                ("t_done", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -72,6 +99,12 @@ def load() -> C.CDLL:
     lib.storm_mapper_free_bytes.argtypes = [P]
     lib.storm_mapper_free_bytes.restype = L
     lib.storm_mapper_destroy.argtypes = [P]
+    lib.storm_server_config_default.argtypes = [C.POINTER(ServerConfig)]
+    lib.storm_server_create.argtypes = [I, C.c_char_p, C.POINTER(ServerConfig), C.POINTER(P)]
+    lib.storm_server_submit.argtypes = [P, C.POINTER(ReadsView), C.POINTER(Params), C.POINTER(L)]
+    lib.storm_server_wait.argtypes = [P, L, P, P, P, P, C.POINTER(RequestResult)]
+    lib.storm_server_get_stats.argtypes = [P, C.POINTER(ServerStats)]
+    lib.storm_server_destroy.argtypes = [P]
     lib.alignment_gfx950__set_genome.argtypes = [P, C.c_size_t, C.c_size_t]
     for w in ("octa", "quad", "pair", "mono"):
         getattr(lib, f"alignment_gfx950__init_{w}").argtypes = [C.c_uint] * 6
@@ -228,6 +261,62 @@ class Mapper:
     def close(self):
         if self.h:
             load().storm_mapper_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """storm_server_*: genome and indexes resident in HBM (both encodings), requests mapped by
+    continuous batching (mode=STORM_MODE_BATCHED) or one at a time with the staged kernels
+    (mode=STORM_MODE_PLAIN, the per-request baseline)."""
+
+    def __init__(self, genome: str, device: int = 0, **cfg):
+        lib = load()
+        c = ServerConfig()
+        lib.storm_server_config_default(C.byref(c))
+        for k, v in cfg.items():
+            if k in ("seeds_nucleotide", "seeds_color") and isinstance(v, str):
+                v = v.encode()
+            setattr(c, k, v)
+        self.cfg = c
+        h = C.c_void_p()
+        check(lib.storm_server_create(device, genome.encode(), C.byref(c), C.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def submit(self, reads, lo: int = 0, hi: int | None = None, **kw) -> int:
+        """Queues reads[lo:hi] (synth.PackedReads fields) with the given storm_params fields."""
+        v, keep = Mapper.view(reads, lo, hi)
+        p = params(**kw)
+        rid = C.c_int64()
+        check(load().storm_server_submit(self.h, C.byref(v), C.byref(p), C.byref(rid)))
+        self._keep[rid.value] = v.n_reads   # the server copied the reads; only the size is kept
+        del keep
+        return rid.value
+
+    def wait(self, rid: int):
+        """Blocks until request `rid` is done: (dict of (n, 4) int32 arrays, result dict)."""
+        n = self._keep.pop(rid)
+        out = {k: np.zeros((n, 4), np.int32) for k in ("score", "ref_id", "ref_start", "sense")}
+        res = RequestResult()
+        rc = load().storm_server_wait(self.h, rid, _ptr(out["score"]), _ptr(out["ref_id"]), _ptr(out["ref_start"]),
+                                      _ptr(out["sense"]), C.byref(res))
+        check(rc)
+        return out, res.as_dict()
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(load().storm_server_get_stats(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self.h:
+            load().storm_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -30,3 +30,18 @@ def test_errors_are_returned_not_exited(tmp_path):
     with pytest.raises(lib.StormError):
         lib.map_files(str(tmp_path / "missing.fa"), str(tmp_path / "missing.fq"), str(tmp_path / "o.sam"),
                       max_hits=0)
+
+
+def test_server_config_defaults_and_argument_errors(tmp_path):
+    """storm_server_config_default and the argument checks of storm_server_create (no GPU call)."""
+    import pytest
+    from storm_amd import lib
+    so = lib.load()
+    c = lib.ServerConfig()
+    so.storm_server_config_default(C.byref(c))
+    assert (c.mode, c.lanes, c.cross_request, c.max_batch_reads, c.encodings) == (lib.STORM_MODE_BATCHED, 3, 1,
+                                                                                  1 << 18, 3)
+    assert c.hbm_reserve_bytes == 8 << 30 and c.stdev_distance == 100
+    for bad in (dict(lanes=0), dict(lanes=17), dict(max_batch_reads=0), dict(encodings=0), dict(mode=7)):
+        with pytest.raises(lib.StormError):
+            lib.Server(str(tmp_path / "missing.fa"), **bad)
