@@ -208,9 +208,19 @@ def cpu_mapping_baseline(lib, d, gpath, pr, kw, color, n_sample):
     if ref_binary(color) is None:
         return None
     cores = host_cores()
-    n_sample = min(n_sample, len(pr.len))
-    r_tiny, _ = write_read_files(d, pr, np.arange(16), color, "tiny")
-    r_samp, files = write_read_files(d, pr, np.arange(n_sample), color, "sample")
+    # the reference maps one read length per run (reads_data.c:156-163): a mixed-length workload
+    # is sampled at its median length
+    lens = np.asarray(pr.len)
+    if lens.min() != lens.max():
+        L_med = int(np.median(lens))
+        pick = np.nonzero(lens == L_med)[0]
+        length_note = f" (all of length {L_med}, the median: the reference takes one read length per run)"
+    else:
+        pick = np.arange(len(lens))
+        length_note = ""
+    n_sample = min(n_sample, len(pick))
+    r_tiny, _ = write_read_files(d, pr, pick[:16], color, "tiny")
+    r_samp, files = write_read_files(d, pr, pick[:n_sample], color, "sample")
     t_fixed = run_reference(color, gpath, r_tiny, kw, os.path.join(d, "ref_tiny.sam"), cores)
     t_all = run_reference(color, gpath, r_samp, kw, os.path.join(d, "ref_sample.sam"), cores)
     gsam = os.path.join(d, "gpu_sample.sam")
@@ -218,7 +228,7 @@ def cpu_mapping_baseline(lib, d, gpath, pr, kw, color, n_sample):
     same = sam_body(gsam) == sam_body(os.path.join(d, "ref_sample.sam"))
     dt = max(t_all - t_fixed, 1e-6)
     return {"value": round(n_sample / dt, 1), "unit": "reads/s", "cores": cores, "kind": "reference",
-            "sample": f"{n_sample} reads of the workload; reference {os.path.basename(ref_binary(color))} -N {cores}: "
+            "sample": f"{n_sample} reads of the workload{length_note}; reference {os.path.basename(ref_binary(color))} -N {cores}: "
                       f"{t_all:.1f}s total minus {t_fixed:.1f}s fixed cost (load + index, measured on 16 reads)",
             "gpu_sam_identical_to_reference": bool(same)}
 

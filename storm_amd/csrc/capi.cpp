@@ -641,31 +641,30 @@ static int tune_fused(storm_mapper* m, int64_t n, int maxL, int max_grid, FusedT
 }
 
 // Reads flagged by the fused kernel (LDS capacity exceeded) go through the staged kernels for
-// this reference; their slots are gathered into a sub-batch and scattered back.
-int staged_subset(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order,
-                         const std::vector<int64_t>& dev_ids, int ref, int& pass_id, TbStore& store, PassStats& ps) {
+// this reference: their planes and slots are gathered on the device into a sub-batch, mapped,
+// and the slots scattered back.
+int staged_subset(storm_mapper* m, const std::vector<int64_t>& dev_ids, int ref, int& pass_id, TbStore& store,
+                  PassStats& ps) {
   FusedState& F = m->fs;
   const int64_t k = (int64_t)dev_ids.size();
-  std::vector<int64_t> sub((size_t)k);
-  for (int64_t i = 0; i < k; ++i) sub[(size_t)i] = order[(size_t)dev_ids[(size_t)i]];
+  std::vector<int32_t> ids((size_t)k);
+  for (int64_t i = 0; i < k; ++i) ids[(size_t)i] = (int32_t)dev_ids[(size_t)i];
+  DevMem d_ids, sl;
   DevReads dr;
-  int rc = upload_reads(m, v, sub, dr);
+  HIPOK(d_ids.ensure((size_t)k * 4));
+  HIPOK(hipMemcpyAsync(d_ids.p, ids.data(), (size_t)k * 4, hipMemcpyHostToDevice, m->st));
+  HIPOK(dr.planes.ensure((size_t)(2 * k) * 2 * RW * 4));
+  HIPOK(dr.qplanes.ensure((size_t)(2 * k) * QW * 4));
+  HIPOK(dr.len.ensure((size_t)k * 4));
+  HIPOK(sl.ensure((size_t)(4 * k) * sizeof(Slot)));
+  dr.n = k;
+  dr.has_qual = F.reads.has_qual;
+  HIPOK(launch_subset_gather(F.reads.planes.as<uint32_t>(), F.reads.qplanes.as<uint32_t>(), F.reads.len.as<int32_t>(),
+                             F.slots.as<Slot>(), d_ids.as<int32_t>(), k, dr.planes.as<uint32_t>(),
+                             dr.qplanes.as<uint32_t>(), dr.len.as<int32_t>(), sl.as<Slot>(), m->st));
+  int rc = run_pass(m, dr, sl.as<Slot>(), ref, pass_id++, 0, k, store, ps);
   if (rc) return rc;
-  std::vector<Slot> hs((size_t)(4 * k));
-  for (int64_t i = 0; i < k; ++i)
-    HIPOK(hipMemcpyAsync(&hs[(size_t)(4 * i)], F.slots.as<Slot>() + 4 * dev_ids[(size_t)i], 4 * sizeof(Slot),
-                         hipMemcpyDeviceToHost, m->st));
-  HIPOK(hipStreamSynchronize(m->st));
-  DevMem sl;
-  HIPOK(sl.ensure(hs.size() * sizeof(Slot)));
-  HIPOK(hipMemcpyAsync(sl.p, hs.data(), hs.size() * sizeof(Slot), hipMemcpyHostToDevice, m->st));
-  rc = run_pass(m, dr, sl.as<Slot>(), ref, pass_id++, 0, k, store, ps);
-  if (rc) return rc;
-  HIPOK(hipMemcpyAsync(hs.data(), sl.p, hs.size() * sizeof(Slot), hipMemcpyDeviceToHost, m->st));
-  HIPOK(hipStreamSynchronize(m->st));
-  for (int64_t i = 0; i < k; ++i)
-    HIPOK(hipMemcpyAsync(F.slots.as<Slot>() + 4 * dev_ids[(size_t)i], &hs[(size_t)(4 * i)], 4 * sizeof(Slot),
-                         hipMemcpyHostToDevice, m->st));
+  HIPOK(launch_subset_scatter(sl.as<Slot>(), d_ids.as<int32_t>(), k, F.slots.as<Slot>(), m->st));
   HIPOK(hipStreamSynchronize(m->st));
   return STORM_OK;
 }
@@ -779,7 +778,7 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
   unsigned long long totals[4] = {0, 0, 0, 0};
   int64_t n_fallback = 0;
   FallbackFn fb = [&](const std::vector<int64_t>& over, int ref, int& pass_id, TbStore& st, PassStats& pst) {
-    return staged_subset(m, v, order, over, ref, pass_id, st, pst);
+    return staged_subset(m, over, ref, pass_id, st, pst);
   };
   rc = fused_core(m, n, maxL, fb, store, ps, totals, n_fallback);
   if (rc) return rc;

@@ -55,6 +55,68 @@ __device__ __forceinline__ void wave_bitonic_sort(int32_t* buf, int n) {
   }
 }
 
+// Sorts buf[0..n) (LDS, n <= 64 E) in registers and applies the keep rule: E elements per lane,
+// element lane*E + k in v[k].  Bitonic network over 64 E keys padded with INT_MAX: compare-exchange
+// stages at distance j < E stay inside a lane; longer ones pair lane with lane ^ (j / E) through a
+// cross-lane shuffle, so no stage waits on LDS or a barrier.  The kept diagonals (first one only if
+// > -I, then each one larger than its predecessor) go to out[0..) in order; returns their number,
+// or -1 when more than cap.
+template <int E>
+__device__ int wave_sort_keep(const int32_t* buf, int n, int I, int32_t* out, int cap) {
+  const int lane = threadIdx.x & 63;
+  int32_t v[E];
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int e = lane * E + k;
+    v[k] = e < n ? buf[e] : 0x7fffffff;
+  }
+#pragma unroll
+  for (int kk = 2; kk <= 64 * E; kk <<= 1) {
+#pragma unroll
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      if (j >= E) {
+        const int lj = j / E;
+        const bool lower = (lane & lj) == 0;
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+          const bool up = ((lane * E + k) & kk) == 0;
+          const int32_t w = __shfl_xor(v[k], lj, 64);
+          v[k] = (lower == up) ? min(v[k], w) : max(v[k], w);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < E; ++k) {
+          if ((k & j) == 0) {
+            const bool up = ((lane * E + k) & kk) == 0;
+            const int32_t a = v[k], b = v[k + j];
+            v[k] = up ? min(a, b) : max(a, b);
+            v[k + j] = up ? max(a, b) : min(a, b);
+          }
+        }
+      }
+    }
+  }
+  const int32_t prev_last = __shfl_up(v[E - 1], 1, 64);
+  int cnt = 0;
+  uint32_t keep = 0;
+#pragma unroll
+  for (int k = 0; k < E; ++k) {
+    const int e = lane * E + k;
+    const int32_t p = k ? v[k - 1] : prev_last;
+    const bool kp = (e < n) && ((e == 0) ? (v[k] > -I) : (v[k] > p));
+    keep |= (uint32_t)kp << k;
+    cnt += kp;
+  }
+  const int incl = wave_scan_incl_i32(cnt);
+  const int total = __shfl(incl, 63, 64);
+  if (total > cap) return -1;
+  int o = incl - cnt;
+#pragma unroll
+  for (int k = 0; k < E; ++k)
+    if ((keep >> k) & 1u) out[o++] = v[k];
+  return total;
+}
+
 // All candidates of read strand rs, ascending, into out[0..), one wavefront.  hbuf is an LDS
 // buffer of `hcap` entries (hcap >= 64, power of two), lk an LDS buffer of 192 ints.  The raw hits
 // of 64 lookups are gathered cooperatively: hit e of the batch goes to lane e mod 64, which finds
@@ -105,6 +167,11 @@ __device__ int wave_candidates(const ReadsDev& r, const SeedSetDev& seeds, int64
   *raw_hits = run;
   if (over) return -1;
   if (run == 0) return 0;
+  if (run <= 64) return wave_sort_keep<1>(hbuf, run, I, out, cap);
+  if (run <= 128) return wave_sort_keep<2>(hbuf, run, I, out, cap);
+  if (run <= 256) return wave_sort_keep<4>(hbuf, run, I, out, cap);
+  if (run <= 512) return wave_sort_keep<8>(hbuf, run, I, out, cap);
+  if (run <= 1024) return wave_sort_keep<16>(hbuf, run, I, out, cap);
   wave_bitonic_sort(hbuf, run);
   int placed = 0;
   for (int e0 = 0; e0 < run; e0 += 64) {

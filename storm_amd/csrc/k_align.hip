@@ -3,8 +3,9 @@
 // Semantics: alignment__align + alignment__traceback (alignment.c:326-360, :488-632, :642-734)
 // on the window max(h - I, -d) of length L + 2I (run.c:667, :855), nucleotide or color space.
 // One thread owns one candidate that passed the filter; the 2I+1 band columns of a row live in
-// registers (template bound IM on I), the per-cell direction byte goes to a scratch matrix in
-// HBM and the same thread walks it back.
+// registers (template bound IM on I).  A score pass decides whether the record can ever be
+// accepted (score > t and S >= t); only those records are aligned again with the per-cell
+// direction byte in a scratch matrix in HBM, which the same thread walks back.
 //
 // The kernel is threshold-free: alongside the score it returns the survival bound
 //   S = min over rows i >= 1 of (max_on_line(i) + best_possible(i)),
@@ -21,8 +22,13 @@ __global__ __launch_bounds__(128) void k_align(AlignArgs a) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= a.n) return;
   const int64_t rec = a.base + k;
-  a.out[rec] = dp_one<IM, COLOR>(a.g, a.r, a.sc, a.rec_rs[rec], a.rec_diag[rec], a.scratch + k * a.scratch_stride,
-                                 a.tb + rec * TBB);
+  // score pass without the direction matrix; the records that can still be accepted (a few per
+  // cent of the filter passes) are aligned again with directions and traceback
+  AlignRec x = dp_one<IM, COLOR, false>(a.g, a.r, a.sc, a.rec_rs[rec], a.rec_diag[rec], nullptr, nullptr);
+  if (x.score > a.sc.t && x.S >= a.sc.t)
+    x = dp_one<IM, COLOR, true>(a.g, a.r, a.sc, a.rec_rs[rec], a.rec_diag[rec], a.scratch + k * a.scratch_stride,
+                                a.tb + rec * TBB);
+  a.out[rec] = x;
 }
 
 hipError_t launch_align(const AlignArgs& a, hipStream_t st) {

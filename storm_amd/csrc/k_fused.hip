@@ -6,8 +6,10 @@
 // of G consecutive reads (both strands).  Their candidates accumulate in the workgroup's own
 // slice of a global work buffer (diagonal, read strand, filter flag, record index), which stays
 // in L2; when the next read might not fit, the accumulated candidates are flushed: filtered two
-// per lane, the passes aligned one per lane, and the hit maps of the completed reads replayed
-// one lane per read.  So a group of reads keeps all 64 lanes busy when one read strand has only a
+// per lane, the passes scored one per lane without a direction matrix (stopping as soon as a
+// record can no longer be accepted), the few records that can still be accepted aligned again
+// with directions and traceback, and the hit maps of the completed reads replayed one lane per
+// read over those records only.  So a group of reads keeps all 64 lanes busy when one read strand has only a
 // few candidates, and a read with thousands of candidates (a large genome) fills the lanes by
 // itself.  Reads of any length share a task; one launch serves one band class (W, I).
 // A read whose raw hits exceed the LDS sort buffer, or whose flush has more passes than the
@@ -24,6 +26,8 @@ constexpr int F_HCAP = 2048;         // raw hits of one read strand (LDS sort bu
 constexpr int F_CCAP = FUSED_CCAP;   // candidates per flush (work buffer)
 constexpr int F_PCAP = FUSED_PCAP;   // filter passes (DP records) per flush
 constexpr int F_GMAX = 64;           // reads per task
+constexpr int F_PENDING = -3;        // Slot.pass while the entry's traceback is still in the flush buffer
+static_assert(TBB % 16 == 0, "traceback records are copied in 16-byte units");
 static_assert(F_CCAP >= 2 * F_HCAP, "a flush must hold both strands of one read");
 
 template <int W, int IM, bool COLOR>
@@ -93,13 +97,40 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 4 :
       __syncthreads();
     }
     if (lane == 0) { stl[0] += (unsigned long long)nc; stl[1] += (unsigned long long)np; }
-    // DP + traceback (stage d): lane l computes records l, l + 64, ...
+    // DP (stage d), score pass: lane l takes records l, l + 64, ...; nothing is stored per cell
+    // and a record stops at the first row where it can no longer be accepted
     const int npl = np < F_PCAP ? np : F_PCAP;
     for (int k = lane; k < npl; k += 64) {
       const int c = rec2cand[k];
       if (rovf[crs[c] >> 1]) continue;
       const int64_t rs = 2 * base + crs[c];
-      recs[k] = dp_one<IM, COLOR>(a.g, a.r, a.sc, (int)rs, cdiag[c], scratch, tbbuf + (int64_t)k * TBB);
+      recs[k] = dp_one<IM, COLOR, false>(a.g, a.r, a.sc, (int)rs, cdiag[c], nullptr, nullptr);
+    }
+    __syncthreads();
+    // the records that can still be accepted (score > t, S >= t), ascending, as
+    // (read strand << 16) | record in the hit buffer (free during a flush)
+    int nv = 0;
+    for (int b0 = 0; b0 < npl; b0 += 64) {
+      const int k = b0 + lane;
+      int v = 0, j = 0;
+      if (k < npl) {
+        j = crs[rec2cand[k]];
+        if (!rovf[j >> 1]) {
+          const AlignRec x = recs[k];
+          v = x.score > a.sc.t && x.S >= a.sc.t;
+        }
+      }
+      const int incl = wave_scan_incl_i32(v);
+      if (v) hbuf[nv + incl - 1] = (j << 16) | k;
+      nv += __shfl(incl, 63, 64);
+    }
+    __syncthreads();
+    // traceback pass over those records only: directions in the lane's scratch, codes to tbbuf
+    for (int e = lane; e < nv; e += 64) {
+      const int k = hbuf[e] & 0xffff;
+      const int c = rec2cand[k];
+      const int64_t rs = 2 * base + crs[c];
+      recs[k] = dp_one<IM, COLOR, true>(a.g, a.r, a.sc, (int)rs, cdiag[c], scratch, tbbuf + (int64_t)k * TBB);
     }
     __syncthreads();
     // hit map replay (run.c:660-697, hit_map.c:50-144): one lane per read
@@ -114,13 +145,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 4 :
       for (int k = 0; k < 4; ++k) s[k] = a.slots[read * 4 + k];
       const int L = a.r.len[read];
       bool dirty = false;
+      // this read's viable records: the entries of strand 2lr, then of strand 2lr + 1
+      int e = 0;
+      {
+        int lo = 0, hi = nv;   // first entry of strand >= 2lr
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if ((hbuf[mid] >> 16) < 2 * lr) lo = mid + 1;
+          else hi = mid;
+        }
+        e = lo;
+      }
       for (int strand = 0; strand < 2; ++strand) {
-        if (!((a.sense_mask >> strand) & 1)) continue;
         const int j = 2 * lr + strand;
         int T = a.sc.t;
-        for (int c = cstart[j]; c < cstart[j + 1]; ++c) {
-          const int kr = crec[c];
-          if (kr < 0) continue;
+        for (; e < nv && (hbuf[e] >> 16) == j; ++e) {
+          const int kr = hbuf[e] & 0xffff;
           const AlignRec x = recs[kr];
           if (!(x.S >= T && x.score > T)) continue;
           st_acc++;
@@ -166,22 +206,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 4 :
               Slot nw;
               nw.score = score; nw.ref_id = a.ref_id; nw.ref_start = ref_start; nw.ref_end = ref_end;
               nw.read_start = read_start; nw.read_end = read_end; nw.sense = strand ? -1 : 1;
-              nw.pass = PASS_FUSED; nw.rec = cell; nw.tb_len = x.tb_len; nw.cell = cell;
+              // the traceback stays in this flush's buffer (record kr) until the read is done
+              nw.pass = F_PENDING; nw.rec = kr; nw.tb_len = x.tb_len; nw.cell = cell;
 #pragma unroll
               for (int q = 0; q < 4; ++q) if (q == p) s[q] = nw;
               dirty = true;
-              // the traceback goes into the freed cell
-              const uint32_t* src = (const uint32_t*)(tbbuf + (int64_t)kr * TBB);
-              uint32_t* dst = (uint32_t*)(a.cells + (read * 4 + cell) * (int64_t)TBB);
-              for (int w = 0; w < TBB / 4; ++w) dst[w] = src[w];
             }
           }
           if (s[3].score > T) T = s[3].score;
         }
       }
       if (dirty) {
+        // tracebacks of the entries this flush inserted and kept, each into its own cell (the four
+        // slots hold a permutation of the read's four cells), then the slots
 #pragma unroll
-        for (int k = 0; k < 4; ++k) a.slots[read * 4 + k] = s[k];
+        for (int k = 0; k < 4; ++k) {
+          if (s[k].pass == F_PENDING) {
+            const uint4* src = (const uint4*)(tbbuf + (int64_t)s[k].rec * TBB);
+            uint4* dst = (uint4*)(a.cells + (read * 4 + s[k].cell) * (int64_t)TBB);
+            const int n16 = ((s[k].tb_len + 1) / 2 + 15) / 16;
+            for (int w = 0; w < n16; ++w) dst[w] = src[w];
+            s[k].pass = PASS_FUSED;
+            s[k].rec = s[k].cell;
+          }
+          a.slots[read * 4 + k] = s[k];
+        }
       }
     }
 #pragma unroll

@@ -97,6 +97,45 @@ __global__ __launch_bounds__(256) void k_gather_tb(const uint8_t* tb, int tbb, c
 }
 
 
+// reads ids[0..k) of a batch (strand planes, quality planes, length, the four slots) into a
+// contiguous subset, for the staged pass over the reads the fused kernel handed back
+__global__ __launch_bounds__(256) void k_subset_gather(const uint32_t* planes, const uint32_t* qplanes, const int32_t* len,
+                                                       const Slot* slots, const int32_t* ids, int64_t k, uint32_t* sp,
+                                                       uint32_t* sq, int32_t* sl, Slot* ss) {
+  constexpr int PW = 2 * 2 * RW, QWS = 2 * QW;   // words per read: both strands
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= k * (PW + QWS)) return;
+  const int64_t b = t / (PW + QWS);
+  const int w = (int)(t % (PW + QWS));
+  const int64_t src = ids[b];
+  if (w < PW) sp[b * PW + w] = planes[src * PW + w];
+  else sq[b * QWS + (w - PW)] = qplanes[src * QWS + (w - PW)];
+  if (w < 4) ss[b * 4 + w] = slots[src * 4 + w];
+  if (w == 0) sl[b] = len[src];
+}
+
+__global__ __launch_bounds__(256) void k_subset_scatter(const Slot* ss, const int32_t* ids, int64_t k, Slot* slots) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * k) return;
+  slots[(int64_t)ids[t >> 2] * 4 + (t & 3)] = ss[t];
+}
+
+hipError_t launch_subset_gather(const uint32_t* planes, const uint32_t* qplanes, const int32_t* len, const Slot* slots,
+                                const int32_t* ids, int64_t k, uint32_t* sp, uint32_t* sq, int32_t* sl, Slot* ss,
+                                hipStream_t st) {
+  if (k == 0) return hipSuccess;
+  const int64_t total = k * (2 * 2 * RW + 2 * QW);
+  hipLaunchKernelGGL(k_subset_gather, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, planes, qplanes, len, slots,
+                     ids, k, sp, sq, sl, ss);
+  return hipGetLastError();
+}
+
+hipError_t launch_subset_scatter(const Slot* ss, const int32_t* ids, int64_t k, Slot* slots, hipStream_t st) {
+  if (k == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_subset_scatter, dim3((unsigned)((4 * k + 255) / 256)), dim3(256), 0, st, ss, ids, k, slots);
+  return hipGetLastError();
+}
+
 hipError_t launch_replay(const ReplayArgs& a, hipStream_t st) {
   if (a.n_reads == 0) return hipSuccess;
   hipLaunchKernelGGL(k_replay, dim3((unsigned)((a.n_reads + 127) / 128)), dim3(128), 0, st, a);

@@ -168,8 +168,9 @@ int prep_device_reads(storm_mapper* m, int64_t n, int64_t code_stride, int64_t q
 int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order, DevReads& d);
 int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int pass_id, int64_t r0, int64_t r1,
              TbStore& store, PassStats& stats);
-int staged_subset(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order,
-                  const std::vector<int64_t>& dev_ids, int ref, int& pass_id, TbStore& store, PassStats& ps);
+// the staged kernels over reads dev_ids of m->fs.reads, slots read from and written back to m->fs.slots
+int staged_subset(storm_mapper* m, const std::vector<int64_t>& dev_ids, int ref, int& pass_id, TbStore& store,
+                  PassStats& ps);
 
 // Reads that the fused kernel handed back (device ids of m->fs.reads) for reference `ref`; the
 // callback maps them with the staged kernels and leaves their slots in m->fs.slots.

@@ -212,39 +212,6 @@ static void select_class(storm_mapper* m, storm_server* s, const storm_params& p
   m->p.ignore_lowercase = s->cfg.ignore_lowercase;
 }
 
-// The staged kernels for reads the fused kernel handed back: a host view of just those reads,
-// built from the requests' host copies with the same rules as the device gather.
-static int serve_fallback(storm_mapper* m, const std::vector<ServeRow>& rows, const std::vector<Request*>& regs,
-                          const std::vector<int64_t>& dev_ids, int ref, int& pass_id, TbStore& store, PassStats& ps) {
-  const int64_t k = (int64_t)dev_ids.size();
-  std::vector<uint8_t> codes((size_t)k * SERVE_CS, 0), qual((size_t)k * SERVE_CS, 0), fq((size_t)k, 0);
-  std::vector<int32_t> len((size_t)k);
-  std::vector<int64_t> order((size_t)(rows.size()), 0);
-  for (int64_t i = 0; i < k; ++i) {
-    const ServeRow& rw = rows[(size_t)dev_ids[(size_t)i]];
-    const Request& r = *regs[(size_t)rw.region];
-    order[(size_t)dev_ids[(size_t)i]] = i;
-    memcpy(&codes[(size_t)i * SERVE_CS], &r.codes[(size_t)(rw.row * r.cs)], (size_t)std::min<int64_t>(r.cs, SERVE_CS));
-    if (r.has_qual) {
-      memcpy(&qual[(size_t)i * SERVE_CS], &r.qual[(size_t)(rw.row * r.qs)], (size_t)std::min<int64_t>(r.qs, SERVE_CS));
-      fq[(size_t)i] = r.has_fq ? r.fq[(size_t)rw.row] : 0;
-    } else {
-      memset(&qual[(size_t)i * SERVE_CS], 0xff, SERVE_CS);
-      fq[(size_t)i] = 3;
-    }
-    len[(size_t)i] = r.len[(size_t)rw.row];
-  }
-  storm_reads_view v{};
-  v.n_reads = k;
-  v.codes = codes.data();
-  v.code_stride = SERVE_CS;
-  v.qual = qual.data();
-  v.qual_stride = SERVE_CS;
-  v.len = len.data();
-  v.first_qual = fq.data();
-  return staged_subset(m, &v, order, dev_ids, ref, pass_id, store, ps);
-}
-
 // One batch of the continuous-batching path on lane `ln`.
 static int run_batch(storm_server* s, storm_server::Lane& ln, const std::vector<Segment>& segs) {
   storm_mapper* m = ln.m.get();
@@ -299,9 +266,8 @@ static int run_batch(storm_server* s, storm_server::Lane& ln, const std::vector<
   PassStats ps;
   unsigned long long totals[4];
   int64_t n_fallback = 0;
-  const std::vector<ServeRow>& rows = ln.h_rows;
   FallbackFn fb = [&](const std::vector<int64_t>& over, int ref, int& pass_id, TbStore& sto, PassStats& pst) {
-    return serve_fallback(m, rows, regs, over, ref, pass_id, sto, pst);
+    return staged_subset(m, over, ref, pass_id, sto, pst);
   };
   rc = fused_core(m, n, maxL, fb, store, ps, totals, n_fallback);
   if (rc) return rc;

@@ -188,6 +188,10 @@ hipError_t launch_mark_referenced(const Slot* slots, int64_t n_slots, int pass, 
 hipError_t launch_gather_cells(const Slot* slots, int64_t n_slots, int max_rank, const uint8_t* cells, int32_t* out_slot,
                                uint8_t* out_tb, int32_t* counter, hipStream_t st);
 hipError_t launch_gather_tb(const uint8_t* tb, const int32_t* recs, int n, uint8_t* out, hipStream_t st);
+hipError_t launch_subset_gather(const uint32_t* planes, const uint32_t* qplanes, const int32_t* len, const Slot* slots,
+                                const int32_t* ids, int64_t k, uint32_t* sp, uint32_t* sq, int32_t* sl, Slot* ss,
+                                hipStream_t st);
+hipError_t launch_subset_scatter(const Slot* ss, const int32_t* ids, int64_t k, Slot* slots, hipStream_t st);
 hipError_t launch_overflow_ranges(const int64_t* seg_off, const uint8_t* overflow, int64_t n_rs, int64_t* begin,
                                   int64_t* end, hipStream_t st);
 // device-wide helpers (hipcub)
