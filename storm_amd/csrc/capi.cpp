@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -133,43 +134,99 @@ static SeedSetDev seed_set(const RefState& rs) {
   return s;
 }
 
-// Uploads the reads order[0..) of the view (all of them, or the subset the fused kernel handed
-// to the staged kernels), in that order.
+// Stable order of the view's reads by length (counting sort: lengths are 1..MAXL); identity
+// reports whether that order is the view's own.
+void length_order(const storm_reads_view* v, std::vector<int64_t>& order, bool& identity) {
+  const int64_t n = v->n_reads;
+  auto bucket = [&](int64_t i) { return (size_t)std::min(std::max((int)v->len[i], 0), MAXL + 1); };
+  std::vector<int64_t> start(MAXL + 3, 0);
+  identity = true;
+  for (int64_t i = 0; i < n; ++i) {
+    start[bucket(i) + 1]++;
+    if (i && v->len[i] < v->len[i - 1]) identity = false;
+  }
+  for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
+  order.resize((size_t)n);
+  for (int64_t i = 0; i < n; ++i) order[(size_t)start[bucket(i)]++] = i;
+}
+
+// Uploads the view's reads and arranges them on the device in order[0..) (a permutation of the
+// view).  The caller's arrays go to the device as they are; a reordering is a device gather.
 int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order, DevReads& d) {
-  const int64_t n = (int64_t)order.size();
+  const int64_t n = v->n_reads;
+  if ((int64_t)order.size() != n) return fail(STORM_ERR_INVALID, "read order does not cover the view");
+  const int w = filter_lanes(m->p.allowed_indels >= 16 ? 15 : m->p.allowed_indels);
+  bool identity = true;
+  for (int64_t i = 0; i < n; ++i) {
+    const int L = v->len[i];
+    if (L <= 0 || L > MAXL)
+      return fail(STORM_ERR_INVALID, "read %lld has length %d (1..%d supported)", (long long)i, L, MAXL);
+    if (L < w)
+      return fail(STORM_ERR_INVALID, "read %lld is shorter (%d) than the filter band lanes (%d)", (long long)i, L, w);
+    if (order[(size_t)i] != i) identity = false;
+  }
   d.n = n;
   d.has_qual = v->qual != nullptr;
   const int64_t cs = v->code_stride, qs = v->qual ? v->qual_stride : 0;
-  std::vector<uint8_t> codes((size_t)(n * cs));
-  std::vector<uint8_t> qual(qs ? (size_t)(n * qs) : 0);
-  std::vector<int32_t> len((size_t)n);
-  std::vector<uint8_t> fq(v->first_qual ? (size_t)n : 0);
-  for (int64_t i = 0; i < n; ++i) {
-    const int64_t j = order[(size_t)i];
-    memcpy(&codes[(size_t)(i * cs)], v->codes + j * cs, (size_t)cs);
-    if (qs) memcpy(&qual[(size_t)(i * qs)], v->qual + j * qs, (size_t)qs);
-    len[(size_t)i] = v->len[j];
-    if (!fq.empty()) fq[(size_t)i] = v->first_qual[j];
-    if (len[(size_t)i] <= 0 || len[(size_t)i] > MAXL)
-      return fail(STORM_ERR_INVALID, "read %lld has length %d (1..%d supported)", (long long)j, len[(size_t)i], MAXL);
-    const int w = filter_lanes(m->p.allowed_indels >= 16 ? 15 : m->p.allowed_indels);
-    if (len[(size_t)i] < w)
-      return fail(STORM_ERR_INVALID, "read %lld is shorter (%d) than the filter band lanes (%d)", (long long)j,
-                  len[(size_t)i], w);
+  const bool has_fq = v->first_qual != nullptr;
+  HIPOK(d.codes.ensure((size_t)(n * cs)));
+  HIPOK(d.len.ensure((size_t)n * 4));
+  if (qs) HIPOK(d.qual.ensure((size_t)(n * qs)));
+  if (has_fq) HIPOK(d.first_qual.ensure((size_t)n));
+  DevReads& src = identity ? d : m->stage;
+  if (!identity) {
+    HIPOK(src.codes.ensure((size_t)(n * cs)));
+    HIPOK(src.len.ensure((size_t)n * 4));
+    if (qs) HIPOK(src.qual.ensure((size_t)(n * qs)));
+    if (has_fq) HIPOK(src.first_qual.ensure((size_t)n));
   }
-  HIPOK(d.codes.ensure(codes.size()));
-  HIPOK(d.len.ensure(len.size() * 4));
-  HIPOK(hipMemcpyAsync(d.codes.p, codes.data(), codes.size(), hipMemcpyHostToDevice, m->st));
-  HIPOK(hipMemcpyAsync(d.len.p, len.data(), len.size() * 4, hipMemcpyHostToDevice, m->st));
-  if (qs) {
-    HIPOK(d.qual.ensure(qual.size()));
-    HIPOK(hipMemcpyAsync(d.qual.p, qual.data(), qual.size(), hipMemcpyHostToDevice, m->st));
+  HIPOK(hipMemcpyAsync(src.codes.p, v->codes, (size_t)(n * cs), hipMemcpyHostToDevice, m->st));
+  HIPOK(hipMemcpyAsync(src.len.p, v->len, (size_t)n * 4, hipMemcpyHostToDevice, m->st));
+  if (qs) HIPOK(hipMemcpyAsync(src.qual.p, v->qual, (size_t)(n * qs), hipMemcpyHostToDevice, m->st));
+  if (has_fq) HIPOK(hipMemcpyAsync(src.first_qual.p, v->first_qual, (size_t)n, hipMemcpyHostToDevice, m->st));
+  if (!identity) {
+    std::vector<int32_t> o32((size_t)n);
+    for (int64_t i = 0; i < n; ++i) o32[(size_t)i] = (int32_t)order[(size_t)i];
+    HIPOK(m->order_dev.ensure((size_t)n * 4));
+    HIPOK(hipMemcpyAsync(m->order_dev.p, o32.data(), (size_t)n * 4, hipMemcpyHostToDevice, m->st));
+    const int32_t* od = m->order_dev.as<int32_t>();
+    HIPOK(launch_permute_bytes(src.codes.as<uint8_t>(), cs, od, n, d.codes.as<uint8_t>(), m->st));
+    HIPOK(launch_permute_i32(src.len.as<int32_t>(), od, n, d.len.as<int32_t>(), m->st));
+    if (qs) HIPOK(launch_permute_bytes(src.qual.as<uint8_t>(), qs, od, n, d.qual.as<uint8_t>(), m->st));
+    if (has_fq) HIPOK(launch_permute_bytes(src.first_qual.as<uint8_t>(), 1, od, n, d.first_qual.as<uint8_t>(), m->st));
+    HIPOK(hipStreamSynchronize(m->st));   // o32 is released on return
   }
-  if (!fq.empty()) {
-    HIPOK(d.first_qual.ensure(fq.size()));
-    HIPOK(hipMemcpyAsync(d.first_qual.p, fq.data(), fq.size(), hipMemcpyHostToDevice, m->st));
+  return prep_device_reads(m, n, cs, qs, qs != 0, has_fq, d);
+}
+
+// the slots (device order `order`) into the caller's four arrays (view order); returns mapped reads
+static int hits_to_host(storm_mapper* m, const Slot* slots, const std::vector<int64_t>& order, int64_t n,
+                        const HitsOut& ho, int64_t& mapped) {
+  bool identity = true;
+  for (int64_t i = 0; i < n && identity; ++i) identity = order[(size_t)i] == i;
+  if (!identity) {
+    std::vector<int32_t> o32((size_t)n);
+    for (int64_t i = 0; i < n; ++i) o32[(size_t)i] = (int32_t)order[(size_t)i];
+    HIPOK(m->order_dev.ensure((size_t)n * 4));
+    HIPOK(hipMemcpyAsync(m->order_dev.p, o32.data(), (size_t)n * 4, hipMemcpyHostToDevice, m->st));
+    HIPOK(hipStreamSynchronize(m->st));
   }
-  return prep_device_reads(m, n, cs, qs, qs != 0, !fq.empty(), d);
+  HIPOK(m->hits_dev.ensure((size_t)(4 * n) * 16));
+  int32_t* hd = m->hits_dev.as<int32_t>();
+  HIPOK(launch_hits_out(slots, identity ? nullptr : m->order_dev.as<int32_t>(), n, hd, hd + 4 * n, hd + 8 * n,
+                        hd + 12 * n, m->st));
+  std::vector<int32_t> ref_tmp;
+  int32_t* ref_host = ho.ref_id;
+  if (!ref_host) { ref_tmp.resize((size_t)(4 * n)); ref_host = ref_tmp.data(); }
+  const size_t bytes = (size_t)(4 * n) * 4;
+  if (ho.score) HIPOK(hipMemcpyAsync(ho.score, hd, bytes, hipMemcpyDeviceToHost, m->st));
+  HIPOK(hipMemcpyAsync(ref_host, hd + 4 * n, bytes, hipMemcpyDeviceToHost, m->st));
+  if (ho.ref_start) HIPOK(hipMemcpyAsync(ho.ref_start, hd + 8 * n, bytes, hipMemcpyDeviceToHost, m->st));
+  if (ho.sense) HIPOK(hipMemcpyAsync(ho.sense, hd + 12 * n, bytes, hipMemcpyDeviceToHost, m->st));
+  HIPOK(hipStreamSynchronize(m->st));
+  mapped = 0;
+  for (int64_t i = 0; i < n; ++i) mapped += ref_host[(size_t)(4 * i)] != -1;
+  return STORM_OK;
 }
 
 int prep_device_reads(storm_mapper* m, int64_t n, int64_t code_stride, int64_t qual_stride, bool has_qual,
@@ -197,6 +254,34 @@ static float elapsed(storm_mapper* m, hipEvent_t a, hipEvent_t b) {
   float ms = 0;
   (void)hipEventElapsedTime(&ms, a, b);
   return ms;
+}
+
+// PASS_HIT_LIMIT, or the value of STORM_PASS_HIT_LIMIT in the environment (read once; tests use a
+// small value to exercise the splitting)
+static int64_t pass_hit_limit() {
+  static const int64_t v = [] {
+    const char* e = getenv("STORM_PASS_HIT_LIMIT");
+    const long long x = e ? atoll(e) : 0;
+    return x > 0 ? (int64_t)x : PASS_HIT_LIMIT;
+  }();
+  return v;
+}
+
+// Reads [r0, r1) in passes of at most `chunk` reads, each halved while its raw hits exceed
+// pass_hit_limit().
+int run_passes(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int& pass_id, int64_t r0, int64_t r1,
+               TbStore& store, PassStats& stats, int64_t chunk) {
+  for (int64_t c0 = r0; c0 < r1;) {
+    const int64_t c1 = std::min<int64_t>(r1, c0 + chunk);
+    const int rc = run_pass(m, dr, slots, ref_id, pass_id++, c0, c1, store, stats);
+    if (rc == PASS_SPLIT) {
+      chunk = std::max<int64_t>(1, (c1 - c0) / 2);
+      continue;
+    }
+    if (rc) return rc;
+    c0 = c1;
+  }
+  return STORM_OK;
 }
 
 // One mapping pass of reads [r0, r1) (device order) against one reference sequence: the stages
@@ -243,6 +328,14 @@ int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int pass_id
   int64_t total = 0;
   HIPOK(hipMemcpyAsync(&total, P.seg_off.as<int64_t>() + n_rs, 8, hipMemcpyDeviceToHost, st));
   HIPOK(hipStreamSynchronize(st));
+  // the device scans and sorts count items in 32 bits and the per-pass buffers grow with the raw
+  // hits: a pass over more than PASS_HIT_LIMIT hits is split by the caller (nothing written yet)
+  const int64_t limit = pass_hit_limit();
+  if (total > limit) {
+    if (r1 - r0 > 1) return PASS_SPLIT;
+    return fail(STORM_ERR_MEMORY, "read %lld has %lld raw index hits (more than %lld); filter repeats with a lower -l",
+                (long long)r0, (long long)total, (long long)limit);
+  }
   stats.hits += total;
   HIPOK(P.hits.ensure((size_t)std::max<int64_t>(total, 1) * 4));
   HIPOK(P.kept.ensure((size_t)std::max<int64_t>(total, 1) * 4));
@@ -468,17 +561,14 @@ static int assemble_slots(storm_mapper* m, const std::vector<Slot>& hs, const st
 }
 
 static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
-                           storm_map_stats* st_out);
-
-static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
-                           storm_map_stats* st_out) {
+                           storm_map_stats* st_out, const HitsOut* ho) {
   const int64_t n = v->n_reads;
   if (n <= 0) return fail(STORM_ERR_INVALID, "no reads");
   // plain mode: equal-length batches (the reference maps one read length per run); the device
   // order groups reads by length, stable within a length
-  std::vector<int64_t> order((size_t)n);
-  for (int64_t i = 0; i < n; ++i) order[(size_t)i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return v->len[a] < v->len[b]; });
+  std::vector<int64_t> order;
+  bool sorted_view = true;
+  length_order(v, order, sorted_view);
   std::vector<std::pair<int64_t, int64_t>> batches;
   if (mode == STORM_MODE_PLAIN) {
     int64_t s = 0;
@@ -503,20 +593,22 @@ static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode,
     if (rc) return rc;
     for (auto& b : batches) {
       // bound the per-pass buffers: at most 1M reads per launch group
-      for (int64_t c0 = b.first; c0 < b.second; c0 += (1 << 20)) {
-        const int64_t c1 = std::min<int64_t>(b.second, c0 + (1 << 20));
-        rc = run_pass(m, dr, slots.as<Slot>(), (int)r, pass_id++, c0, c1, store, ps);
-        if (rc) return rc;
-      }
+      rc = run_passes(m, dr, slots.as<Slot>(), (int)r, pass_id, b.first, b.second, store, ps, 1 << 20);
+      if (rc) return rc;
     }
     if (m->res->refs.size() > 1 && !m->res->pinned) release_ref(m, r);
   }
-  std::vector<Slot> hs((size_t)(4 * n));
-  HIPOK(hipMemcpyAsync(hs.data(), slots.p, hs.size() * sizeof(Slot), hipMemcpyDeviceToHost, m->st));
-  HIPOK(hipStreamSynchronize(m->st));
   int64_t mapped = 0;
-  rc = assemble_slots(m, hs, order, store, nullptr, out_slots, mapped);
-  if (rc) return rc;
+  if (ho) {
+    rc = hits_to_host(m, slots.as<Slot>(), order, n, *ho, mapped);
+    if (rc) return rc;
+  } else {
+    std::vector<Slot> hs((size_t)(4 * n));
+    HIPOK(hipMemcpyAsync(hs.data(), slots.p, hs.size() * sizeof(Slot), hipMemcpyDeviceToHost, m->st));
+    HIPOK(hipStreamSynchronize(m->st));
+    rc = assemble_slots(m, hs, order, store, nullptr, out_slots, mapped);
+    if (rc) return rc;
+  }
   if (st_out) {
     st_out->n_reads = n;
     st_out->n_mapped = mapped;
@@ -662,7 +754,7 @@ int staged_subset(storm_mapper* m, const std::vector<int64_t>& dev_ids, int ref,
   HIPOK(launch_subset_gather(F.reads.planes.as<uint32_t>(), F.reads.qplanes.as<uint32_t>(), F.reads.len.as<int32_t>(),
                              F.slots.as<Slot>(), d_ids.as<int32_t>(), k, dr.planes.as<uint32_t>(),
                              dr.qplanes.as<uint32_t>(), dr.len.as<int32_t>(), sl.as<Slot>(), m->st));
-  int rc = run_pass(m, dr, sl.as<Slot>(), ref, pass_id++, 0, k, store, ps);
+  int rc = run_passes(m, dr, sl.as<Slot>(), ref, pass_id, 0, k, store, ps, k);
   if (rc) return rc;
   HIPOK(launch_subset_scatter(sl.as<Slot>(), d_ids.as<int32_t>(), k, F.slots.as<Slot>(), m->st));
   HIPOK(hipStreamSynchronize(m->st));
@@ -683,11 +775,6 @@ int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback,
   HIPOK(launch_slots_init(F.slots.as<Slot>(), 4 * n, m->st));
   const int32_t n32 = (int32_t)n;
   const size_t n_refs = m->res->refs.size();
-  if (F.graphs.size() != n_refs) {
-    F.drop_graphs();
-    F.graphs.assign(n_refs, nullptr);
-    F.graph_sig.assign(n_refs, {});
-  }
   int pass_id = 0;
   for (int q = 0; q < 4; ++q) totals[q] = 0;
   n_fallback = 0;
@@ -711,8 +798,9 @@ int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback,
                                     (const void*)(intptr_t)a.scratch_stride, (const void*)(intptr_t)a.sense_mask};
     const int* scw = (const int*)&a.sc;
     for (size_t q = 0; q < sizeof(ScoreDev) / sizeof(int); ++q) sig.push_back((const void*)(intptr_t)scw[q]);
-    if (!F.graphs[r] || F.graph_sig[r] != sig) {
-      if (F.graphs[r]) { (void)hipGraphExecDestroy(F.graphs[r]); F.graphs[r] = nullptr; }
+    sig.push_back((const void*)r);
+    hipGraphExec_t exec = F.find_graph(sig);
+    if (!exec) {
       hipGraph_t g;
       HIPOK(hipStreamBeginCapture(m->st, hipStreamCaptureModeThreadLocal));
       hipError_t e1 = launch_fused_reset(a.task_counter, a.stats, m->st);
@@ -720,17 +808,15 @@ int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback,
       HIPOK(hipStreamEndCapture(m->st, &g));
       HIPOK(e1);
       HIPOK(e2);
-      hipGraphExec_t ex;
-      HIPOK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+      HIPOK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
       (void)hipGraphDestroy(g);
-      F.graphs[r] = ex;
-      F.graph_sig[r] = sig;
+      F.add_graph(sig, exec);
     }
     hipEvent_t e0, e1;
     HIPOK(hipEventCreate(&e0));
     HIPOK(hipEventCreate(&e1));
     HIPOK(hipEventRecord(e0, m->st));
-    HIPOK(hipGraphLaunch(F.graphs[r], m->st));
+    HIPOK(hipGraphLaunch(exec, m->st));
     HIPOK(hipEventRecord(e1, m->st));
     unsigned long long st3[4];
     HIPOK(hipMemcpyAsync(st3, F.stats.p, sizeof st3, hipMemcpyDeviceToHost, m->st));
@@ -751,22 +837,25 @@ int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback,
     }
     if (n_refs > 1 && !m->res->pinned) {
       release_ref(m, r);
-      if (F.graphs[r]) { (void)hipGraphExecDestroy(F.graphs[r]); F.graphs[r] = nullptr; }
+      F.drop_graph(sig);   // its genome and index buffers are gone
     }
   }
   return STORM_OK;
 }
 
+static int batched_stats(storm_mapper* m, int64_t n, int64_t mapped, const unsigned long long totals[4],
+                         const PassStats& ps, int64_t n_fallback, double t0, double ti, storm_map_stats* st_out);
+
 static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vector<HostSlot>& out_slots,
-                            storm_map_stats* st_out) {
+                            storm_map_stats* st_out, const HitsOut* ho) {
   FusedState& F = m->fs;
   const int64_t n = v->n_reads;
   if (n <= 0) return fail(STORM_ERR_INVALID, "no reads");
   if (n > (int64_t)INT32_MAX / 4) return fail(STORM_ERR_INVALID, "batch too large");
   // device order: by length (tasks then hold reads of similar length)
-  std::vector<int64_t> order((size_t)n);
-  for (int64_t i = 0; i < n; ++i) order[(size_t)i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return v->len[a] < v->len[b]; });
+  std::vector<int64_t> order;
+  bool sorted_view = true;
+  length_order(v, order, sorted_view);
   int maxL = 0;
   for (int64_t i = 0; i < n; ++i) maxL = std::max(maxL, (int)v->len[i]);
   const double t0 = now_s();
@@ -782,6 +871,12 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
   };
   rc = fused_core(m, n, maxL, fb, store, ps, totals, n_fallback);
   if (rc) return rc;
+  int64_t mapped = 0;
+  if (ho) {
+    rc = hits_to_host(m, F.slots.as<Slot>(), order, n, *ho, mapped);
+    if (rc) return rc;
+    return batched_stats(m, n, mapped, totals, ps, n_fallback, t0, ti, st_out);
+  }
   // tracebacks of the fused slots that will be written (ranks < -M)
   int32_t ng = 0;
   std::vector<Slot> hs((size_t)(4 * n));
@@ -803,9 +898,13 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
   std::unordered_map<int64_t, const uint8_t*> cells;
   cells.reserve((size_t)ng * 2 + 1);
   for (int32_t q = 0; q < ng; ++q) cells[gids[(size_t)q]] = &gtb[(size_t)q * TBB];
-  int64_t mapped = 0;
   rc = assemble_slots(m, hs, order, store, &cells, out_slots, mapped);
   if (rc) return rc;
+  return batched_stats(m, n, mapped, totals, ps, n_fallback, t0, ti, st_out);
+}
+
+static int batched_stats(storm_mapper* m, int64_t n, int64_t mapped, const unsigned long long totals[4],
+                         const PassStats& ps, int64_t n_fallback, double t0, double ti, storm_map_stats* st_out) {
   if (st_out) {
     st_out->n_reads = n;
     st_out->n_mapped = mapped;
@@ -848,9 +947,9 @@ static int64_t count_lookups(const storm_mapper* m, const storm_reads_view* v) {
 }
 
 int map_view(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
-                    storm_map_stats* st_out) {
-  int rc = (mode == STORM_MODE_BATCHED) ? map_view_batched(m, v, out_slots, st_out)
-                                         : map_view_staged(m, v, mode, out_slots, st_out);
+             storm_map_stats* st_out, const HitsOut* ho) {
+  int rc = (mode == STORM_MODE_BATCHED) ? map_view_batched(m, v, out_slots, st_out, ho)
+                                         : map_view_staged(m, v, mode, out_slots, st_out, ho);
   if (rc == STORM_OK && st_out) st_out->n_lookups = count_lookups(m, v);
   return rc;
 }
@@ -980,16 +1079,11 @@ int storm_mapper_map_hits(storm_mapper* m, const storm_reads_view* reads, int mo
   HIPOK(hipSetDevice(m->device));
   std::vector<HostSlot> slots;
   storm_map_stats local{};
+  HitsOut ho{score, ref_id, ref_start, sense};
   m->want_tb = false;
-  int rc = map_view(m, reads, mode, slots, &local);
+  int rc = map_view(m, reads, mode, slots, &local, &ho);
   m->want_tb = true;
   if (rc) return rc;
-  for (size_t k = 0; k < slots.size(); ++k) {
-    if (score) score[k] = slots[k].score;
-    if (ref_id) ref_id[k] = slots[k].ref_id;
-    if (ref_start) ref_start[k] = slots[k].ref_start;
-    if (sense) sense[k] = slots[k].sense;
-  }
   if (stats) *stats = local;
   return STORM_OK;
 }

@@ -136,6 +136,55 @@ hipError_t launch_subset_scatter(const Slot* ss, const int32_t* ids, int64_t k, 
   return hipGetLastError();
 }
 
+// rows of a byte matrix (stride bytes per row) into the order given: dst row i = src row order[i]
+__global__ __launch_bounds__(256) void k_permute_bytes(const uint8_t* src, int64_t stride, const int32_t* order, int64_t n,
+                                                       uint8_t* dst) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * stride) return;
+  const int64_t i = t / stride, b = t % stride;
+  dst[t] = src[(int64_t)order[i] * stride + b];
+}
+
+__global__ __launch_bounds__(256) void k_permute_i32(const int32_t* src, const int32_t* order, int64_t n, int32_t* dst) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[t] = src[order[t]];
+}
+
+// the per-read slots (device order) as four int32 arrays in the caller's order: entry order[i]*4+k
+__global__ __launch_bounds__(256) void k_hits_out(const Slot* slots, const int32_t* order, int64_t n, int32_t* score,
+                                                  int32_t* ref_id, int32_t* ref_start, int32_t* sense) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * n) return;
+  const int64_t o = (order ? (int64_t)order[t >> 2] : (t >> 2)) * 4 + (t & 3);
+  const Slot s = slots[t];
+  score[o] = s.score;
+  ref_id[o] = s.ref_id;
+  ref_start[o] = s.ref_start;
+  sense[o] = s.sense;
+}
+
+hipError_t launch_permute_bytes(const uint8_t* src, int64_t stride, const int32_t* order, int64_t n, uint8_t* dst,
+                                hipStream_t st) {
+  if (n == 0 || stride == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_permute_bytes, dim3((unsigned)((n * stride + 255) / 256)), dim3(256), 0, st, src, stride, order, n,
+                     dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_permute_i32(const int32_t* src, const int32_t* order, int64_t n, int32_t* dst, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_permute_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, order, n, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_hits_out(const Slot* slots, const int32_t* order, int64_t n, int32_t* score, int32_t* ref_id,
+                           int32_t* ref_start, int32_t* sense, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_hits_out, dim3((unsigned)((4 * n + 255) / 256)), dim3(256), 0, st, slots, order, n, score, ref_id,
+                     ref_start, sense);
+  return hipGetLastError();
+}
+
 hipError_t launch_replay(const ReplayArgs& a, hipStream_t st) {
   if (a.n_reads == 0) return hipSuccess;
   hipLaunchKernelGGL(k_replay, dim3((unsigned)((a.n_reads + 127) / 128)), dim3(128), 0, st, a);

@@ -116,15 +116,36 @@ struct FusedState {
   DevMem slots, cells, overflow, n_dev, counter, stats, scratch, tbbuf, work, gids, gtb, gcount;
   int64_t cap_reads = 0, scratch_stride = 0;
   int grid = 0;
-  std::vector<hipGraphExec_t> graphs;   // per reference (nullptr = not captured)
-  std::vector<std::vector<const void*>> graph_sig;   // buffers and settings a graph was captured with
+  // captured graphs (reset + fused kernel) keyed by the buffers, settings and reference they were
+  // captured with; a context that alternates parameter classes keeps one per class
+  std::vector<std::pair<std::vector<const void*>, hipGraphExec_t>> graphs;
+  static constexpr size_t MAX_GRAPHS = 16;
   std::map<int64_t, FusedTune> tune;    // key: (W, I, color, length bucket)
   bool tuned_any = false;
-  ~FusedState() {
-    for (auto g : graphs) if (g) (void)hipGraphExecDestroy(g);
+  ~FusedState() { drop_graphs(); }
+  hipGraphExec_t find_graph(const std::vector<const void*>& sig) const {
+    for (const auto& g : graphs)
+      if (g.first == sig) return g.second;
+    return nullptr;
+  }
+  void add_graph(const std::vector<const void*>& sig, hipGraphExec_t e) {
+    if (graphs.size() >= MAX_GRAPHS) {
+      (void)hipGraphExecDestroy(graphs.front().second);
+      graphs.erase(graphs.begin());
+    }
+    graphs.emplace_back(sig, e);
+  }
+  void drop_graph(const std::vector<const void*>& sig) {
+    for (size_t k = 0; k < graphs.size(); ++k)
+      if (graphs[k].first == sig) {
+        (void)hipGraphExecDestroy(graphs[k].second);
+        graphs.erase(graphs.begin() + (long)k);
+        return;
+      }
   }
   void drop_graphs() {
-    for (auto& g : graphs) { if (g) (void)hipGraphExecDestroy(g); g = nullptr; }
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    graphs.clear();
   }
 };
 
@@ -148,6 +169,9 @@ struct storm_mapper {
   storm::PassScratch ps;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   storm::FusedState fs;
+  storm::DevReads stage;       // the caller's reads as uploaded, before the device reordering
+  storm::DevMem order_dev;     // device row -> caller's read index
+  storm::DevMem hits_dev;      // per-slot results on their way to the caller
   bool want_tb = true;   // false while a call returns hits only: tracebacks stay on the device
 };
 
@@ -166,8 +190,15 @@ int ensure_ref_resident(storm_mapper* m, size_t r);
 int prep_device_reads(storm_mapper* m, int64_t n, int64_t code_stride, int64_t qual_stride, bool has_qual,
                       bool has_first_qual, DevReads& d);
 int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order, DevReads& d);
+// one staged pass over reads [r0, r1) of dr; PASS_SPLIT (nothing written) when its raw index hits
+// exceed PASS_HIT_LIMIT
+constexpr int PASS_SPLIT = 1;
+constexpr int64_t PASS_HIT_LIMIT = 1LL << 30;
 int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int pass_id, int64_t r0, int64_t r1,
              TbStore& store, PassStats& stats);
+// reads [r0, r1) in passes of at most `chunk` reads, split further as PASS_SPLIT asks
+int run_passes(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int& pass_id, int64_t r0, int64_t r1,
+               TbStore& store, PassStats& stats, int64_t chunk);
 // the staged kernels over reads dev_ids of m->fs.reads, slots read from and written back to m->fs.slots
 int staged_subset(storm_mapper* m, const std::vector<int64_t>& dev_ids, int ref, int& pass_id, TbStore& store,
                   PassStats& ps);
@@ -182,7 +213,13 @@ using FallbackFn = std::function<int(const std::vector<int64_t>& dev_ids, int re
 int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback, TbStore& store, PassStats& ps,
                unsigned long long totals[4], int64_t& n_fallback);
 
+// caller arrays for the per-read slots (n_reads x 4 each, any may be null)
+struct HitsOut {
+  int32_t *score, *ref_id, *ref_start, *sense;
+};
+void length_order(const storm_reads_view* v, std::vector<int64_t>& order, bool& identity);
+// ho != null: the slots go straight to the caller's arrays (no host slot assembly, no tracebacks)
 int map_view(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
-             storm_map_stats* st_out);
+             storm_map_stats* st_out, const HitsOut* ho = nullptr);
 
 }  // namespace storm

@@ -227,19 +227,21 @@ static int run_batch(storm_server* s, storm_server::Lane& ln, const std::vector<
     }
     n += g.row1 - g.row0;
   }
-  ln.h_rows.clear();
-  ln.h_rows.reserve((size_t)n);
+  // the batch's reads ordered by length (counting sort; lengths are 1..MAXL, checked at submit)
+  std::vector<int64_t> start(MAXL + 2, 0);
   int maxL = 0;
+  for (const Segment& g : segs)
+    for (int64_t row = g.row0; row < g.row1; ++row) start[(size_t)g.r->len[(size_t)row] + 1]++;
+  for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
+  ln.h_rows.resize((size_t)n);
   for (const Segment& g : segs) {
     const int32_t ri = (int32_t)(std::find(regs.begin(), regs.end(), g.r) - regs.begin());
     for (int64_t row = g.row0; row < g.row1; ++row) {
-      ln.h_rows.push_back(ServeRow{ri, (int32_t)row});
-      maxL = std::max(maxL, (int)g.r->len[(size_t)row]);
+      const int L = g.r->len[(size_t)row];
+      ln.h_rows[(size_t)start[(size_t)L]++] = ServeRow{ri, (int32_t)row};
+      maxL = std::max(maxL, L);
     }
   }
-  std::stable_sort(ln.h_rows.begin(), ln.h_rows.end(), [&](const ServeRow& a, const ServeRow& b) {
-    return regs[(size_t)a.region]->len[(size_t)a.row] < regs[(size_t)b.region]->len[(size_t)b.row];
-  });
   for (Request* r : regs) HIPOK(hipStreamWaitEvent(m->st, r->ready, 0));
   HIPOK(ln.rows.ensure((size_t)n * sizeof(ServeRow)));
   HIPOK(ln.regions.ensure(ln.h_regions.size() * sizeof(ServeRegion)));
@@ -293,15 +295,14 @@ static int run_plain(storm_server* s, storm_server::Lane& ln, Request& r) {
   v.first_qual = r.has_fq ? r.fq.data() : nullptr;
   std::vector<HostSlot> slots;
   storm_map_stats ms{};
-  int rc = map_view(m, &v, STORM_MODE_PLAIN, slots, &ms);
+  const size_t ns = (size_t)r.n * 4;
+  std::vector<int32_t> f((size_t)4 * ns);
+  HitsOut ho{f.data(), f.data() + ns, f.data() + 2 * ns, f.data() + 3 * ns};
+  int rc = map_view(m, &v, STORM_MODE_PLAIN, slots, &ms, &ho);
   if (rc) return rc;
-  r.hits.resize((size_t)r.n * 16);
-  for (size_t k = 0; k < slots.size(); ++k) {
-    r.hits[4 * k + 0] = slots[k].score;
-    r.hits[4 * k + 1] = slots[k].ref_id;
-    r.hits[4 * k + 2] = slots[k].ref_start;
-    r.hits[4 * k + 3] = slots[k].sense;
-  }
+  r.hits.resize(4 * ns);
+  for (size_t k = 0; k < ns; ++k)
+    for (size_t q = 0; q < 4; ++q) r.hits[4 * k + q] = f[q * ns + k];
   std::lock_guard<std::mutex> lk(s->mu);
   s->st.candidates += ms.n_candidates;
   return STORM_OK;

@@ -192,6 +192,11 @@ hipError_t launch_subset_gather(const uint32_t* planes, const uint32_t* qplanes,
                                 const int32_t* ids, int64_t k, uint32_t* sp, uint32_t* sq, int32_t* sl, Slot* ss,
                                 hipStream_t st);
 hipError_t launch_subset_scatter(const Slot* ss, const int32_t* ids, int64_t k, Slot* slots, hipStream_t st);
+hipError_t launch_permute_bytes(const uint8_t* src, int64_t stride, const int32_t* order, int64_t n, uint8_t* dst,
+                                hipStream_t st);
+hipError_t launch_permute_i32(const int32_t* src, const int32_t* order, int64_t n, int32_t* dst, hipStream_t st);
+hipError_t launch_hits_out(const Slot* slots, const int32_t* order, int64_t n, int32_t* score, int32_t* ref_id,
+                           int32_t* ref_start, int32_t* sense, hipStream_t st);
 hipError_t launch_overflow_ranges(const int64_t* seg_off, const uint8_t* overflow, int64_t n_rs, int64_t* begin,
                                   int64_t* end, hipStream_t st);
 // device-wide helpers (hipcub)
