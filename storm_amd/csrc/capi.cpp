@@ -10,6 +10,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <unordered_map>
@@ -693,6 +694,15 @@ static int tune_fused(storm_mapper* m, int64_t n, int maxL, int max_grid, FusedT
   const int64_t key = ((int64_t)sc.W << 40) | ((int64_t)sc.I << 32) | ((int64_t)m->p.color << 24) | (int64_t)((maxL + 31) / 32);
   auto it = F.tune.find(key);
   if (it != F.tune.end()) { out = it->second; return STORM_OK; }
+  if (m->shared_tune) {   // another context on the same device tuned this class already
+    std::lock_guard<std::mutex> lk(m->shared_tune->mu);
+    auto s = m->shared_tune->best.find(key);
+    if (s != m->shared_tune->best.end()) {
+      F.tune[key] = s->second;
+      out = s->second;
+      return STORM_OK;
+    }
+  }
   const int64_t sample = std::min<int64_t>(n, 32768);
   const int32_t ns = (int32_t)sample;
   HIPOK(hipMemcpyAsync(F.n_dev.p, &ns, 4, hipMemcpyHostToDevice, m->st));
@@ -728,6 +738,10 @@ static int tune_fused(storm_mapper* m, int64_t n, int maxL, int max_grid, FusedT
   (void)hipEventDestroy(e1);
   HIPOK(launch_slots_init(F.slots.as<Slot>(), 4 * sample, m->st));
   F.tune[key] = best;
+  if (m->shared_tune) {
+    std::lock_guard<std::mutex> lk(m->shared_tune->mu);
+    m->shared_tune->best.emplace(key, best);
+  }
   out = best;
   return STORM_OK;
 }

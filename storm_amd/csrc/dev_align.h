@@ -1,4 +1,4 @@
-// dev_align.h -- one banded DP + traceback (used by k_align and k_fused); semantics in k_align.hip.
+// dev_align.h -- banded glocal DP + traceback (used by k_align and k_fused); semantics in k_align.hip.
 #pragma once
 #include "storm_kernels.h"
 
@@ -18,52 +18,58 @@ __device__ __forceinline__ int m_make(int from, int ind, int chk, int autoc, int
 // divergent branch around the computation of the unused operand
 __device__ __forceinline__ int msel(bool a, int x, int y) { return y ^ ((y ^ x) & -(int)a); }
 
-// One candidate: read strand rs (2*read + strand), diagonal h.
-// TB = true: tbm is this thread's scratch of L*(2I+1) direction bytes and, when the record can be
-// accepted, the traceback (alignment order, 2 codes per byte) goes to tb.
-// TB = false: the score pass alone, nothing is stored.  A record can only ever be accepted when
-// score > T and S >= T with T >= t (T starts at -t and only rises, run.c:568,683-685), so this
-// pass stops at the first row where the bound drops below t (S < t then holds) and returns
-// score = MIN_SCORE; otherwise it returns the same score, S and best_j as the TB pass.
-template <int IM, bool COLOR, bool TB = true>
-__device__ AlignRec dp_one(const GenomeDev& g, const ReadsDev& r, const ScoreDev& scd, int rs, int h, uint8_t* tbm,
-                           uint8_t* tb) {
-  constexpr int NCM = 2 * IM + 1;
-  const int L = r.len[rs >> 1];
-  const int I = scd.I, d = scd.d;
-  const int NC = 2 * I + 1;
-  const int win = (h - I > -d) ? h - I : -d;
-  const uint32_t* pl = read_planes(r, rs);
-  const int top_match = scd.ps[0][3];
-  const int gap_open = scd.gap_open, gap_ext = scd.gap_ext;
-  const int I4 = 4 * scd.I;   // a meta low byte below 4I has fewer than I indels
-  // pair scores by quality level, in registers: a lane-indexed read of the argument block
-  // would be a memory round trip per cell
-  const int pm0 = scd.ps[0][0], pm1 = scd.ps[0][1], pm2 = scd.ps[0][2], pm3 = scd.ps[0][3];
-  const int px0 = scd.ps[1][0], px1 = scd.ps[1][1], px2 = scd.ps[1][2], px3 = scd.ps[1][3];
-  auto level_score = [](int q, int s0, int s1, int s2, int s3) {
-    return msel(q & 2, msel(q & 1, s3, s2), msel(q & 1, s1, s0));
-  };
+// Scoring constants of the DP in registers (a lane-indexed read of the argument block would be a
+// memory round trip per cell).
+struct DPConst {
+  int I, d, NC, I4, t, top_match, gap_open, gap_ext;
+  int pm0, pm1, pm2, pm3, px0, px1, px2, px3;   // pair_scores[match / mismatch][level]
+  __device__ explicit DPConst(const ScoreDev& s)
+      : I(s.I), d(s.d), NC(2 * s.I + 1), I4(4 * s.I), t(s.t), top_match(s.ps[0][3]), gap_open(s.gap_open),
+        gap_ext(s.gap_ext), pm0(s.ps[0][0]), pm1(s.ps[0][1]), pm2(s.ps[0][2]), pm3(s.ps[0][3]), px0(s.ps[1][0]),
+        px1(s.ps[1][1]), px2(s.ps[1][2]), px3(s.ps[1][3]) {}
+  __device__ int match(int q) const { return msel(q & 2, msel(q & 1, pm3, pm2), msel(q & 1, pm1, pm0)); }
+  __device__ int mismatch(int q) const { return msel(q & 2, msel(q & 1, px3, px2), msel(q & 1, px1, px0)); }
+};
 
+// The DP of one candidate (read strand rs = 2*read + strand, diagonal h) as a state machine over
+// rows.  The 2I+1 band columns of the current row, the genome planes and the read's codes and
+// quality levels are held in registers: bit c of the row-i genome words is code win + i + c; per
+// plane NW words from word index wi, a new word fetched every 32 rows, one word ahead of its use,
+// so a row issues no dependent loads.
+template <int IM, bool COLOR>
+struct DPLane {
+  static constexpr int NCM = 2 * IM + 1;
+  static constexpr int NW = NCM > 32 ? 4 : 3;
   int sc[NCM];
   int meta[NCM];
-  // Sliding bit-plane windows of the genome: bit c of row i's word is code win + i + c.  Per plane
-  // NW words from word index wi are held in registers; a new word is fetched every 32 rows, one
-  // word ahead of its use, so the row loop issues no dependent loads.
-  constexpr int NW = NCM > 32 ? 4 : 3;
-  const int64_t wbase = (int64_t)win + GPAD;
-  int64_t wi = wbase >> 5;
-  int sh = (int)(wbase & 31);
   uint32_t gl[NW], gh[NW], gm[NW];
+  int64_t wi;
+  int sh;
+  const uint32_t* pl;
+  const uint32_t* qp;
+  uint32_t rlo, rhi, rq;
+  int has_qual;
+  int L, i, win, S, bp;
+
+  // row 0 (alignment.c:326-360); tbm (TB) receives the row's direction bytes
+  template <bool TB>
+  __device__ void begin(const GenomeDev& g, const ReadsDev& r, const DPConst& k, int rs, int h, uint8_t* tbm) {
+    L = r.len[rs >> 1];
+    win = (h - k.I > -k.d) ? h - k.I : -k.d;
+    pl = read_planes(r, rs);
+    qp = r.qual + (int64_t)rs * QW;
+    has_qual = r.has_qual;
+    const int64_t wbase = (int64_t)win + GPAD;
+    wi = wbase >> 5;
+    sh = (int)(wbase & 31);
 #pragma unroll
-  for (int k = 0; k < NW; ++k) { gl[k] = g.lo[wi + k]; gh[k] = g.hi[wi + k]; gm[k] = g.mask[wi + k]; }
-  // read codes and quality levels, one word per 32 (codes) or 16 (levels) rows
-  const uint32_t* qp = r.qual + (int64_t)rs * QW;
-  uint32_t rlo = pl[0], rhi = pl[RW], rq = r.has_qual ? qp[0] : 0xffffffffu;
-  // row 0 (alignment.c:326-360)
-  {
+    for (int q = 0; q < NW; ++q) { gl[q] = g.lo[wi + q]; gh[q] = g.hi[wi + q]; gm[q] = g.mask[wi + q]; }
+    rlo = pl[0];
+    rhi = pl[RW];
+    rq = has_qual ? qp[0] : 0xffffffffu;
+    const int NC = k.NC, I = k.I;
     const int r0 = (int)((rlo & 1u) | ((rhi & 1u) << 1)), q0 = (int)(rq & 3u);
-    const int s_m = level_score(q0, pm0, pm1, pm2, pm3), s_x = level_score(q0, px0, px1, px2, px3);
+    const int s_m = k.match(q0), s_x = k.mismatch(q0);
     const uint32_t wl0 = __builtin_amdgcn_alignbit(gl[1], gl[0], sh), wh0 = __builtin_amdgcn_alignbit(gh[1], gh[0], sh);
     const uint32_t wm0 = __builtin_amdgcn_alignbit(gm[1], gm[0], sh);
     const uint32_t wl1 = NCM > 32 ? __builtin_amdgcn_alignbit(gl[2], gl[1], sh) : 0u;
@@ -93,25 +99,34 @@ __device__ AlignRec dp_one(const GenomeDev& g, const ReadsDev& r, const ScoreDev
         meta[c] = m_make(3, I, 0, 0, 0);
       }
     }
+    bp = (COLOR ? (L - 2) : L) * k.top_match;
+    S = 0x7fffffff;
+    i = 0;
   }
-  int bp = (COLOR ? (L - 2) : L) * top_match;
-  int S = 0x7fffffff;
-  for (int i = 1; i < L; ++i) {
-    // next row: advance the genome windows by one code
+
+  // true while rows remain (row i + 1 < L)
+  __device__ bool more() const { return i + 1 < L; }
+
+  // the next row (i + 1) of the recurrence, and the survival bound after it
+  template <bool TB>
+  __device__ void row(const GenomeDev& g, const DPConst& k, uint8_t* tbm) {
+    ++i;
+    // advance the genome windows by one code
     if (++sh == 32) {
       sh = 0;
       ++wi;
 #pragma unroll
-      for (int k = 0; k + 1 < NW; ++k) { gl[k] = gl[k + 1]; gh[k] = gh[k + 1]; gm[k] = gm[k + 1]; }
+      for (int q = 0; q + 1 < NW; ++q) { gl[q] = gl[q + 1]; gh[q] = gh[q + 1]; gm[q] = gm[q + 1]; }
       gl[NW - 1] = g.lo[wi + NW - 1];
       gh[NW - 1] = g.hi[wi + NW - 1];
       gm[NW - 1] = g.mask[wi + NW - 1];
     }
     if ((i & 31) == 0) { rlo = pl[i >> 5]; rhi = pl[RW + (i >> 5)]; }
-    if ((i & 15) == 0 && r.has_qual) rq = qp[i >> 4];
+    if ((i & 15) == 0 && has_qual) rq = qp[i >> 4];
+    const int NC = k.NC, I = k.I, I4 = k.I4, gap_open = k.gap_open, gap_ext = k.gap_ext;
     const int ri = (int)(((rlo >> (i & 31)) & 1u) | (((rhi >> (i & 31)) & 1u) << 1));
     const int qi = (int)((rq >> ((i & 15) << 1)) & 3u);
-    const int s_m = level_score(qi, pm0, pm1, pm2, pm3), s_x = level_score(qi, px0, px1, px2, px3);
+    const int s_m = k.match(qi), s_x = k.mismatch(qi);
     const uint32_t wl0 = __builtin_amdgcn_alignbit(gl[1], gl[0], sh), wh0 = __builtin_amdgcn_alignbit(gh[1], gh[0], sh);
     const uint32_t wm0 = __builtin_amdgcn_alignbit(gm[1], gm[0], sh);
     const uint32_t wl1 = NCM > 32 ? __builtin_amdgcn_alignbit(gl[2], gl[1], sh) : 0u;
@@ -140,10 +155,14 @@ __device__ AlignRec dp_one(const GenomeDev& g, const ReadsDev& r, const ScoreDev
         // The recurrence in select form (alignment.c:465-604): every lane runs the same
         // instructions whatever its record, with no divergent branches per cell.
         int match_score = cs;
-        if (COLOR) {   // alignment.c:465-468
-          const int prev = m_chk(cmeta), ref = m_auto(cmeta);
-          // non-short-circuit & and |: short-circuit operators become divergent branches
-          const bool refund = (prev != 0) & (((prev ^ cchk) == 0) | ((prev ^ cchk) == ref)) & (cs != MIN_SCORE) & (cs < 0);
+        // color state of the corner, kept in place in its meta word: chk bits 8-9, auto 10-11,
+        // fresh 12-14; xw = the checksum this cell would carry on a diagonal move, at bits 8-9
+        const int xw = COLOR ? ((cmeta ^ (cchk << 8)) & 0x300) : 0;
+        const int iauto_w = COLOR ? (cmeta & 0xC00) : 0;
+        const bool ichk0 = COLOR ? ((cmeta & 0x300) == 0) : true;     // no pending checksum
+        const bool x_is_auto = (xw << 2) == iauto_w;                  // corrected by the pending auto
+        if (COLOR) {   // alignment.c:465-468; & and | (not && ||): no divergent branches
+          const bool refund = (!ichk0) & ((xw == 0) | x_is_auto) & (cs != MIN_SCORE) & (cs < 0);
           match_score = msel(refund, -2 * cs, cs);
         }
         const int diag = csc + match_score;
@@ -162,73 +181,96 @@ __device__ AlignRec dp_one(const GenomeDev& g, const ReadsDev& r, const ScoreDev
         const int l_lo = (l_lo8 & 0xfc) + 5;
         const int n_lo = msel(take_l, l_lo, msel(take_t, t_lo, d_lo));
         const int nfrom = n_lo & 3;
-        int nchk = 0, nauto = 0, nfresh = 0;
+        int nhi = 0;   // color state bits 8-14 of the new meta word
         if (COLOR) {
-          // diagonal move (alignment.c:533-578)
-          const int ichk = m_chk(cmeta), iauto = m_auto(cmeta), ifresh = m_fresh(cmeta);
-          const int x = cchk ^ ichk;
-          const bool f1 = ifresh > 1;
-          // a colour mismatch here (cchk != 0): the four cases of the reference, in order
-          const bool a1 = ichk == 0, a2 = (!a1) & (x == iauto), a3 = (!a1) & (!a2) & (x == 0);
-          const int a_chk = msel(a2, 0, x);
-          const int a_auto = msel(a1, x, msel(a2, 0, msel(f1, iauto, msel(a3, 0, x))));
-          const int a_fresh = msel(a1, qi, msel(a2, 0, msel(f1, ifresh - 1, msel(a3, 0, qi))));
-          // a colour match (cchk == 0)
-          const bool b1 = iauto != 0, b_keep = b1 & f1 & (x != 0);
-          const int b_chk = msel(b1, x, 0);
-          const int b_auto = msel(b_keep, iauto, 0);
-          const int b_fresh = msel(b_keep, ifresh - 1, 0);
-          const bool cmis = cchk != 0;
-          const int d_chk = msel(cmis, a_chk, b_chk), d_auto = msel(cmis, a_auto, b_auto);
-          const int d_fresh = msel(cmis, a_fresh, b_fresh);
-          // gap from the top (alignment.c:580-589) or from the left (alignment.c:590-599)
-          const int tf = m_fresh(tmeta), lf = m_fresh(left_meta);
-          const int t_chk = m_chk(tmeta) ^ ri, l_chk = m_chk(left_meta) ^ gj;
-          const int t_auto = msel(tf > 1, m_auto(cmeta), cauto), l_auto = msel(lf > 1, m_auto(cmeta), cauto);
-          const int t_fresh = msel(tf > 1, tf - 1, cfresh), l_fresh = msel(lf > 1, lf - 1, cfresh);
-          nchk = msel(take_l, l_chk, msel(take_t, t_chk, d_chk));
-          nauto = msel(take_l, l_auto, msel(take_t, t_auto, d_auto));
-          nfresh = msel(take_l, l_fresh, msel(take_t, t_fresh, d_fresh));
+          // diagonal move (alignment.c:533-578), the reference's cases in order, each as a whole
+          // state word; K keeps the corner's auto with its freshness decremented
+          const int fw = cmeta & 0x7000;
+          const bool f1 = fw > 0x1000;                                  // ifresh > 1
+          const int K = iauto_w | (fw - 0x1000);
+          const int A1w = xw | (xw << 2) | (qi << 12);                  // (x, x, qi)
+          const bool a2 = (!ichk0) & x_is_auto, a3 = (!ichk0) & (!a2) & (xw == 0);
+          const int m_a = msel(ichk0, A1w, msel(a2, 0, msel(f1, xw | K, msel(a3, 0, A1w))));
+          const bool b1 = iauto_w != 0, b_keep = b1 & f1 & (xw != 0);
+          const int m_b = msel(b1, msel(b_keep, xw | K, xw), 0);
+          const int d_hi = msel(cchk != 0, m_a, m_b);
+          // gap from the top (alignment.c:580-589) or from the left (alignment.c:590-599): chk of
+          // the neighbour xor the read / genome code; auto of the corner and the neighbour's
+          // freshness - 1 while it is > 1, else the cell's own initial state
+          const int c_af = (cauto << 10) | (cfresh << 12);
+          const int tfw = tmeta & 0x7000, lfw = left_meta & 0x7000;
+          const int t_hi = ((tmeta ^ (ri << 8)) & 0x300) | msel(tfw > 0x1000, iauto_w | (tfw - 0x1000), c_af);
+          const int l_hi = ((left_meta ^ (gj << 8)) & 0x300) | msel(lfw > 0x1000, iauto_w | (lfw - 0x1000), c_af);
+          nhi = msel(take_l, l_hi, msel(take_t, t_hi, d_hi));
         }
         if (TB) tbm[i * NC + c] = (uint8_t)(nfrom | ((nsc > csc) ? 4 : 0));
         sc[c] = nsc;
-        meta[c] = COLOR ? (n_lo | (nchk << 8) | (nauto << 10) | (nfresh << 12)) : n_lo;
+        meta[c] = COLOR ? (n_lo | nhi) : n_lo;
         left_sc = nsc;
         left_meta = meta[c];
         if (max_on_line < nsc) max_on_line = nsc;
       }
     }
     if (max_on_line + bp < S) S = max_on_line + bp;
-    bp -= top_match;
-    if (!TB && S < scd.t) {   // can never be accepted: the rest of the rows cannot raise S
-      AlignRec o;
-      o.score = MIN_SCORE;
-      o.S = S;
-      o.win = win;
-      o.best_j = -1;
-      o.best_i0 = -1;
-      o.best_j0 = -1;
-      o.tb_len = 0;
-      return o;
-    }
+    bp -= k.top_match;
   }
-  // leftmost maximum of the last row (alignment.c:621-630)
-  int best = MIN_SCORE, bc = -1;
-#pragma unroll
-  for (int c = 0; c < NCM; ++c)
-    if (c < NC && best < sc[c]) { best = sc[c]; bc = c; }
 
-  AlignRec o;
-  o.score = best;
-  o.S = S;
-  o.win = win;
-  o.best_j = (int16_t)(bc >= 0 ? L - 1 + bc : -1);
-  o.best_i0 = -1;
-  o.best_j0 = -1;
-  o.tb_len = 0;
+  // score pass result after an early stop (S < t): never accepted
+  __device__ AlignRec rejected() const {
+    AlignRec o;
+    o.score = MIN_SCORE;
+    o.S = S;
+    o.win = win;
+    o.best_j = -1;
+    o.best_i0 = -1;
+    o.best_j0 = -1;
+    o.tb_len = 0;
+    return o;
+  }
+
+  // after the last row: the leftmost maximum of the last row (alignment.c:621-630)
+  __device__ AlignRec result(const DPConst& k, int& bc) const {
+    int best = MIN_SCORE;
+    bc = -1;
+#pragma unroll
+    for (int c = 0; c < NCM; ++c)
+      if (c < k.NC && best < sc[c]) { best = sc[c]; bc = c; }
+    AlignRec o;
+    o.score = best;
+    o.S = S;
+    o.win = win;
+    o.best_j = (int16_t)(bc >= 0 ? L - 1 + bc : -1);
+    o.best_i0 = -1;
+    o.best_j0 = -1;
+    o.tb_len = 0;
+    return o;
+  }
+};
+
+// One candidate: read strand rs (2*read + strand), diagonal h.
+// TB = true: tbm is this thread's scratch of L*(2I+1) direction bytes and, when the record can be
+// accepted, the traceback (alignment order, 2 codes per byte) goes to tb.
+// TB = false: the score pass alone, nothing is stored.  A record can only ever be accepted when
+// score > T and S >= T with T >= t (T starts at -t and only rises, run.c:568,683-685), so this
+// pass stops at the first row where the bound drops below t (S < t then holds) and returns
+// score = MIN_SCORE; otherwise it returns the same score, S and best_j as the TB pass.
+template <int IM, bool COLOR, bool TB = true>
+__device__ AlignRec dp_one(const GenomeDev& g, const ReadsDev& r, const ScoreDev& scd, int rs, int h, uint8_t* tbm,
+                           uint8_t* tb) {
+  const DPConst k(scd);
+  DPLane<IM, COLOR> st;
+  st.template begin<TB>(g, r, k, rs, h, tbm);
+  while (st.more()) {
+    st.template row<TB>(g, k, tbm);
+    if (!TB && st.S < k.t) return st.rejected();   // the rest of the rows cannot raise S
+  }
+  int bc;
+  AlignRec o = st.result(k, bc);
   // A record can be accepted only if score > T and S >= T with T >= t (run.c:568,675-685).
-  if (TB && bc >= 0 && best > scd.t && S >= scd.t) {
-    int i = L - 1, c = bc, n = 0, color = 0;
+  if (TB && bc >= 0 && o.score > k.t && st.S >= k.t) {
+    const int NC = k.NC, win = st.win;
+    const uint32_t* pl = st.pl;
+    int i = st.L - 1, c = bc, n = 0, color = 0;
     while (i >= 0 && c >= 0 && c < NC) {
       const int j = i + c;
       o.best_i0 = (int16_t)i;

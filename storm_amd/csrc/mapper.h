@@ -7,6 +7,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -149,6 +150,12 @@ struct FusedState {
   }
 };
 
+// autotuner results shared by the contexts of one device (the serving loop's lanes)
+struct SharedTune {
+  std::mutex mu;
+  std::map<int64_t, FusedTune> best;
+};
+
 struct PassStats {
   int64_t cand = 0, rec = 0, hits = 0;
   float ms_seed = 0, ms_filter = 0, ms_align = 0, ms_replay = 0, ms_lookup = 0;
@@ -172,6 +179,7 @@ struct storm_mapper {
   storm::DevReads stage;       // the caller's reads as uploaded, before the device reordering
   storm::DevMem order_dev;     // device row -> caller's read index
   storm::DevMem hits_dev;      // per-slot results on their way to the caller
+  storm::SharedTune* shared_tune = nullptr;   // tuner results of sibling contexts, or null
   bool want_tb = true;   // false while a call returns hits only: tracebacks stay on the device
 };
 

@@ -166,6 +166,7 @@ struct storm_server {
     std::vector<ServeRegion> h_regions;
   };
   std::vector<std::unique_ptr<Lane>> lanes;
+  SharedTune tune;   // the lanes share autotuner results
 };
 
 namespace storm {
@@ -493,6 +494,7 @@ int storm_server_create(int device, const char* genome_path, const storm_server_
     ln->m.reset(new storm_mapper());
     ln->m->device = device;
     ln->m->want_tb = false;
+    ln->m->shared_tune = &s->tune;
     HIPOK(hipStreamCreateWithFlags(&ln->m->st, hipStreamNonBlocking));
     s->lanes.push_back(std::move(ln));
   }
