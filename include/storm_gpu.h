@@ -104,6 +104,28 @@ int storm_filter_fetch(storm_filter_plan *plan, uint8_t *scores, uint8_t *pass);
 void storm_filter_plan_destroy(storm_filter_plan *plan);
 
 /* ------------------------------------------------------------------------------------------
+ * Seed index of one reference sequence, built on the device: index__build_reference
+ * (index.c:19-195, declared index.h:49-62), with the contiguous layout of index.c:157-188.
+ * packed:   the sequence as ReferenceDBType.sequence holds it (reference_data.h:25-53): 2-bit
+ *           codes, code n in bits 2*(n&3) of byte n>>2 (codes.h:125-130), size codes.
+ * region_start/region_end: the masked regions [start, end) (ReferenceDBType.masked_regions),
+ *           ascending and disjoint; windows start-span .. end-2 are not indexed (index.c:66-72).
+ * seed:     one seed pattern ('#' care, '-' don't care), as given to -s.
+ * stdev_distance: -l; keys occurring more than avg + l*sd times are erased (index.c:117-156);
+ *           <= 0 disables the filter.
+ * storm_index_fetch copies first[4^w + 1] and map[n_map] (n_map = first[4^w]) to host buffers.
+ * The mapper and the server build their indexes with the same kernels.
+ */
+typedef struct storm_index storm_index;
+int storm_index_build(int device, const uint8_t *packed, int64_t size, const int32_t *region_start,
+                      const int32_t *region_end, int n_regions, const char *seed, int stdev_distance,
+                      storm_index **out);
+int storm_index_sizes(const storm_index *ix, int64_t *n_keys, int64_t *n_map, int64_t *n_erased,
+                      double *ms_device);
+int storm_index_fetch(const storm_index *ix, int32_t *first, int32_t *map);
+void storm_index_destroy(storm_index *ix);
+
+/* ------------------------------------------------------------------------------------------
  * Group 2: whole mapping runs (reads_against_references, run.c:764-1109, with -M).
  * storm_map_files reads the same files as the reference (FASTA genome; FASTQ, FASTA or
  * csfasta(+.qual) reads), maps on the GPU and writes the reference's -M SAM.

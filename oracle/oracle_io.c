@@ -686,3 +686,63 @@ void orc_write_sam(FILE *out, const orc_result_t *res, const orc_reads_t *db, co
   free(cigar); free(seq);
   if (sym) { for (int r = 0; r < n_refs; ++r) free(sym[r]); free(sym); }
 }
+
+/* ----------------------------------------------------------------- test helpers */
+
+/* Sequence ref_no of a FASTA file as load_reference holds it: a malloc'd copy of the packed
+ * codes (size codes, 4 per byte, no padding) and of the masked regions [start, end).  Both
+ * helpers load the file from rand()'s initial state (srand(1)), as a fresh process does, so the
+ * codes drawn for N are the same in both. */
+int orc_fasta_sequence(const char *path, int color, int ignore_lowercase, int ref_no, uint8_t **packed, int *size,
+                       int **starts, int **ends, int *n_regions) {
+  orc_params_t p;
+  orc_params_default(&p, color);
+  p.ignore_lowercase = ignore_lowercase;
+  orc_ref_t *refs = NULL;
+  srand(1);
+  int n = orc_load_reference(path, 100, &p, &refs);
+  if (n <= ref_no || ref_no < 0) { orc_refs_free(refs, n > 0 ? n : 0); return -1; }
+  const orc_ref_t *r = &refs[ref_no];
+  const size_t bytes = ((size_t)r->size + 3) / 4;
+  *packed = xmalloc(bytes + 1);
+  memcpy(*packed, r->seq, bytes);
+  *size = r->size;
+  *n_regions = r->n_regions;
+  *starts = xmalloc(sizeof(int) * (size_t)(r->n_regions + 1));
+  *ends = xmalloc(sizeof(int) * (size_t)(r->n_regions + 1));
+  for (int k = 0; k < r->n_regions; ++k) { (*starts)[k] = r->regions[k].start; (*ends)[k] = r->regions[k].end; }
+  orc_refs_free(refs, n);
+  return 0;
+}
+
+/* The index of one seed pattern over sequence ref_no of a FASTA file (index.c:19-195):
+ * malloc'd first[4^w + 1] and map[first[4^w]]. */
+int orc_fasta_index(const char *path, int color, int ignore_lowercase, int ref_no, const char *seed,
+                    int stdev_distance, int **first, int *n_first, int **map, int *n_map) {
+  orc_params_t p;
+  orc_params_default(&p, color);
+  p.ignore_lowercase = ignore_lowercase;
+  p.filter_repeats = stdev_distance > 0;
+  p.stdev_distance = stdev_distance > 0 ? stdev_distance : 0;
+  orc_seed_t *seeds = NULL;
+  int ns = orc_seed_parse_list(seed, &seeds);
+  if (ns != 1) { if (ns > 0) orc_seeds_free(seeds, ns); return -1; }
+  orc_ref_t *refs = NULL;
+  srand(1);
+  int n = orc_load_reference(path, 100, &p, &refs);
+  if (n <= ref_no || ref_no < 0) { orc_refs_free(refs, n > 0 ? n : 0); orc_seeds_free(seeds, ns); return -1; }
+  orc_index_t idx;
+  orc_index_build(&refs[ref_no], &seeds[0], &p, &idx);
+  *n_first = idx.index_size + 1;
+  *first = xmalloc(sizeof(int) * (size_t)*n_first);
+  memcpy(*first, idx.first, sizeof(int) * (size_t)*n_first);
+  *n_map = idx.first[idx.index_size];
+  *map = xmalloc(sizeof(int) * (size_t)(*n_map + 1));
+  memcpy(*map, idx.map, sizeof(int) * (size_t)*n_map);
+  orc_index_free(&idx);
+  orc_refs_free(refs, n);
+  orc_seeds_free(seeds, ns);
+  return 0;
+}
+
+void orc_free(void *p) { free(p); }

@@ -144,6 +144,13 @@ int orc_index_build(const orc_ref_t *ref, const orc_seed_t *seed, const orc_para
 void orc_index_free(orc_index_t *idx);
 int orc_index_next_hit(const orc_index_t *idx, int key, int *hit_pointer);
 
+/* --- test helpers (oracle_io.c): one FASTA sequence and one seed's index, malloc'd copies --- */
+int orc_fasta_sequence(const char *path, int color, int ignore_lowercase, int ref_no, uint8_t **packed, int *size,
+                       int **starts, int **ends, int *n_regions);
+int orc_fasta_index(const char *path, int color, int ignore_lowercase, int ref_no, const char *seed,
+                    int stdev_distance, int **first, int *n_first, int **map, int *n_map);
+void orc_free(void *p);
+
 /* --- candidates (run.c:555-747): ascending diagonals, returns count --- */
 int orc_candidates(const orc_index_t *idx, const orc_seed_t *seeds, int n_seeds,
                    const uint8_t *read, int read_len, int allowed_indels,

@@ -37,28 +37,6 @@ double now_s() {
 }
 
 // ------------------------------------------------------------------------------ genome planes
-static int64_t plane_words(int64_t size) { return (GPAD + size + GPAD) / 32 + 4; }
-
-static int upload_genome(const uint8_t* code, const uint8_t* mask, int64_t size, DevGenome& g, hipStream_t st) {
-  const int64_t nw = plane_words(size);
-  std::vector<uint32_t> lo((size_t)nw, 0), hi((size_t)nw, 0), mk((size_t)nw, 0xffffffffu);
-  for (int64_t i = 0; i < size; ++i) {
-    const int64_t b = i + GPAD;
-    const uint32_t bit = 1u << (b & 31);
-    if (code[i] & 1) lo[(size_t)(b >> 5)] |= bit;
-    if (code[i] & 2) hi[(size_t)(b >> 5)] |= bit;
-    if (!(mask && mask[i])) mk[(size_t)(b >> 5)] &= ~bit;
-  }
-  HIPOK(g.lo.ensure(nw * 4));
-  HIPOK(g.hi.ensure(nw * 4));
-  HIPOK(g.mask.ensure(nw * 4));
-  HIPOK(hipMemcpyAsync(g.lo.p, lo.data(), nw * 4, hipMemcpyHostToDevice, st));
-  HIPOK(hipMemcpyAsync(g.hi.p, hi.data(), nw * 4, hipMemcpyHostToDevice, st));
-  HIPOK(hipMemcpyAsync(g.mask.p, mk.data(), nw * 4, hipMemcpyHostToDevice, st));
-  HIPOK(hipStreamSynchronize(st));
-  g.size = size;
-  return STORM_OK;
-}
 
 int filter_lanes(int d) { return d <= 1 ? 2 : d <= 3 ? 4 : d <= 7 ? 8 : 16; }   // run.c:109-127
 
@@ -89,24 +67,12 @@ int ensure_ref_resident(storm_mapper* m, size_t r) {
   RefState& rs = *m->res->rstate[r];
   if (rs.resident) return STORM_OK;
   const HostRef& ref = m->res->refs[r];
-  int rc = upload_genome(ref.code.data(), ref.mask.data(), ref.size, rs.g, m->st);
-  if (rc) return rc;
   const double t0 = now_s();
-  rs.idx.clear();
-  rs.idx.resize(m->res->seeds.size());
-  std::vector<HostIndex> hidx(m->res->seeds.size());
-  std::vector<std::thread> th;
-  for (size_t s = 0; s < m->res->seeds.size(); ++s)
-    th.emplace_back([&, s]() { build_index(ref, m->res->seeds[s], m->p, hidx[s]); });
-  for (auto& t : th) t.join();
-  for (size_t s = 0; s < m->res->seeds.size(); ++s) {
-    rs.idx[s].spec = m->res->seeds[s];
-    HIPOK(rs.idx[s].first.ensure(hidx[s].first.size() * 4));
-    HIPOK(rs.idx[s].map.ensure(hidx[s].map.size() * 4 + 16));
-    HIPOK(hipMemcpyAsync(rs.idx[s].first.p, hidx[s].first.data(), hidx[s].first.size() * 4, hipMemcpyHostToDevice, m->st));
-    HIPOK(hipMemcpyAsync(rs.idx[s].map.p, hidx[s].map.data(), hidx[s].map.size() * 4, hipMemcpyHostToDevice, m->st));
-  }
-  HIPOK(hipStreamSynchronize(m->st));
+  DevMem dcode;
+  int rc = upload_genome(ref.code.data(), ref.mask.empty() ? nullptr : ref.mask.data(), ref.size, rs.g, m->st, &dcode);
+  if (rc) return rc;
+  rc = build_indexes(ref, dcode, m->res->seeds, m->p.stdev_distance, rs.idx, m->st);
+  if (rc) return rc;
   m->res->index_seconds += now_s() - t0;
   rs.resident = true;
   return STORM_OK;

@@ -422,6 +422,24 @@ void sort_reads_by_tag(HostReads& r) {
 }
 
 // ------------------------------------------------------------------------------------ index
+// index.c:117-156: average and standard deviation of the key counts in double, summed in key
+// order with the reference's expression order, then every key whose count exceeds
+// avg + l * sd is marked in erased[].  `count` is the number of indexed windows.
+int64_t repeat_filter(const int32_t* counts, int64_t ksize, int64_t count, int stdev_distance, uint8_t* erased) {
+  int64_t n = 0;
+  double avg = 0;
+  for (int64_t k = 0; k < ksize; ++k)
+    if (counts[k] > 0) avg += 1.0 / (int)count * counts[k] * counts[k];
+  avg /= (int)count;
+  double sd = 0;
+  for (int64_t k = 0; k < ksize; ++k)
+    if (counts[k] > 0) sd += (counts[k] - avg) * (counts[k] - avg) / (int)count;
+  sd = sqrt(sd);
+  for (int64_t k = 0; k < ksize; ++k)
+    if (counts[k] && (counts[k] - avg) > stdev_distance * sd) { erased[k] = 1; n++; }
+  return n;
+}
+
 // index.c:19-195 with the contiguous layout of index.c:157-188: positions q in [0, size-span]
 // in ascending order per key, masked windows skipped (index.c:66-72), keys whose count exceeds
 // avg + l*sd erased (index.c:117-156, same double expression order).
@@ -458,18 +476,7 @@ void build_index(const HostRef& ref, const SeedSpec& seed, const storm_params& p
   }
   const int64_t count = (int64_t)keys.size();
   std::vector<uint8_t> erased((size_t)ksize, 0);
-  if (p.stdev_distance > 0) {
-    double avg = 0;
-    for (int64_t k = 0; k < ksize; ++k)
-      if (counts[k] > 0) avg += 1.0 / count * counts[k] * counts[k];
-    avg /= count;
-    double sd = 0;
-    for (int64_t k = 0; k < ksize; ++k)
-      if (counts[k] > 0) sd += (counts[k] - avg) * (counts[k] - avg) / count;
-    sd = sqrt(sd);
-    for (int64_t k = 0; k < ksize; ++k)
-      if (counts[k] && (counts[k] - avg) > p.stdev_distance * sd) { erased[k] = 1; out.erased++; }
-  }
+  if (p.stdev_distance > 0) out.erased = repeat_filter(counts.data(), ksize, count, p.stdev_distance, erased.data());
   for (int64_t k = 0; k < ksize; ++k) out.first[k + 1] = out.first[k] + (erased[k] ? 0 : counts[k]);
   out.map.resize((size_t)out.first[ksize] + 1);
   std::vector<int32_t> fill(out.first.begin(), out.first.end() - 1);

@@ -47,6 +47,7 @@ int load_reference(const char* path, const storm_params& p, std::vector<HostRef>
 int load_reads(const char* path, const char* qual_path, const storm_params& p, HostReads& out, std::string& err);
 void sort_reads_by_tag(HostReads& r);
 void build_index(const HostRef& ref, const SeedSpec& seed, const storm_params& p, HostIndex& out);
+int64_t repeat_filter(const int32_t* counts, int64_t ksize, int64_t count, int stdev_distance, uint8_t* erased);
 
 struct HostSlot {
   int32_t score, ref_id, ref_start, ref_end, read_start, read_end, sense;

@@ -68,6 +68,19 @@ struct DevGenome {
 struct DevIndex {
   DevMem first, map;
   SeedSpec spec;
+  int64_t n_map = 0;   // positions held (first[4^w])
+};
+
+// scratch of the device index build, reused across the seeds of a sequence
+struct IndexWork {
+  DevMem keys0, keys1, vals0, vals1, st, en, erased, tmp;
+  int32_t* host_counts = nullptr;   // pinned, 4^w key counts for the repeat filter
+  int64_t host_counts_n = 0;
+  std::vector<uint8_t> host_erased;
+  IndexWork() = default;
+  IndexWork(const IndexWork&) = delete;
+  IndexWork& operator=(const IndexWork&) = delete;
+  ~IndexWork();
 };
 
 struct RefState {
@@ -189,6 +202,19 @@ int filter_lanes(int d);
 void fill_score(const storm_params& p, ScoreDev& sc);
 const char* default_seeds(int color);
 int check_params(const storm_params* p);
+
+// Codes (one per byte) and mask bytes (or null) of a sequence to the device bit planes; with
+// keep_code the device copy of the codes is handed over (for the index build).
+int upload_genome(const uint8_t* code, const uint8_t* mask, int64_t size, DevGenome& g, hipStream_t st,
+                  DevMem* keep_code = nullptr);
+// index__build_reference (index.c:19-195) on the device: d_code = the sequence's codes, one per
+// byte; d_regions = its masked regions [start, end), sorted.  ix.first / ix.map in the contiguous
+// layout (index.c:157-188).
+int build_index_dev(const uint8_t* d_code, int64_t size, const int2* d_regions, int n_regions, const SeedSpec& seed,
+                    int stdev_distance, DevIndex& ix, IndexWork& w, hipStream_t st, int64_t* erased_out);
+// every seed's index of one sequence; erased (optional) receives the erased key count per seed
+int build_indexes(const HostRef& ref, const DevMem& d_code, const std::vector<SeedSpec>& seeds, int stdev_distance,
+                  std::vector<DevIndex>& out, hipStream_t st, std::vector<int64_t>* erased = nullptr);
 
 // Loads a FASTA genome in the encoding of p.color and parses the seeds (no device work).
 int load_resident(const char* genome_path, const storm_params& p, std::shared_ptr<Resident>& out);

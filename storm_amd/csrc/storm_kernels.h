@@ -170,6 +170,22 @@ hipError_t launch_serve_gather(const ServeGatherArgs& a, hipStream_t st);
 hipError_t launch_serve_scatter(const Slot* slots, const ServeRow* rows, int64_t n, const ServeRegion* regions,
                                 hipStream_t st);
 
+// genome planes and the device index build (k_index.hip)
+hipError_t launch_unpack_codes(const uint8_t* packed, int64_t size, uint8_t* code, hipStream_t st);
+hipError_t launch_genome_planes(const uint8_t* code, const uint8_t* mask, int64_t size, int64_t nw, uint32_t* lo,
+                                uint32_t* hi, uint32_t* mk, hipStream_t st);
+hipError_t launch_index_keys(const uint8_t* code, int64_t map_size, const int2* regions, int n_regions,
+                             const SeedDev& seed, uint32_t ks, uint32_t* keys, int32_t* vals, hipStream_t st);
+hipError_t launch_index_runs(const uint32_t* keys, int64_t n, int32_t* st_, int32_t* en, hipStream_t st);
+hipError_t launch_index_counts(const int32_t* st_, int32_t* en, int64_t ks, hipStream_t st);
+hipError_t launch_index_erase(const uint8_t* erased, int32_t* counts, int64_t ks, hipStream_t st);
+hipError_t launch_index_scatter(const uint32_t* keys, const int32_t* vals, int64_t n, const int32_t* st_,
+                                const int32_t* first, const uint8_t* erased, uint32_t ks, int32_t* map,
+                                hipStream_t st);
+hipError_t sort_key_pos_raw(uint32_t* k0, uint32_t* k1, int32_t* v0, int32_t* v1, int64_t n, int end_bit, void* tmp,
+                            size_t& tmp_bytes, int& result_sel, hipStream_t st);
+hipError_t scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, void* tmp, size_t& tmp_bytes,
+                              hipStream_t st);
 hipError_t launch_fused(const FusedArgs& a, int grid, hipStream_t st);
 hipError_t launch_fused_reset(int32_t* counter, unsigned long long* stats, hipStream_t st);
 int fused_blocks_per_cu(int W, int I, int color);

@@ -105,6 +105,10 @@ def load() -> C.CDLL:
     lib.storm_server_wait.argtypes = [P, L, P, P, P, P, C.POINTER(RequestResult)]
     lib.storm_server_get_stats.argtypes = [P, C.POINTER(ServerStats)]
     lib.storm_server_destroy.argtypes = [P]
+    lib.storm_index_build.argtypes = [I, P, L, P, P, I, C.c_char_p, I, C.POINTER(P)]
+    lib.storm_index_sizes.argtypes = [P, C.POINTER(L), C.POINTER(L), C.POINTER(L), C.POINTER(D)]
+    lib.storm_index_fetch.argtypes = [P, P, P]
+    lib.storm_index_destroy.argtypes = [P]
     lib.alignment_gfx950__set_genome.argtypes = [P, C.c_size_t, C.c_size_t]
     for w in ("octa", "quad", "pair", "mono"):
         getattr(lib, f"alignment_gfx950__init_{w}").argtypes = [C.c_uint] * 6
@@ -178,6 +182,30 @@ class FilterPlan:
             self.close()
         except Exception:
             pass
+
+
+def build_index(packed: np.ndarray, size: int, region_start, region_end, seed: str, stdev_distance: int = 100,
+                device: int = 0):
+    """storm_index_*: the CSR index of one seed over one packed sequence, built on the device.
+
+    Returns (first, map, n_erased, device_ms): first has 4^w + 1 entries, map first[4^w]
+    (index.c:157-188 layout)."""
+    lib = load()
+    packed = np.ascontiguousarray(packed, dtype=np.uint8)
+    rs = np.ascontiguousarray(region_start, dtype=np.int32)
+    re_ = np.ascontiguousarray(region_end, dtype=np.int32)
+    h = C.c_void_p()
+    check(lib.storm_index_build(device, _ptr(packed), int(size), _ptr(rs), _ptr(re_), len(rs), seed.encode(),
+                                int(stdev_distance), C.byref(h)))
+    try:
+        nk, nm, ne, ms = C.c_int64(), C.c_int64(), C.c_int64(), C.c_double()
+        check(lib.storm_index_sizes(h, C.byref(nk), C.byref(nm), C.byref(ne), C.byref(ms)))
+        first = np.zeros(nk.value + 1, np.int32)
+        mp = np.zeros(max(nm.value, 1), np.int32)
+        check(lib.storm_index_fetch(h, _ptr(first), _ptr(mp)))
+        return first, mp[:nm.value], ne.value, ms.value
+    finally:
+        lib.storm_index_destroy(h)
 
 
 def map_files(genome: str, reads: str, sam: str, qual: str | None = None, mode: int = STORM_MODE_BATCHED,

@@ -45,3 +45,21 @@ def test_server_config_defaults_and_argument_errors(tmp_path):
     for bad in (dict(lanes=0), dict(lanes=17), dict(max_batch_reads=0), dict(encodings=0), dict(mode=7)):
         with pytest.raises(lib.StormError):
             lib.Server(str(tmp_path / "missing.fa"), **bad)
+
+
+def test_index_build_argument_errors():
+    """storm_index_build rejects bad region lists and seeds before any device call."""
+    import numpy as np
+    import pytest
+    from storm_amd import lib
+    packed = np.zeros(100, np.uint8)
+    for starts, ends in (([5, 3], [9, 8]),        # not ascending
+                         ([5], [5]),             # empty region
+                         ([5, 8], [10, 12]),     # overlapping
+                         ([390], [401])):        # past the sequence end
+        with pytest.raises(lib.StormError, match="masked region"):
+            lib.build_index(packed, 400, starts, ends, "##-#")
+    with pytest.raises(lib.StormError):
+        lib.build_index(packed, 400, [], [], "##-x")
+    with pytest.raises(lib.StormError, match="exactly one seed"):
+        lib.build_index(packed, 400, [], [], "##-#;#-#")
