@@ -419,10 +419,12 @@ int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int pass_id
     HIPOK(hipStreamSynchronize(st));
     for (int32_t x : lens) maxL = std::max(maxL, (int)x);
   }
-  aa.scratch_stride = (((int64_t)maxL * (2 * sc.I + 1)) + 15) & ~15LL;
+  aa.scratch_stride = (int64_t)maxL * dp_dir_words(sc.I) * 4;
   const int64_t budget = 256LL << 20;
-  int64_t chunk = std::max<int64_t>(1, budget / std::max<int64_t>(aa.scratch_stride, 1));
-  HIPOK(P.align_scratch.ensure((size_t)(std::min(chunk, std::max<int64_t>(n_rec, 1)) * aa.scratch_stride)));
+  // chunks of whole wavefronts: the direction words of a wavefront's 64 records are interleaved
+  int64_t chunk = std::max<int64_t>(64, (budget / std::max<int64_t>(aa.scratch_stride, 1)) & ~63LL);
+  const int64_t chunk_alloc = (std::min(chunk, std::max<int64_t>(n_rec, 1)) + 63) & ~63LL;
+  HIPOK(P.align_scratch.ensure((size_t)(chunk_alloc * aa.scratch_stride)));
   aa.scratch = P.align_scratch.as<uint8_t>();
   for (int64_t b0 = 0; b0 < n_rec; b0 += chunk) {
     aa.base = b0;
@@ -746,7 +748,7 @@ int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback,
   FusedState& F = m->fs;
   ScoreDev sc{};
   fill_score(m->p, sc);
-  const int64_t stride = (((int64_t)maxL * (2 * sc.I + 1)) + 15) & ~15LL;
+  const int64_t stride = (int64_t)maxL * dp_dir_words(sc.I) * 4;
   const int bpc = fused_blocks_per_cu(sc.W, sc.I, m->p.color);
   if (bpc <= 0) return fail(STORM_ERR_DEVICE, "fused kernel: no occupancy for this band class");
   const int max_grid = device_cus(m->device) * bpc;

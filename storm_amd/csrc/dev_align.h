@@ -36,10 +36,14 @@ struct DPConst {
 // quality levels are held in registers: bit c of the row-i genome words is code win + i + c; per
 // plane NW words from word index wi, a new word fetched every 32 rows, one word ahead of its use,
 // so a row issues no dependent loads.
+// Directions (TB): the cell (i, c) is nibble c & 7 of word i * DW + (c >> 3) of the lane; the
+// lanes of a wavefront interleave their words (tbw = wavefront base + lane, word k at tbw[64 k]),
+// so each row is DW coalesced 256-byte stores of whole cache lines.
 template <int IM, bool COLOR>
 struct DPLane {
   static constexpr int NCM = 2 * IM + 1;
   static constexpr int NW = NCM > 32 ? 4 : 3;
+  static constexpr int DW = (NCM + 7) / 8;
   int sc[NCM];
   int meta[NCM];
   uint32_t gl[NW], gh[NW], gm[NW];
@@ -51,9 +55,9 @@ struct DPLane {
   int has_qual;
   int L, i, win, S, bp;
 
-  // row 0 (alignment.c:326-360); tbm (TB) receives the row's direction bytes
+  // row 0 (alignment.c:326-360); tbw (TB) receives the row's directions
   template <bool TB>
-  __device__ void begin(const GenomeDev& g, const ReadsDev& r, const DPConst& k, int rs, int h, uint8_t* tbm) {
+  __device__ void begin(const GenomeDev& g, const ReadsDev& r, const DPConst& k, int rs, int h, uint32_t* tbw) {
     L = r.len[rs >> 1];
     win = (h - k.I > -k.d) ? h - k.I : -k.d;
     pl = read_planes(r, rs);
@@ -93,11 +97,14 @@ struct DPLane {
           sc[c] = MIN_SCORE;
           meta[c] = m_make(3, 0, 0, 0, 0);
         }
-        if (TB) tbm[c] = 3;
       } else {   // columns past the band hold the border, so the top neighbour needs no test
         sc[c] = MIN_SCORE;
         meta[c] = m_make(3, I, 0, 0, 0);
       }
+    }
+    if (TB) {   // row 0: every cell from the corner (3)
+#pragma unroll
+      for (int w = 0; w < DW; ++w) tbw[64 * w] = 0x33333333u;
     }
     bp = (COLOR ? (L - 2) : L) * k.top_match;
     S = 0x7fffffff;
@@ -109,7 +116,7 @@ struct DPLane {
 
   // the next row (i + 1) of the recurrence, and the survival bound after it
   template <bool TB>
-  __device__ void row(const GenomeDev& g, const DPConst& k, uint8_t* tbm) {
+  __device__ void row(const GenomeDev& g, const DPConst& k, uint32_t* tbw) {
     ++i;
     // advance the genome windows by one code
     if (++sh == 32) {
@@ -137,6 +144,9 @@ struct DPLane {
     const uint32_t mis0 = (wl0 ^ rb0) | (wh0 ^ rb1), mis1 = NCM > 32 ? ((wl1 ^ rb0) | (wh1 ^ rb1)) : 0u;
     int max_on_line = MIN_SCORE;
     int left_sc = MIN_SCORE, left_meta = m_make(3, I, 0, 0, 0);   // border (alignment.c:303-318)
+    uint32_t dw[DW];
+#pragma unroll
+    for (int w = 0; w < DW; ++w) dw[w] = 0;
 #pragma unroll
     for (int c = 0; c < NCM; ++c) {
       if (c < NC) {
@@ -203,13 +213,17 @@ struct DPLane {
           const int l_hi = ((left_meta ^ (gj << 8)) & 0x300) | msel(lfw > 0x1000, iauto_w | (lfw - 0x1000), c_af);
           nhi = msel(take_l, l_hi, msel(take_t, t_hi, d_hi));
         }
-        if (TB) tbm[i * NC + c] = (uint8_t)(nfrom | ((nsc > csc) ? 4 : 0));
+        if (TB) dw[c >> 3] |= (uint32_t)(nfrom | ((nsc > csc) ? 4 : 0)) << ((c & 7) * 4);
         sc[c] = nsc;
         meta[c] = COLOR ? (n_lo | nhi) : n_lo;
         left_sc = nsc;
         left_meta = meta[c];
         if (max_on_line < nsc) max_on_line = nsc;
       }
+    }
+    if (TB) {
+#pragma unroll
+      for (int w = 0; w < DW; ++w) tbw[64 * (i * DW + w)] = dw[w];
     }
     if (max_on_line + bp < S) S = max_on_line + bp;
     bp -= k.top_match;
@@ -248,20 +262,21 @@ struct DPLane {
 };
 
 // One candidate: read strand rs (2*read + strand), diagonal h.
-// TB = true: tbm is this thread's scratch of L*(2I+1) direction bytes and, when the record can be
-// accepted, the traceback (alignment order, 2 codes per byte) goes to tb.
+// TB = true: tbw is this lane's direction words (L rows of DPLane::DW words, interleaved with the
+// other lanes of the wavefront) and, when the record can be accepted, the traceback (alignment
+// order, 2 codes per byte) goes to tb.
 // TB = false: the score pass alone, nothing is stored.  A record can only ever be accepted when
 // score > T and S >= T with T >= t (T starts at -t and only rises, run.c:568,683-685), so this
 // pass stops at the first row where the bound drops below t (S < t then holds) and returns
 // score = MIN_SCORE; otherwise it returns the same score, S and best_j as the TB pass.
 template <int IM, bool COLOR, bool TB = true>
-__device__ AlignRec dp_one(const GenomeDev& g, const ReadsDev& r, const ScoreDev& scd, int rs, int h, uint8_t* tbm,
+__device__ AlignRec dp_one(const GenomeDev& g, const ReadsDev& r, const ScoreDev& scd, int rs, int h, uint32_t* tbw,
                            uint8_t* tb) {
   const DPConst k(scd);
   DPLane<IM, COLOR> st;
-  st.template begin<TB>(g, r, k, rs, h, tbm);
+  st.template begin<TB>(g, r, k, rs, h, tbw);
   while (st.more()) {
-    st.template row<TB>(g, k, tbm);
+    st.template row<TB>(g, k, tbw);
     if (!TB && st.S < k.t) return st.rejected();   // the rest of the rows cannot raise S
   }
   int bc;
@@ -275,7 +290,8 @@ __device__ AlignRec dp_one(const GenomeDev& g, const ReadsDev& r, const ScoreDev
       const int j = i + c;
       o.best_i0 = (int16_t)i;
       o.best_j0 = (int16_t)j;
-      const int cell = tbm[i * NC + c];
+      constexpr int DW = DPLane<IM, COLOR>::DW;
+      const int cell = (int)(tbw[64 * (i * DW + (c >> 3))] >> ((c & 7) * 4)) & 0xF;
       const int from = cell & 3;
       const int ri = rcode(pl, i);
       int code;

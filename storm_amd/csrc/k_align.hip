@@ -5,7 +5,8 @@
 // One thread owns one candidate that passed the filter; the 2I+1 band columns of a row live in
 // registers (template bound IM on I).  A score pass decides whether the record can ever be
 // accepted (score > t and S >= t); only those records are aligned again with the per-cell
-// direction byte in a scratch matrix in HBM, which the same thread walks back.
+// directions in scratch words (4 bits per cell, interleaved across the lanes of a wavefront),
+// which the same thread walks back.
 //
 // The kernel is threshold-free: alongside the score it returns the survival bound
 //   S = min over rows i >= 1 of (max_on_line(i) + best_possible(i)),
@@ -26,7 +27,8 @@ __global__ __launch_bounds__(128) void k_align(AlignArgs a) {
   // cent of the filter passes) are aligned again with directions and traceback
   AlignRec x = dp_one<IM, COLOR, false>(a.g, a.r, a.sc, a.rec_rs[rec], a.rec_diag[rec], nullptr, nullptr);
   if (x.score > a.sc.t && x.S >= a.sc.t)
-    x = dp_one<IM, COLOR, true>(a.g, a.r, a.sc, a.rec_rs[rec], a.rec_diag[rec], a.scratch + k * a.scratch_stride,
+    x = dp_one<IM, COLOR, true>(a.g, a.r, a.sc, a.rec_rs[rec], a.rec_diag[rec],
+                                (uint32_t*)(a.scratch + (k >> 6) * 64 * a.scratch_stride) + (k & 63),
                                 a.tb + rec * TBB);
   a.out[rec] = x;
 }

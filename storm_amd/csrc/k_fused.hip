@@ -45,7 +45,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 4 :
   const int G = a.group;
   const int I = a.sc.I;
   const FilterConst fk = filter_const(a.sc.fmatch, a.sc.fmismatch, a.sc.fopen, a.sc.fext, a.sc.fthr);
-  uint8_t* scratch = a.scratch + (int64_t)blockIdx.x * 64 * a.scratch_stride + (int64_t)lane * a.scratch_stride;
+  uint32_t* tbw = (uint32_t*)(a.scratch + (int64_t)blockIdx.x * 64 * a.scratch_stride) + lane;
   uint8_t* tbbuf = a.tbbuf + (int64_t)blockIdx.x * F_PCAP * TBB;
   uint8_t* wb = a.work + (int64_t)blockIdx.x * FUSED_WORK_BYTES;
   int32_t* cdiag = (int32_t*)wb;
@@ -125,12 +125,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 4 :
       nv += __shfl(incl, 63, 64);
     }
     __syncthreads();
-    // traceback pass over those records only: directions in the lane's scratch, codes to tbbuf
+    // traceback pass over those records only: directions in the wavefront's scratch words, codes
+    // to tbbuf
     for (int e = lane; e < nv; e += 64) {
       const int k = hbuf[e] & 0xffff;
       const int c = rec2cand[k];
       const int64_t rs = 2 * base + crs[c];
-      recs[k] = dp_one<IM, COLOR, true>(a.g, a.r, a.sc, (int)rs, cdiag[c], scratch, tbbuf + (int64_t)k * TBB);
+      recs[k] = dp_one<IM, COLOR, true>(a.g, a.r, a.sc, (int)rs, cdiag[c], tbw, tbbuf + (int64_t)k * TBB);
     }
     __syncthreads();
     // hit map replay (run.c:660-697, hit_map.c:50-144): one lane per read

@@ -44,8 +44,8 @@ struct AlignArgs {
   const int32_t* rec_diag;   // diagonal h
   int64_t n;                 // records in this launch
   int64_t base;              // index of the first record (into rec_* and outputs)
-  uint8_t* scratch;          // n * scratch_stride bytes
-  int64_t scratch_stride;
+  uint8_t* scratch;          // direction words: ceil(n / 64) wavefronts x 64 x scratch_stride bytes
+  int64_t scratch_stride;    // bytes per record: maxL x dp_dir_words(I) x 4
   AlignRec* out;
   uint8_t* tb;               // TBB bytes per record, traceback order (end to start)
 };
@@ -128,8 +128,8 @@ struct FusedArgs {
   Slot* slots;               // 4 per read
   uint8_t* cells;            // 4 * TBB bytes per read: the traceback cells
   uint8_t* overflow;         // per read
-  uint8_t* scratch;          // per workgroup: 64 * scratch_stride direction bytes
-  int64_t scratch_stride;
+  uint8_t* scratch;          // per workgroup: 64 * scratch_stride bytes of direction words
+  int64_t scratch_stride;    // bytes per lane: maxL x dp_dir_words(I) x 4
   uint8_t* tbbuf;            // per workgroup: FUSED_PCAP * TBB traceback bytes of the current flush
   uint8_t* work;             // per workgroup: FUSED_WORK_BYTES of candidates and records
   int ref_id;
@@ -169,6 +169,11 @@ struct ServeGatherArgs {
 hipError_t launch_serve_gather(const ServeGatherArgs& a, hipStream_t st);
 hipError_t launch_serve_scatter(const Slot* slots, const ServeRow* rows, int64_t n, const ServeRegion* regions,
                                 hipStream_t st);
+
+// The DP kernels are compiled for band bounds IM = 1, 3, 7, 15, 31 (2IM + 1 columns); -i I uses the
+// smallest IM >= I.  Traceback directions are 4 bits per cell, 8 cells per 32-bit word.
+__host__ __device__ inline int dp_band_bound(int I) { return I <= 1 ? 1 : I <= 3 ? 3 : I <= 7 ? 7 : I <= 15 ? 15 : 31; }
+__host__ __device__ inline int dp_dir_words(int I) { return (2 * dp_band_bound(I) + 1 + 7) / 8; }
 
 // genome planes and the device index build (k_index.hip)
 hipError_t launch_unpack_codes(const uint8_t* packed, int64_t size, uint8_t* code, hipStream_t st);
