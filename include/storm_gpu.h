@@ -171,7 +171,24 @@ typedef struct storm_reads_view {
   const uint8_t *first_base;
   const uint8_t *first_qual;
   const char *db_name;   /* @RG ID/SM; NULL -> "*" */
+  const uint8_t *no_qual; /* per read, 1 = the read has no quality (its FASTQ quality line is empty or
+                           * starts with '#', which the reference reads as a comment,
+                           * reads_data.c:459-532): QUAL '*' in the SAM, levels 3 in the DP (its qual
+                           * row is 0xFF, first_qual 3); NULL = every read as the qual array says */
 } storm_reads_view;
+
+/* Reads loaded from a file into packed arrays ordered by tag: load_reads_db + sort_reads_db
+ * (reads_data.h:63-68, reads_data.c:59-67,73-566).  use_device != 0 parses, packs and sorts FASTQ
+ * files (four lines per record) on the device; FASTA, csfasta(+qual) and irregular FASTQ files are
+ * loaded by the host.  storm_reads_get fills a view of the arrays (valid until
+ * storm_reads_destroy), and optionally the n_reads tags, whether the device loaded them and its
+ * kernel time.  storm_map_files loads FASTQ files this way. */
+typedef struct storm_reads storm_reads;
+int storm_reads_load(int device, const char *reads_path, const char *qual_path, const storm_params *params,
+                     int use_device, storm_reads **out);
+int storm_reads_get(const storm_reads *reads, storm_reads_view *view, uint64_t *tags, int *loaded_on_device,
+                    double *device_ms);
+void storm_reads_destroy(storm_reads *reads);
 
 typedef struct storm_mapper storm_mapper;
 /* Loads a FASTA genome, builds the seed indexes (host) and keeps genome + indexes resident in

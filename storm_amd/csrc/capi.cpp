@@ -1029,6 +1029,7 @@ static int build_host_reads(const storm_reads_view* v, int color, HostReads& hr)
     if (hr.first_base.empty()) hr.first_base.assign((size_t)hr.n, 0);
     if (hr.first_qual.empty()) hr.first_qual.assign((size_t)hr.n, 0);
   }
+  if (v->qual && v->no_qual) hr.no_qual.assign(v->no_qual, v->no_qual + hr.n);
   for (int64_t i = 0; i < hr.n; ++i) {
     hr.names[(size_t)i] = v->names ? std::string(v->names[i]) : ("r" + std::to_string(i));
     hr.max_len = std::max(hr.max_len, (int)v->len[i]);
@@ -1077,10 +1078,17 @@ int storm_map_files(int device, const char* genome_path, const char* reads_path,
   if (rc) return rc;
   const double t0 = now_s();
   HostReads hr;
-  std::string err;
-  rc = load_reads(reads_path, qual_path, *params, hr, err);
-  if (rc) return fail(rc, "%s", err.c_str());
-  sort_reads_by_tag(hr);   // run.c:826
+  bool on_device = false;
+  if (!qual_path) {   // FASTQ: parsed, packed and ordered by tag on the device
+    rc = load_reads_device(device, reads_path, *params, hr, on_device, nullptr);
+    if (rc) return rc;
+  }
+  if (!on_device) {
+    std::string err;
+    rc = load_reads(reads_path, qual_path, *params, hr, err);
+    if (rc) return fail(rc, "%s", err.c_str());
+    sort_reads_by_tag(hr);   // run.c:826
+  }
   const double t1 = now_s();
   storm_mapper* m = nullptr;
   rc = storm_mapper_create(device, genome_path, params, &m);
@@ -1097,6 +1105,7 @@ int storm_map_files(int device, const char* genome_path, const char* reads_path,
   v.len = hr.len.data();
   v.first_base = hr.first_base.empty() ? nullptr : hr.first_base.data();
   v.first_qual = hr.first_qual.empty() ? nullptr : hr.first_qual.data();
+  v.no_qual = hr.no_qual.empty() ? nullptr : hr.no_qual.data();
   v.db_name = hr.db_name.c_str();
   storm_map_stats local{};
   rc = storm_mapper_map(m, &v, mode, sam_path, argc, argv, &local);

@@ -261,21 +261,57 @@ int load_reads(const char* path, const char* qual_path, const storm_params& p, H
       title = data.substr(lines[li].first + 9, lines[li].second - 9);
     ++li;
   }
-  if (fastq) {   // reads_data.c:351-566, records of 4 lines
-    while (li < lines.size()) {
-      if (lines[li].second == 0) { ++li; continue; }
-      const char* h = data.c_str() + lines[li].first;
-      if (h[0] != '@' || li + 3 >= lines.size() + 0 || li + 1 >= lines.size()) {
-        err = "reads file seems corrupt near line " + std::to_string(li + 1);
+  if (fastq) {   // reads_data.c:351-566
+    // get_input_line_type_fastq (load_data.c:85-107): lines are typed by their position relative to
+    // the first line after the comments and by their first character; crt is the type of the
+    // previous typed line (an empty line leaves it unchanged)
+    enum { LT_UNK, LT_COM, LT_HEAD, LT_SEQ, LT_QHEAD, LT_QUAL };
+    const size_t li0 = li;
+    int crt = LT_UNK;
+    auto type = [&](size_t j) {
+      if (lines[j].second == 0) return (int)LT_UNK;
+      const char c0 = data[lines[j].first];
+      const size_t rel = (j - li0) % 4;
+      int t;
+      if (crt == LT_COM || c0 == '#') t = LT_COM;
+      else if (rel == 0 && c0 == '@') t = LT_HEAD;
+      else if (rel == 2 && c0 == '+') t = LT_QHEAD;
+      else if (rel == 1 && crt == LT_HEAD) t = LT_SEQ;
+      else if (rel == 3 && crt == LT_QHEAD) t = LT_QUAL;
+      else t = LT_UNK;
+      crt = (t == LT_SEQ || t == LT_HEAD || t == LT_QUAL || t == LT_QHEAD) ? t : LT_UNK;
+      return t;
+    };
+    // first pass (reads_data.c:395-434): a header must be followed by a sequence line, and an
+    // untyped line (empty, or out of place) makes the file corrupt
+    int64_t count = 0;
+    int next = -1;
+    for (size_t j = li; j < lines.size();) {
+      const int t = type(j++);
+      if (t == LT_HEAD && j < lines.size()) {
+        next = type(j++);
+        if (next == LT_SEQ) { ++count; continue; }
+      }
+      if ((t == LT_HEAD && next != LT_SEQ) || t == LT_UNK) {
+        err = "reads file seems corrupt at line " + std::to_string(j);
         return STORM_ERR_INVALID;
       }
-      RawRead r;
-      r.name = parse_read_name(h, lines[li].second);
-      r.seq = data.substr(lines[li + 1].first, lines[li + 1].second);
-      if (li + 3 < lines.size() && lines[li + 2].second > 0 && data[lines[li + 2].first] == '+')
-        r.qual = data.substr(lines[li + 3].first, lines[li + 3].second);
-      raw.push_back(std::move(r));
-      li += 4;
+    }
+    // second pass (reads_data.c:452-532): a header starts a read, the sequence and quality lines
+    // fill it (a quality line typed as a comment leaves the read without quality)
+    raw.reserve((size_t)count);
+    for (size_t j = li; j < lines.size() && (int64_t)raw.size() <= count; ++j) {
+      const int t = type(j);
+      if (t == LT_HEAD) {
+        if ((int64_t)raw.size() == count) break;
+        RawRead r;
+        r.name = parse_read_name(data.c_str() + lines[j].first, lines[j].second);
+        raw.push_back(std::move(r));
+      } else if (t == LT_SEQ && !raw.empty()) {
+        raw.back().seq = data.substr(lines[j].first, lines[j].second);
+      } else if (t == LT_QUAL && !raw.empty()) {
+        raw.back().qual = data.substr(lines[j].first, lines[j].second);
+      }
     }
   } else {       // reads_data.c:73-349, '>' headers, sequence lines until the next header
     while (li < lines.size()) {
@@ -356,7 +392,14 @@ int load_reads(const char* path, const char* qual_path, const storm_params& p, H
       if (ct > tag) tag = ct;
     }
     out.tag[i] = tag;
-    if (!out.has_qual || r.qual.empty()) continue;
+    if (!out.has_qual) continue;
+    if (r.qual.empty()) {   // no quality: levels 3 everywhere, as the reference's NULL quality
+      if (out.no_qual.empty()) out.no_qual.assign((size_t)n, 0);
+      out.no_qual[(size_t)i] = 1;
+      memset(&out.qual[(size_t)(i * out.qual_stride)], 0xFF, (size_t)out.qual_stride);
+      if (p.color) out.first_qual[(size_t)i] = 3;
+      continue;
+    }
     uint8_t* qs = &out.qual[(size_t)(i * out.qual_stride)];
     std::fill(q.begin(), q.end(), 0);
     if (fastq) {   // reads_data.c:516-531
@@ -411,6 +454,10 @@ void sort_reads_by_tag(HostReads& r) {
   if (!r.first_base.empty()) { s.first_base.resize(r.n); s.first_qual.resize(r.n); }
   for (int64_t i = 0; i < r.n; ++i) {
     const int64_t j = perm[i];
+    if (!r.no_qual.empty()) {
+      if (s.no_qual.empty()) s.no_qual.assign((size_t)r.n, 0);
+      s.no_qual[i] = r.no_qual[j];
+    }
     s.names[i] = std::move(r.names[j]);
     s.len[i] = r.len[j];
     s.tag[i] = r.tag[j];
@@ -589,15 +636,18 @@ int write_sam(const char* path, const std::vector<HostRef>& refs, const HostRead
   const double lg = log(.3);
   const int delta_m = p.match - p.mismatch;
   const int M = p.max_hits;
-  std::string cigar, seq, line;
-  for (int64_t id = 0; id < reads.n; ++id) {
+  // the records of read `id` appended to `line` (hit_map.c:325-504); reads are formatted in
+  // parallel chunks and written in read order
+  auto format_read = [&](int64_t id, std::string& line, std::string& cigar, std::string& seq) {
     const HostSlot* row = &slots[(size_t)(4 * id)];
     const int L = reads.len[id];
-    const uint8_t* qs = reads.has_qual ? &reads.qual[(size_t)(id * reads.qual_stride)] : nullptr;
+    const bool rq = reads.has_qual && !(!reads.no_qual.empty() && reads.no_qual[(size_t)id]);
+    const uint8_t* qs = rq ? &reads.qual[(size_t)(id * reads.qual_stride)] : nullptr;
     auto reportable = [&](int k) {   // hit_map.c:359-370
       return k < M && row[k].ref_start != -1 &&
              (p.uniqueness == 0 || ((k + 1 < M) && (row[k].score - row[k + 1].score) >= p.uniqueness) || (k + 1 == M));
     };
+    char buf[96];
     if (row[0].ref_id != -1) {
       int nrep = 0;
       for (int k = 0; reportable(k); ++k) nrep++;
@@ -613,9 +663,7 @@ int write_sam(const char* path, const std::vector<HostRef>& refs, const HostRead
         mq = mq < 254 ? mq : 254;
         const HostRef& ref = refs[(size_t)row[k].ref_id];
         int ed = traceback_to_cigar(row[k], ref, color ? &sym[(size_t)row[k].ref_id] : nullptr, color, cigar, seq);
-        line.clear();
         line += reads.names[(size_t)id];
-        char buf[96];
         snprintf(buf, sizeof buf, "\t%d\t", flag);
         line += buf;
         line += ref.name.empty() ? "*" : ref.name;
@@ -639,14 +687,15 @@ int write_sam(const char* path, const std::vector<HostRef>& refs, const HostRead
         }
         snprintf(buf, sizeof buf, "\tAS:i:%d\tNM:i:%d\tNH:i:%d\n", row[k].score, ed, nrep);
         line += buf;
-        fwrite(line.data(), 1, line.size(), out);
       }
     } else if (p.print_all) {
       const std::string& nm = reads.names[(size_t)id];
       if (color) {
-        fprintf(out, "%s\t4\t*\t0\t255\t*\t*\t0\t0\t*\t*\n", nm.c_str());
+        line += nm;
+        line += "\t4\t*\t0\t255\t*\t*\t0\t0\t*\t*\n";
       } else {
-        line = nm + "\t4\t*\t0\t255\t*\t*\t0\t0\t";
+        line += nm;
+        line += "\t4\t*\t0\t255\t*\t*\t0\t0\t";
         const uint8_t* cs = &reads.codes[(size_t)(id * reads.code_stride)];
         for (int q = 0; q < L; ++q) line += BASES[get2(cs, q)];
         line += '\t';
@@ -654,9 +703,27 @@ int write_sam(const char* path, const std::vector<HostRef>& refs, const HostRead
           for (int q = 0; q < L; ++q) { int v = p.qual_base + UB[get2(qs, q)]; line += (char)(v < 126 ? v : 126); }
         } else line += '*';
         line += '\n';
-        fwrite(line.data(), 1, line.size(), out);
       }
     }
+  };
+  const int nt = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+  const int64_t per_thread = 32768, block = per_thread * nt;
+  std::vector<std::string> part((size_t)nt);
+  for (int64_t b0 = 0; b0 < reads.n; b0 += block) {
+    const int64_t b1 = std::min(reads.n, b0 + block);
+    auto work = [&](int t) {
+      std::string& out_t = part[(size_t)t];
+      out_t.clear();
+      std::string cigar, seq;
+      const int64_t r0 = b0 + t * per_thread, r1 = std::min(b1, r0 + per_thread);
+      for (int64_t id = r0; id < r1; ++id) format_read(id, out_t, cigar, seq);
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt && b0 + t * per_thread < b1; ++t) th.emplace_back(work, t);
+    work(0);
+    for (auto& x : th) x.join();
+    for (int t = 0; t < nt; ++t)
+      if (!part[(size_t)t].empty()) fwrite(part[(size_t)t].data(), 1, part[(size_t)t].size(), out);
   }
   if (fclose(out) != 0) { err = std::string("error writing ") + path; return STORM_ERR_IO; }
   return STORM_OK;

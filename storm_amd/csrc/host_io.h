@@ -30,6 +30,7 @@ struct HostReads {                     // ReadsDBType (reads_data.h:46-55), one 
   std::vector<uint8_t> qual;           // 2-bit quality levels, qual_stride bytes per read
   std::vector<uint8_t> first_base, first_qual;   // color mode
   std::vector<uint64_t> tag;
+  std::vector<uint8_t> no_qual;        // per read (or empty): quality line ignored, QUAL '*'
 };
 
 struct SeedSpec {                      // SeedType (seed.h:41-55)

@@ -216,6 +216,11 @@ int build_index_dev(const uint8_t* d_code, int64_t size, const int2* d_regions, 
 int build_indexes(const HostRef& ref, const DevMem& d_code, const std::vector<SeedSpec>& seeds, int stdev_distance,
                   std::vector<DevIndex>& out, hipStream_t st, std::vector<int64_t>* erased = nullptr);
 
+// FASTQ reads parsed, packed and ordered by tag on the device (read_ingest.cpp).  handled = false
+// when the file is not a four-lines-per-record FASTQ file: the caller then uses load_reads.
+int load_reads_device(int device, const char* path, const storm_params& p, HostReads& out, bool& handled,
+                      double* device_ms);
+
 // Loads a FASTA genome in the encoding of p.color and parses the seeds (no device work).
 int load_resident(const char* genome_path, const storm_params& p, std::shared_ptr<Resident>& out);
 int ensure_ref_resident(storm_mapper* m, size_t r);

@@ -175,6 +175,35 @@ hipError_t launch_serve_scatter(const Slot* slots, const ServeRow* rows, int64_t
 __host__ __device__ inline int dp_band_bound(int I) { return I <= 1 ? 1 : I <= 3 ? 3 : I <= 7 ? 7 : I <= 15 ? 15 : 31; }
 __host__ __device__ inline int dp_dir_words(int I) { return (2 * dp_band_bound(I) + 1 + 7) / 8; }
 
+// FASTQ reads parsed on the device (k_ingest.hip)
+struct IngestArgs {
+  const uint8_t* text;       // the file's records (after the leading comment lines)
+  int64_t n_bytes;
+  const int64_t* nl;         // byte offsets of the newlines
+  int64_t n_nl;
+  int64_t n_reads;           // records of four lines
+  int color, qual_base;
+  int32_t* len;              // out: read length, -1 irregular record, -2 empty read
+  int64_t* header;           // out: offset of the header line
+  uint8_t* hasq;             // out: quality line not empty
+  uint8_t* codes;            // out: n_reads x code_stride packed codes (zeroed)
+  uint8_t* qual;             // out: packed quality levels (zeroed), or null
+  int64_t code_stride;
+  uint64_t* tag;             // out
+  uint8_t* first_base;       // out (color)
+  uint8_t* first_qual;       // out (color)
+  uint8_t* err;              // out: quality character below the base
+};
+int64_t nl_chunks(int64_t n_bytes);
+hipError_t launch_nl_count(const uint8_t* d, int64_t n, int32_t* counts, int64_t n_chunks, hipStream_t st);
+hipError_t launch_nl_positions(const uint8_t* d, int64_t n, const int32_t* first, int64_t n_chunks, int64_t* nl,
+                               hipStream_t st);
+hipError_t launch_fastq_len(const IngestArgs& a, hipStream_t st);
+hipError_t launch_fastq_pack(const IngestArgs& a, hipStream_t st);
+hipError_t launch_iota_i32(int32_t* v, int64_t n, hipStream_t st);
+hipError_t sort_tags(uint64_t* k0, uint64_t* k1, int32_t* v0, int32_t* v1, int64_t n, void* tmp, size_t& tmp_bytes,
+                     int& result_sel, hipStream_t st);
+
 // genome planes and the device index build (k_index.hip)
 hipError_t launch_unpack_codes(const uint8_t* packed, int64_t size, uint8_t* code, hipStream_t st);
 hipError_t launch_genome_planes(const uint8_t* code, const uint8_t* mask, int64_t size, int64_t nw, uint32_t* lo,

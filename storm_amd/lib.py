@@ -44,7 +44,7 @@ class ReadsView(C.Structure):
     _fields_ = [("n_reads", C.c_int64), ("names", C.POINTER(C.c_char_p)), ("codes", C.c_void_p),
                 ("code_stride", C.c_int64), ("qual", C.c_void_p), ("qual_stride", C.c_int64),
                 ("len", C.c_void_p), ("first_base", C.c_void_p), ("first_qual", C.c_void_p),
-                ("db_name", C.c_char_p)]
+                ("db_name", C.c_char_p), ("no_qual", C.c_void_p)]
 
 
 class ServerConfig(C.Structure):
@@ -109,6 +109,9 @@ def load() -> C.CDLL:
     lib.storm_index_sizes.argtypes = [P, C.POINTER(L), C.POINTER(L), C.POINTER(L), C.POINTER(D)]
     lib.storm_index_fetch.argtypes = [P, P, P]
     lib.storm_index_destroy.argtypes = [P]
+    lib.storm_reads_load.argtypes = [I, C.c_char_p, C.c_char_p, C.POINTER(Params), I, C.POINTER(P)]
+    lib.storm_reads_get.argtypes = [P, C.POINTER(ReadsView), P, C.POINTER(I), C.POINTER(D)]
+    lib.storm_reads_destroy.argtypes = [P]
     lib.alignment_gfx950__set_genome.argtypes = [P, C.c_size_t, C.c_size_t]
     for w in ("octa", "quad", "pair", "mono"):
         getattr(lib, f"alignment_gfx950__init_{w}").argtypes = [C.c_uint] * 6
@@ -206,6 +209,47 @@ def build_index(packed: np.ndarray, size: int, region_start, region_end, seed: s
         return first, mp[:nm.value], ne.value, ms.value
     finally:
         lib.storm_index_destroy(h)
+
+
+def load_reads(path: str, qual: str | None = None, use_device: bool = True, device: int = 0, **kw) -> dict:
+    """storm_reads_*: a reads file loaded into packed arrays ordered by tag (load_reads_db +
+    sort_reads_db).  FASTQ files are parsed on the device when use_device is set."""
+    lib = load()
+    p = params(**kw)
+    h = C.c_void_p()
+    check(lib.storm_reads_load(device, path.encode(), qual.encode() if qual else None, C.byref(p), int(use_device),
+                               C.byref(h)))
+    try:
+        v = ReadsView()
+        n = None
+        dev, ms = C.c_int(), C.c_double()
+        check(lib.storm_reads_get(h, C.byref(v), None, C.byref(dev), C.byref(ms)))
+        n = v.n_reads
+        tags = np.zeros(max(n, 1), np.uint64)
+        check(lib.storm_reads_get(h, C.byref(v), _ptr(tags), None, None))
+
+        def arr(ptr, count, dtype):
+            if not ptr or count == 0:
+                return None
+            ct = {np.uint8: C.c_uint8, np.int32: C.c_int32}[dtype]
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(count,)).copy()
+        out = {
+            "n": n,
+            "names": [v.names[i].decode() for i in range(n)],
+            "codes": arr(v.codes, n * v.code_stride, np.uint8).reshape(n, v.code_stride),
+            "qual": None if not v.qual else arr(v.qual, n * v.qual_stride, np.uint8).reshape(n, v.qual_stride),
+            "len": arr(v.len, n, np.int32),
+            "first_base": arr(v.first_base, n, np.uint8),
+            "first_qual": arr(v.first_qual, n, np.uint8),
+            "db_name": v.db_name.decode() if v.db_name else None,
+            "no_qual": arr(v.no_qual, n, np.uint8),
+            "tags": tags[:n],
+            "on_device": bool(dev.value),
+            "device_ms": ms.value,
+        }
+        return out
+    finally:
+        lib.storm_reads_destroy(h)
 
 
 def map_files(genome: str, reads: str, sam: str, qual: str | None = None, mode: int = STORM_MODE_BATCHED,

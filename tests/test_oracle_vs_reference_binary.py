@@ -107,3 +107,20 @@ def test_oracle_equals_reference_binary_without_qualities(mode, tmp_path, oracle
     gpath, rargs = make_inputs(str(tmp_path), mode, "plain", 600, 50, 31, fasta_reads=True)
     a, b = run_both(str(tmp_path), mode, gpath, rargs, ["-i", "3", "-M", "2"])
     assert a == b
+
+
+def test_oracle_equals_reference_binary_hash_quality_lines(tmp_path, oracle_built):
+    """FASTQ quality lines starting with '#' are comments to the reference's reader
+    (get_input_line_type_fastq, load_data.c:85-107): those reads map without quality, QUAL '*'."""
+    from storm_amd import synth
+    from reads_files import write_fastq_hash_quals
+    tmp = str(tmp_path)
+    gpath = os.path.join(tmp, "g.fa")
+    full = _genome(gpath, "plain", 41)
+    r = synth.simulate_nucleotide_reads(full, 900, 50, 48, sub=0.03, indel=0.01)
+    rpath = os.path.join(tmp, "r.fq")
+    write_fastq_hash_quals(rpath, r)
+    a, b = run_both(tmp, "n", gpath, ["-r", rpath], ["-i", "3", "-M", "2"])
+    recs = [line.split("\t") for line in a if not line.startswith("@")]
+    assert len(recs) > 5 and any(f[10] == "*" for f in recs) and any(f[10] != "*" for f in recs)
+    assert a == b
