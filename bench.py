@@ -34,8 +34,10 @@ VALU_PEAK_PK16 = 78.6        # T 16-bit ops/s with packed VALU instructions (2 p
 # VALU lane-instructions k_filter<8> issues per cell at L=75 (SQ_INSTS_VALU x 64 / cells, rocprofv3 --pmc,
 # profiles/k_filter_W8_L75_10Mcand.md); the kernel's own work measure for the VALU-issue roofline
 VALU_INSTR_PER_CELL_W8_L75 = 5.447
-# HBM bytes per candidate of the same dispatch (FETCH_SIZE + WRITE_SIZE, KiB x 1024 / candidates)
-HBM_BYTES_PER_CAND_W8_L75 = 24.0
+# HBM bytes per candidate of the same dispatch: (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00) x 1024 /
+# candidates, the factors calibrated on a device-to-device copy of known size in the same run
+# (profiles/k_filter_W8_L75_10Mcand.md)
+HBM_BYTES_PER_CAND_W8_L75 = 45.8
 
 
 def log(*a):
