@@ -261,6 +261,28 @@ struct DPLane {
   }
 };
 
+// Nucleotide mode: true when the DP of candidate (rs, h) cannot score above the filter's u8 score
+// fscore, so a record with fscore <= t can never be accepted (score > T >= t is required) and the
+// DP may be skipped.  Holds when every quality level of the read strand is 3 (the DP's pair scores
+// are then the filter's +m / -x; at lower levels its mismatch costs less), I <= 15 and the
+// window is not clamped at the sequence start (the DP band, diagonals h-I .. h+I, then lies inside
+// the filter's 2W diagonals from h-d, d = I), and fscore < 255 (the filter saturates): the filter
+// is a local alignment over a superset of the DP's paths with gap costs never above the DP's
+// (one merged gap state, no indel cap) and no masked cells, so its maximum bounds every glocal
+// DP path's score.
+__device__ __forceinline__ bool dp_bounded_by_filter(const ReadsDev& r, const DPConst& k, int rs, int h) {
+  if (k.I > 15 || !(h - k.I > -k.d)) return false;
+  if (!r.has_qual) return true;
+  const int L = r.len[rs >> 1];
+  const uint32_t* qp = r.qual + (int64_t)rs * QW;
+  for (int w = 0; w * 16 < L; ++w) {
+    const int n = L - w * 16 < 16 ? L - w * 16 : 16;
+    const uint32_t m = n == 16 ? 0xffffffffu : ((1u << (2 * n)) - 1u);
+    if ((qp[w] & m) != m) return false;
+  }
+  return true;
+}
+
 // One candidate: read strand rs (2*read + strand), diagonal h.
 // TB = true: tbw is this lane's direction words (L rows of DPLane::DW words, interleaved with the
 // other lanes of the wavefront) and, when the record can be accepted, the traceback (alignment

@@ -71,8 +71,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 4 :
         int sa, sb;
         filter_pair<W>(a.g, read_planes(a.r, rsa), read_planes(a.r, rsb), La, Lb,
                        (int64_t)cdiag[c0] - a.sc.d + GPAD, (int64_t)cdiag[c1] - a.sc.d + GPAD, fk, sa, sb);
-        cpass[c0] = (uint8_t)(sa > fk.thr);
-        if (c1 != c0) cpass[c1] = (uint8_t)(sb > fk.thr);
+        // the u8 score of a pass (> thr >= 0, so non-zero), 0 for a rejected candidate
+        cpass[c0] = (uint8_t)(sa > fk.thr ? sa : 0);
+        if (c1 != c0) cpass[c1] = (uint8_t)(sb > fk.thr ? sb : 0);
       }
     }
     __syncthreads();
@@ -104,6 +105,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 4 :
       const int c = rec2cand[k];
       if (rovf[crs[c] >> 1]) continue;
       const int64_t rs = 2 * base + crs[c];
+      const int fs = cpass[c];
+      if (!COLOR && fs <= a.sc.t && fs < 255 && dp_bounded_by_filter(a.r, DPConst(a.sc), (int)rs, cdiag[c])) {
+        AlignRec x;   // DP score <= fs <= t: never accepted
+        x.score = MIN_SCORE;
+        x.S = MIN_SCORE;
+        x.win = 0;
+        x.best_j = x.best_i0 = x.best_j0 = -1;
+        x.tb_len = 0;
+        recs[k] = x;
+        continue;
+      }
       recs[k] = dp_one<IM, COLOR, false>(a.g, a.r, a.sc, (int)rs, cdiag[c], nullptr, nullptr);
     }
     __syncthreads();
