@@ -103,14 +103,16 @@ static SeedSetDev seed_set(const RefState& rs) {
 
 // Stable order of the view's reads by length (counting sort: lengths are 1..MAXL); identity
 // reports whether that order is the view's own.
+// Longest reads first (stable within a length): the persistent kernel hands out the costliest
+// tasks first and the short ones fill the tail, and equal lengths stay contiguous for the plain path.
 void length_order(const storm_reads_view* v, std::vector<int64_t>& order, bool& identity) {
   const int64_t n = v->n_reads;
-  auto bucket = [&](int64_t i) { return (size_t)std::min(std::max((int)v->len[i], 0), MAXL + 1); };
+  auto bucket = [&](int64_t i) { return (size_t)(MAXL + 1 - std::min(std::max((int)v->len[i], 0), MAXL + 1)); };
   std::vector<int64_t> start(MAXL + 3, 0);
   identity = true;
   for (int64_t i = 0; i < n; ++i) {
     start[bucket(i) + 1]++;
-    if (i && v->len[i] < v->len[i - 1]) identity = false;
+    if (i && v->len[i] > v->len[i - 1]) identity = false;
   }
   for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
   order.resize((size_t)n);

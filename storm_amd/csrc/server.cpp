@@ -228,18 +228,19 @@ static int run_batch(storm_server* s, storm_server::Lane& ln, const std::vector<
     }
     n += g.row1 - g.row0;
   }
-  // the batch's reads ordered by length (counting sort; lengths are 1..MAXL, checked at submit)
+  // the batch's reads ordered by length, longest first (counting sort; lengths are 1..MAXL,
+  // checked at submit): the costliest tasks of the persistent kernel go first
   std::vector<int64_t> start(MAXL + 2, 0);
   int maxL = 0;
   for (const Segment& g : segs)
-    for (int64_t row = g.row0; row < g.row1; ++row) start[(size_t)g.r->len[(size_t)row] + 1]++;
+    for (int64_t row = g.row0; row < g.row1; ++row) start[(size_t)(MAXL - g.r->len[(size_t)row]) + 1]++;
   for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
   ln.h_rows.resize((size_t)n);
   for (const Segment& g : segs) {
     const int32_t ri = (int32_t)(std::find(regs.begin(), regs.end(), g.r) - regs.begin());
     for (int64_t row = g.row0; row < g.row1; ++row) {
       const int L = g.r->len[(size_t)row];
-      ln.h_rows[(size_t)start[(size_t)L]++] = ServeRow{ri, (int32_t)row};
+      ln.h_rows[(size_t)start[(size_t)(MAXL - L)]++] = ServeRow{ri, (int32_t)row};
       maxL = std::max(maxL, L);
     }
   }
