@@ -750,7 +750,7 @@ int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback,
   FusedState& F = m->fs;
   ScoreDev sc{};
   fill_score(m->p, sc);
-  const int64_t stride = (int64_t)maxL * dp_dir_words(sc.I) * 4;
+  const int64_t stride = 2 * (int64_t)maxL * dp_dir_words(sc.I) * 4;   // two records per lane (I <= 7)
   const int bpc = fused_blocks_per_cu(sc.W, sc.I, m->p.color);
   if (bpc <= 0) return fail(STORM_ERR_DEVICE, "fused kernel: no occupancy for this band class");
   const int max_grid = device_cus(m->device) * bpc;

@@ -16,6 +16,7 @@
 // record buffer, is flagged in `overflow` and left untouched; the host sends it through the
 // staged path.
 #include "dev_align.h"
+#include "dev_align2.h"
 #include "dev_filter.h"
 #include "dev_seed.h"
 #include "storm_kernels.h"
@@ -30,8 +31,11 @@ constexpr int F_PENDING = -3;        // Slot.pass while the entry's traceback is
 static_assert(TBB % 16 == 0, "traceback records are copied in 16-byte units");
 static_assert(F_CCAP >= 2 * F_HCAP, "a flush must hold both strands of one read");
 
+// Three wavefronts per SIMD for the bands of the two-records-per-lane DP (up to 168 VGPRs: the
+// pair's columns, color states and both genome windows stay in registers; at four the compiler
+// spills them), two for the wide bands.
 template <int W, int IM, bool COLOR>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 4 : 2))) void k_fused(FusedArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 3 : 2))) void k_fused(FusedArgs a) {
   __shared__ int32_t hbuf[F_HCAP];
   __shared__ int32_t lk[192];
   __shared__ int32_t cstart[2 * F_GMAX + 1];
@@ -45,6 +49,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 4 :
   const int G = a.group;
   const int I = a.sc.I;
   const FilterConst fk = filter_const(a.sc.fmatch, a.sc.fmismatch, a.sc.fopen, a.sc.fext, a.sc.fthr);
+  // direction words of this lane, interleaved with the other lanes' (64 lanes x scratch_stride bytes
+  // per workgroup: two records per lane in the pair pass, one in the single-record pass)
   uint32_t* tbw = (uint32_t*)(a.scratch + (int64_t)blockIdx.x * 64 * a.scratch_stride) + lane;
   uint8_t* tbbuf = a.tbbuf + (int64_t)blockIdx.x * F_PCAP * TBB;
   uint8_t* wb = a.work + (int64_t)blockIdx.x * FUSED_WORK_BYTES;
@@ -101,22 +107,49 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 4 :
     // DP (stage d), score pass: lane l takes records l, l + 64, ...; nothing is stored per cell
     // and a record stops at the first row where it can no longer be accepted
     const int npl = np < F_PCAP ? np : F_PCAP;
-    for (int k = lane; k < npl; k += 64) {
+    // the record needs its score pass (0), is skipped (1: its filter score bounds its DP score at
+    // or below t, so it is written as never accepted), or belongs to an overflowing read (2)
+    auto triage = [&](int k, int& rs, int& h) -> int {
       const int c = rec2cand[k];
-      if (rovf[crs[c] >> 1]) continue;
-      const int64_t rs = 2 * base + crs[c];
+      if (rovf[crs[c] >> 1]) return 2;
+      rs = (int)(2 * base + crs[c]);
+      h = cdiag[c];
       const int fs = cpass[c];
-      if (!COLOR && fs <= a.sc.t && fs < 255 && dp_bounded_by_filter(a.r, DPConst(a.sc), (int)rs, cdiag[c])) {
-        AlignRec x;   // DP score <= fs <= t: never accepted
+      if (!COLOR && fs <= a.sc.t && fs < 255 && dp_bounded_by_filter(a.r, DPConst(a.sc), rs, h)) {
+        AlignRec x;
         x.score = MIN_SCORE;
         x.S = MIN_SCORE;
         x.win = 0;
         x.best_j = x.best_i0 = x.best_j0 = -1;
         x.tb_len = 0;
         recs[k] = x;
-        continue;
+        return 1;
       }
-      recs[k] = dp_one<IM, COLOR, false>(a.g, a.r, a.sc, (int)rs, cdiag[c], nullptr, nullptr);
+      return 0;
+    };
+    if constexpr (IM <= 7) {
+      // two records per lane, one per 16-bit half (records 2l, 2l+1, then 2l+128, ...), directions
+      // stored as they are computed and the traceback of a record that can be accepted walked at
+      // once, so no record is aligned twice
+      for (int k0 = 2 * lane; k0 < npl; k0 += 128) {
+        int rs0 = 0, h0 = 0, rs1 = 0, h1 = 0;
+        const bool run0 = triage(k0, rs0, h0) == 0;
+        const bool run1 = k0 + 1 < npl && triage(k0 + 1, rs1, h1) == 0;
+        if (run0 || run1) {   // a lone record runs in both halves; the copy writes nothing
+          AlignRec x0, x1;
+          const int kA = run0 ? k0 : k0 + 1;
+          dp_score_pair<IM, COLOR, true>(a.g, a.r, a.sc, run0 ? rs0 : rs1, run0 ? h0 : h1, run1 ? rs1 : rs0,
+                                         run1 ? h1 : h0, x0, x1, tbw, tbbuf + (int64_t)kA * TBB,
+                                         (run0 && run1) ? tbbuf + (int64_t)(k0 + 1) * TBB : nullptr);
+          if (run0) recs[k0] = x0;
+          if (run1) recs[k0 + 1] = run0 ? x1 : x0;
+        }
+      }
+    } else {
+      for (int k = lane; k < npl; k += 64) {
+        int rs = 0, h = 0;
+        if (triage(k, rs, h) == 0) recs[k] = dp_one<IM, COLOR, false>(a.g, a.r, a.sc, rs, h, nullptr, nullptr);
+      }
     }
     __syncthreads();
     // the records that can still be accepted (score > t, S >= t), ascending, as
@@ -137,8 +170,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 4 :
       nv += __shfl(incl, 63, 64);
     }
     __syncthreads();
-    // traceback pass over those records only: directions in the wavefront's scratch words, codes
-    // to tbbuf
+    // traceback pass over those records only (bands past 2IM+1 = 15 columns; the pair pass above
+    // has done it for narrower ones): directions in the wavefront's scratch words, codes to tbbuf
+    if constexpr (IM > 7)
     for (int e = lane; e < nv; e += 64) {
       const int k = hbuf[e] & 0xffff;
       const int c = rec2cand[k];

@@ -129,7 +129,7 @@ struct FusedArgs {
   uint8_t* cells;            // 4 * TBB bytes per read: the traceback cells
   uint8_t* overflow;         // per read
   uint8_t* scratch;          // per workgroup: 64 * scratch_stride bytes of direction words
-  int64_t scratch_stride;    // bytes per lane: maxL x dp_dir_words(I) x 4
+  int64_t scratch_stride;    // bytes per lane: 2 x maxL x dp_dir_words(I) x 4
   uint8_t* tbbuf;            // per workgroup: FUSED_PCAP * TBB traceback bytes of the current flush
   uint8_t* work;             // per workgroup: FUSED_WORK_BYTES of candidates and records
   int ref_id;
