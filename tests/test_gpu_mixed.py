@@ -41,7 +41,7 @@ def _write_subset(mode, reads, idx, path, qpath):
                 fq.write(b">" + nm.encode() + b"\n" + " ".join(str(int(x)) for x in q).encode() + b"\n")
 
 
-@pytest.mark.parametrize("mode,i", [("n", 3), ("c", 3), ("n", 7)])
+@pytest.mark.parametrize("mode,i", [("n", 3), ("c", 3), ("n", 7), ("c", 7)])
 def test_mixed_lengths_match_per_length_oracle(mode, i, tmp_path, oracle_built):
     from storm_amd import lib, synth
     g = synth.gc_window_genome(300_000, 5)
