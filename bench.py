@@ -38,6 +38,9 @@ VALU_INSTR_PER_CELL_W8_L75 = 5.447
 # candidates, the factors calibrated on a device-to-device copy of known size in the same run
 # (profiles/k_filter_W8_L75_10Mcand.md)
 HBM_BYTES_PER_CAND_W8_L75 = 45.8
+# algorithmic bytes per candidate: packed read (19 B at L = 75) + genome window (L + 2W - 1 = 90
+# codes, 23 B) + the u8 score and the pass byte
+ALG_BYTES_PER_CAND_W8_L75 = 44.0
 
 
 def log(*a):
@@ -268,6 +271,10 @@ def map_config(lib, name, gpath, pr, kw, color, n_sample_cpu, sam_reads, skip_pl
                         "gpu_ms": {k: round(sp["gpu_ms_" + k], 2) for k in ("seed", "lookup", "filter", "align",
                                                                             "replay")},
                         "index_lookup_GBps": round(lk_bytes / (sp["gpu_ms_lookup"] * 1e-3) / 1e9, 1)
+                        if sp["gpu_ms_lookup"] > 0 else None,
+                        # the same against the 8 TB/s HBM peak (8 B of first[] per (seed, strand, position)
+                        # lookup; random gathers, each touching its own cache line)
+                        "index_lookup_hbm_frac": round(lk_bytes / (sp["gpu_ms_lookup"] * 1e-3) / 8e12, 4)
                         if sp["gpu_ms_lookup"] > 0 else None}
         res["batched_over_plain"] = round(tp / tb, 2)
         res["plain_equals_batched"] = bool(same)
@@ -574,6 +581,11 @@ def main():
                      "unit": "T VALU lane-instr/s", "frac": round(achieved / VALU_PEAK_INT32, 4),
                      "traffic": int(HBM_BYTES_PER_CAND_W8_L75 * n_cand),
                      "work_per_cell": VALU_INSTR_PER_CELL_W8_L75,
+                     "hbm": {"achieved": round(ALG_BYTES_PER_CAND_W8_L75 * n_cand / (ms * 1e-3) / 1e9, 1),
+                             "peak": 8000.0, "unit": "GB/s",
+                             "frac": round(ALG_BYTES_PER_CAND_W8_L75 * n_cand / (ms * 1e-3) / 8e12, 4),
+                             "algorithmic_bytes_per_candidate": ALG_BYTES_PER_CAND_W8_L75,
+                             "measured_bytes_per_candidate": HBM_BYTES_PER_CAND_W8_L75},
                      "algorithmic": {"ops_per_cell": OPS_PER_CELL, "achieved_tops": round(algorithmic, 2),
                                      "peak_packed16_tops": VALU_PEAK_PK16,
                                      "frac": round(algorithmic / VALU_PEAK_PK16, 4)}},
