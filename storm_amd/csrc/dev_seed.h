@@ -129,40 +129,74 @@ __device__ int wave_candidates(const ReadsDev& r, const SeedSetDev& seeds, int64
   const int lane = threadIdx.x & 63;
   const int L = r.len[rs >> 1];
   const uint32_t* pl = read_planes(r, rs);
+  // lookups of one batch (seed s, positions k0 .. k0+63): position and the bucket bounds
+  // first[key], first[key+1]; issued one batch ahead, so their latency overlaps the batch before
+  auto lookup = [&](int s, int k0, int& rp, int& b0, int& b1) {
+    const SeedDev& sd = seeds.s[s];
+    const int k = k0 + lane;
+    rp = 0;
+    b0 = b1 = 0;
+    if (k < seed_npos(sd, L)) {
+      rp = sd.n_pos > 0 ? sd.pos[k] : k;
+      const int key = seed_key(sd, pl, rp);
+      b0 = sd.first[key];
+      b1 = sd.first[key + 1];
+    }
+  };
+  auto next_batch = [&](int& s, int& k0) {
+    k0 += 64;
+    while (s < seeds.n && k0 >= seed_npos(seeds.s[s], L)) { ++s; k0 = 0; }
+  };
   int run = 0;
   bool over = false;
-  for (int s = 0; s < seeds.n; ++s) {
+  int s = 0, k0 = -64;
+  next_batch(s, k0);
+  int rp = 0, b0 = 0, b1 = 0;
+  if (s < seeds.n) lookup(s, k0, rp, b0, b1);
+  while (s < seeds.n) {
+    int s2 = s, k2 = k0;
+    next_batch(s2, k2);
+    int rp2 = 0, b02 = 0, b12 = 0;
+    if (s2 < seeds.n) lookup(s2, k2, rp2, b02, b12);
     const SeedDev& sd = seeds.s[s];
-    const int np = seed_npos(sd, L);
-    for (int k0 = 0; k0 < np; k0 += 64) {
-      const int k = k0 + lane;
-      int rp = 0, f0 = 0, c = 0;
-      if (k < np) {
-        rp = sd.n_pos > 0 ? sd.pos[k] : k;
-        const int key = seed_key(sd, pl, rp);
-        f0 = sd.first[key];
-        c = sd.first[key + 1] - f0;
-      }
-      const int incl = wave_scan_incl_i32(c);
-      const int tot = __shfl(incl, 63, 64);
-      if (!over && run + tot <= hcap) {
-        lk[lane] = f0 - (incl - c);   // bucket entry of batch hit e is lk[q] + e
-        lk[64 + lane] = rp;
-        lk[128 + lane] = incl;
-        __syncthreads();
-        for (int e = lane; e < tot; e += 64) {
-          int q = 0;   // smallest lookup q with incl[q] > e
+    const int c = b1 - b0;
+    const int incl = wave_scan_incl_i32(c);
+    const int tot = __shfl(incl, 63, 64);
+    if (!over && run + tot <= hcap) {
+      lk[lane] = b0 - (incl - c);   // bucket entry of batch hit e is lk[q] + e
+      lk[64 + lane] = rp;
+      lk[128 + lane] = incl;
+      __syncthreads();
+      // hit e of the batch: lookup q = the first with incl[q] > e; four hits per lane at a time,
+      // so four bucket reads are in flight together
+      for (int e0 = lane; e0 < tot; e0 += 256) {
+        int q[4], v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int e = e0 + 64 * u;
+          int qq = 0;
 #pragma unroll
           for (int step = 32; step > 0; step >>= 1)
-            if (lk[128 + q + step - 1] <= e) q += step;
-          hbuf[run + e] = sd.map[lk[q] + e] - lk[64 + q];
+            if (lk[128 + qq + step - 1] <= e) qq += step;
+          q[u] = qq;
+          v[u] = e < tot ? sd.map[lk[qq] + e] : 0;
         }
-        __syncthreads();
-      } else {
-        over = true;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int e = e0 + 64 * u;
+          if (e < tot) hbuf[run + e] = v[u] - lk[64 + q[u]];
+        }
       }
-      run += tot;
+      __syncthreads();
+    } else {
+      over = true;
     }
+    run += tot;
+    s = s2;
+    k0 = k2;
+    rp = rp2;
+    b0 = b02;
+    b1 = b12;
   }
   *raw_hits = run;
   if (over) return -1;
