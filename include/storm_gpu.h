@@ -104,6 +104,51 @@ int storm_filter_fetch(storm_filter_plan *plan, uint8_t *scores, uint8_t *pass);
 void storm_filter_plan_destroy(storm_filter_plan *plan);
 
 /* ------------------------------------------------------------------------------------------
+ * Batch DP + traceback (stage d) over (read strand, diagonal) records, inputs resident in HBM.
+ * Replaces alignment__align + alignment__traceback (alignment.c:488-734, declared
+ * alignment.h:280-289) as process_read calls them for each filter pass (run.c:665-690,
+ * :720-735), for a whole batch.
+ *   genome_packed : one sequence, 2-bit codes (code n at bits 2*(n&3) of byte n>>2), genome_codes
+ *   genome_mask   : genome_codes bytes, nonzero = masked (ReferenceDBType.mask), or NULL
+ *   reads_packed  : n_reads forward-strand reads, read_stride bytes apart, lengths read_len (1..256)
+ *   qual          : 2-bit quality levels (0..3) per read packed like the codes, qual_stride bytes
+ *                   apart, or NULL (every level 3, as for reads without quality)
+ *   first_qual    : color space: per read, the level of the primer-side color, or NULL (0)
+ *   rec_rs        : per record, 2 * read + strand; strand 1 is the reverse complement (color:
+ *                   the reversed colors), codes.c:429-439, read_quality.c:47-64
+ *   rec_diag      : per record, the diagonal h (read code 0 against genome code h); the DP
+ *                   window starts at win = max(h - i, -min(i, 15)) (run.c:667)
+ *   params        : color, match, mismatch, gap_open, gap_extend, allowed_indels (-i), t
+ * storm_align_run runs the batch `reps` times on the resident data (mean time per run in ms,
+ * HIP events); storm_align_fetch copies per record a storm_align_result and STORM_TB_BYTES
+ * traceback bytes:
+ *   score  : alignment__align's result with min_accepted = t: MIN_SCORE (-16383) when some row's
+ *            max + best possible score drops below t (alignment.c:613-616), else the leftmost
+ *            maximum of the last row
+ *   bound  : the minimum over rows >= 1 of that row bound (up to the row that stopped the DP);
+ *            for a running threshold T >= t, alignment__align(T) returns score when bound >= T
+ *            and MIN_SCORE otherwise
+ *   win, best_j : window start; the last row's column j of the maximum (best_i = L - 1), -1 if none
+ *   best_i0, best_j0, tb_len, tb : when score > t, where the traceback ends (row 0) and the
+ *            traceback in alignment__traceback's order and packing (TRACEBACK_SEQ_SET,
+ *            alignment.h:81-83, end of the alignment first); else -1, -1, 0
+ */
+#define STORM_TB_BYTES 144
+typedef struct storm_align_result {
+  int32_t score, bound, win;
+  int16_t best_j, best_i0, best_j0, tb_len;
+} storm_align_result;
+typedef struct storm_align_plan storm_align_plan;
+int storm_align_plan_create(int device, const uint8_t *genome_packed, const uint8_t *genome_mask, int64_t genome_codes,
+                            const uint8_t *reads_packed, int64_t read_stride, const uint8_t *qual, int64_t qual_stride,
+                            const uint8_t *first_qual, const int32_t *read_len, int64_t n_reads, const int32_t *rec_rs,
+                            const int32_t *rec_diag, int64_t n_rec, const storm_params *params,
+                            storm_align_plan **out);
+int storm_align_run(storm_align_plan *plan, int reps, double *ms_per_run);
+int storm_align_fetch(storm_align_plan *plan, storm_align_result *results, uint8_t *tb);
+void storm_align_plan_destroy(storm_align_plan *plan);
+
+/* ------------------------------------------------------------------------------------------
  * Seed index of one reference sequence, built on the device: index__build_reference
  * (index.c:19-195, declared index.h:49-62), with the contiguous layout of index.c:157-188.
  * packed:   the sequence as ReferenceDBType.sequence holds it (reference_data.h:25-53): 2-bit

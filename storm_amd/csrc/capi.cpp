@@ -198,6 +198,32 @@ static int hits_to_host(storm_mapper* m, const Slot* slots, const std::vector<in
   return STORM_OK;
 }
 
+// records per k_align launch: whole wavefronts (the direction words of a wavefront's 64 records
+// are interleaved) within a 256 MiB direction scratch
+static int64_t align_chunk(int64_t scratch_stride) {
+  const int64_t budget = 256LL << 20;
+  return std::max<int64_t>(64, (budget / std::max<int64_t>(scratch_stride, 1)) & ~63LL);
+}
+
+int64_t align_scratch_bytes(int I, int maxL, int64_t n_rec) {
+  const int64_t stride = (int64_t)maxL * dp_dir_words(I) * 4;
+  const int64_t chunk_alloc = (std::min(align_chunk(stride), std::max<int64_t>(n_rec, 1)) + 63) & ~63LL;
+  return chunk_alloc * stride;
+}
+
+int align_records(AlignArgs aa, int maxL, int64_t n_rec, DevMem& scratch, hipStream_t st) {
+  aa.scratch_stride = (int64_t)maxL * dp_dir_words(aa.sc.I) * 4;
+  const int64_t chunk = align_chunk(aa.scratch_stride);
+  HIPOK(scratch.ensure((size_t)align_scratch_bytes(aa.sc.I, maxL, n_rec)));
+  aa.scratch = scratch.as<uint8_t>();
+  for (int64_t b0 = 0; b0 < n_rec; b0 += chunk) {
+    aa.base = b0;
+    aa.n = std::min(chunk, n_rec - b0);
+    HIPOK(launch_align(aa, st));
+  }
+  return STORM_OK;
+}
+
 int prep_device_reads(storm_mapper* m, int64_t n, int64_t code_stride, int64_t qual_stride, bool has_qual,
                       bool has_first_qual, DevReads& d) {
   d.n = n;
@@ -421,17 +447,9 @@ int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int pass_id
     HIPOK(hipStreamSynchronize(st));
     for (int32_t x : lens) maxL = std::max(maxL, (int)x);
   }
-  aa.scratch_stride = (int64_t)maxL * dp_dir_words(sc.I) * 4;
-  const int64_t budget = 256LL << 20;
-  // chunks of whole wavefronts: the direction words of a wavefront's 64 records are interleaved
-  int64_t chunk = std::max<int64_t>(64, (budget / std::max<int64_t>(aa.scratch_stride, 1)) & ~63LL);
-  const int64_t chunk_alloc = (std::min(chunk, std::max<int64_t>(n_rec, 1)) + 63) & ~63LL;
-  HIPOK(P.align_scratch.ensure((size_t)(chunk_alloc * aa.scratch_stride)));
-  aa.scratch = P.align_scratch.as<uint8_t>();
-  for (int64_t b0 = 0; b0 < n_rec; b0 += chunk) {
-    aa.base = b0;
-    aa.n = std::min(chunk, n_rec - b0);
-    HIPOK(launch_align(aa, st));
+  {
+    const int rc = align_records(aa, maxL, n_rec, P.align_scratch, st);
+    if (rc) return rc;
   }
   HIPOK(hipEventRecord(e[3], st));
   // hit map

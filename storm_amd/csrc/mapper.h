@@ -226,6 +226,11 @@ int load_resident(const char* genome_path, const storm_params& p, std::shared_pt
 int ensure_ref_resident(storm_mapper* m, size_t r);
 
 // Packed reads already in d.codes / d.qual / d.first_qual / d.len (device): builds the strand planes.
+// DP + traceback (k_align) of records [0, n_rec) of `aa`, maxL = the longest read among them: launches
+// over chunks of whole wavefronts sized to a 256 MiB direction scratch, grown in `scratch`
+int align_records(AlignArgs aa, int maxL, int64_t n_rec, DevMem& scratch, hipStream_t st);
+// bytes of direction scratch align_records needs for n_rec records of reads up to maxL at -i I
+int64_t align_scratch_bytes(int I, int maxL, int64_t n_rec);
 int prep_device_reads(storm_mapper* m, int64_t n, int64_t code_stride, int64_t qual_stride, bool has_qual,
                       bool has_first_qual, DevReads& d);
 int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order, DevReads& d);

@@ -91,6 +91,11 @@ def load() -> C.CDLL:
     lib.storm_filter_run.argtypes = [P, I, C.POINTER(D)]
     lib.storm_filter_fetch.argtypes = [P, P, P]
     lib.storm_filter_plan_destroy.argtypes = [P]
+    lib.storm_align_plan_create.argtypes = [I, P, P, L, P, L, P, L, P, P, L, P, P, L, C.POINTER(Params),
+                                            C.POINTER(P)]
+    lib.storm_align_run.argtypes = [P, I, C.POINTER(D)]
+    lib.storm_align_fetch.argtypes = [P, P, P]
+    lib.storm_align_plan_destroy.argtypes = [P]
     lib.storm_map_files.argtypes = [I, C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(Params), I, C.c_char_p, I,
                                     P, C.POINTER(MapStats)]
     lib.storm_mapper_create.argtypes = [I, C.c_char_p, C.POINTER(Params), C.POINTER(P)]
@@ -178,6 +183,70 @@ class FilterPlan:
     def close(self):
         if self.h:
             load().storm_filter_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+ALIGN_RESULT = np.dtype([("score", "<i4"), ("bound", "<i4"), ("win", "<i4"), ("best_j", "<i2"),
+                         ("best_i0", "<i2"), ("best_j0", "<i2"), ("tb_len", "<i2")])
+TB_BYTES = 144
+MIN_SCORE = -16383
+
+
+class AlignPlan:
+    """Batch DP + traceback over (read strand, diagonal) records with all inputs resident in HBM
+    (storm_align_plan_*).  reads_codes: per read its forward codes; qual: per read its levels
+    0..3, or None (every level 3); first_qual: color space, per read the primer-side level."""
+
+    def __init__(self, genome_codes: np.ndarray, genome_mask, reads_codes, qual, read_len: np.ndarray,
+                 rec_rs: np.ndarray, rec_diag: np.ndarray, p: Params, first_qual=None, device=0):
+        from .synth import pack_codes
+        lib = load()
+        g = pack_codes(np.ascontiguousarray(genome_codes, dtype=np.uint8))
+        gm = None if genome_mask is None else np.ascontiguousarray(genome_mask, dtype=np.uint8)
+        read_len = np.ascontiguousarray(read_len, dtype=np.int32)
+        n = len(read_len)
+        stride = (int(read_len.max()) + 3) // 4 if n else 1
+        rp = np.zeros(n * stride + 16, np.uint8)
+        qp = None if qual is None else np.zeros(n * stride + 16, np.uint8)
+        for k in range(n):
+            nb = (int(read_len[k]) + 3) // 4
+            rp[k * stride:k * stride + nb] = pack_codes(np.asarray(reads_codes[k], np.uint8))
+            if qp is not None:
+                qp[k * stride:k * stride + nb] = pack_codes(np.asarray(qual[k], np.uint8))
+        fq = None if first_qual is None else np.ascontiguousarray(first_qual, dtype=np.uint8)
+        rec_rs = np.ascontiguousarray(rec_rs, dtype=np.int32)
+        rec_diag = np.ascontiguousarray(rec_diag, dtype=np.int32)
+        self._keep = (g, gm, rp, qp, fq, read_len, rec_rs, rec_diag)
+        self.n = len(rec_rs)
+        h = C.c_void_p()
+        nul = C.c_void_p()
+        check(lib.storm_align_plan_create(device, _ptr(g), nul if gm is None else _ptr(gm), len(genome_codes),
+                                          _ptr(rp), stride, nul if qp is None else _ptr(qp), stride,
+                                          nul if fq is None else _ptr(fq), _ptr(read_len), n, _ptr(rec_rs),
+                                          _ptr(rec_diag), self.n, C.byref(p), C.byref(h)))
+        self.h = h
+
+    def run(self, reps: int = 1) -> float:
+        ms = C.c_double()
+        check(load().storm_align_run(self.h, reps, C.byref(ms)))
+        return ms.value
+
+    def fetch(self):
+        """(results, tb): the structured storm_align_result array and n x TB_BYTES traceback bytes."""
+        res = np.zeros(self.n, ALIGN_RESULT)
+        tb = np.zeros((self.n, TB_BYTES), np.uint8)
+        check(load().storm_align_fetch(self.h, _ptr(res), _ptr(tb)))
+        return res, tb
+
+    def close(self):
+        if self.h:
+            load().storm_align_plan_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -63,3 +63,28 @@ def test_index_build_argument_errors():
         lib.build_index(packed, 400, [], [], "##-x")
     with pytest.raises(lib.StormError, match="exactly one seed"):
         lib.build_index(packed, 400, [], [], "##-#;#-#")
+
+
+def test_align_plan_argument_errors():
+    """storm_align_plan_create rejects bad reads, records and options before any device call."""
+    import numpy as np
+    import pytest
+    from storm_amd import lib
+    g = np.zeros(2000, np.uint8)
+    reads = [np.zeros(50, np.uint8), np.zeros(60, np.uint8)]
+    lens = np.array([50, 60], np.int32)
+    ok_rs, ok_h = np.array([0, 3], np.int32), np.array([10, 500], np.int32)
+    cases = [
+        (dict(rs=np.array([4], np.int32), h=np.array([10], np.int32)), "read strand"),
+        (dict(rs=np.array([-1], np.int32), h=np.array([10], np.int32)), "read strand"),
+        (dict(rs=np.array([1], np.int32), h=np.array([2000 + 1024 - 100], np.int32)), "genome end"),
+        (dict(p=lib.params(0, allowed_indels=32)), "-i"),
+        (dict(p=lib.params(0, mismatch=3)), "scores"),
+        (dict(lens=np.array([50, 257], np.int32), reads=[np.zeros(50, np.uint8), np.zeros(257, np.uint8)]),
+         "length"),
+        (dict(lens=np.array([0, 60], np.int32), reads=[np.zeros(0, np.uint8), np.zeros(60, np.uint8)]), "length"),
+    ]
+    for kw, msg in cases:
+        with pytest.raises(lib.StormError, match=msg):
+            lib.AlignPlan(g, None, kw.get("reads", reads), None, kw.get("lens", lens), kw.get("rs", ok_rs),
+                          kw.get("h", ok_h), kw.get("p", lib.params(0)))
