@@ -47,12 +47,15 @@ typedef struct storm_params {
   int qual_base;          /* -b, default 33 */
   int stdev_distance;     /* -l, repeat filter, default 100; <= 0 disables it */
   int sense_mask;         /* 1 forward (-F), 2 reverse (-R), 3 both (default) */
-  int max_hits;           /* -M, 1..4 */
+  int max_hits;           /* -M, 1..4; 0 = the ordered mode (no -M, the reference's default):
+                           * reads placed on a genome map, coordinate-sorted SAM (genome_map.c) */
   int uniqueness;         /* -u threshold, 0 = off */
   int print_all;          /* -A */
   int ignore_lowercase;   /* -j */
   int strict_length;      /* 1: reject mixed read lengths like the reference (reads_data.c:156-163) */
   const char *seeds;      /* -s, ';'-separated spaced seeds; NULL = reference defaults (seed.h:29-32) */
+  int greedy;             /* -G: ordered mode places each read at its best hit (genome_map.c:186-232)
+                           * instead of the contextual placement (genome_map.c:432-536) */
 } storm_params;
 
 void storm_params_default(storm_params *p, int color);

@@ -593,34 +593,85 @@ static int row_reportable(const orc_hit_t *row, int rank, const orc_params_t *p)
           (rank + 1 == M));
 }
 
-/* hit_map.c:197-504 */
-void orc_write_sam(FILE *out, const orc_result_t *res, const orc_reads_t *db, const orc_ref_t *refs, int n_refs,
-                   const orc_params_t *p, int argc, char **argv) {
-  static const char BASES[4] = {'A', 'C', 'G', 'T'};
-  static const short UB[4] = {2, 7, 10, 93};
-  const int L = db->read_len;
-  uint8_t **sym = NULL;
-  if (p->color) {   /* hit_map.c:236-249: base-space symbols of each color reference */
-    sym = xcalloc((size_t)n_refs, sizeof(uint8_t *));
-    for (int r = 0; r < n_refs; ++r) {
-      sym[r] = xcalloc((size_t)clen((long)refs[r].size + 1) + 1, 1);
-      uint8_t s = refs[r].first_base;
-      long k;
-      for (k = 0; k < refs[r].size; ++k) { set_nth(sym[r], k, s); s = (uint8_t)(s ^ orc_nth(refs[r].seq, k)); }
-      set_nth(sym[r], k, s);
-    }
+/* MAPQ of rank k of a hit map row (hit_map.c:374-399, genome_map.c:700-724) */
+static int orc_mapq(const orc_hit_t *row, int k, const orc_params_t *p) {
+  const double lg = log(.3);
+  const int delta_m = p->match - p->mismatch;   /* main.c:435 */
+  int s1 = row[0].score;
+  int s2 = (k == 0) ? (row[1].score > p->t ? row[1].score : p->t) : row[k].score;
+  double n = (k == 0) ? (double)(s1 - s2) / (double)delta_m : (double)(s2 - s1) / (double)delta_m;
+  int mq = -4.343 * (n * lg);
+  mq = (int)(mq - 4.343 * lg);
+  mq = mq >= 0 ? mq : 0;
+  return mq < 254 ? mq : 254;
+}
+
+/* base-space symbol before each position of a color reference (hit_map.c:236-249; the ordered
+ * writer's running ref_symbol, genome_map.c:679,789 is the same sequence) */
+static uint8_t **orc_ref_symbols(const orc_ref_t *refs, int n_refs) {
+  uint8_t **sym = xcalloc((size_t)n_refs, sizeof(uint8_t *));
+  for (int r = 0; r < n_refs; ++r) {
+    sym[r] = xcalloc((size_t)clen((long)refs[r].size + 1) + 1, 1);
+    uint8_t s = refs[r].first_base;
+    long k;
+    for (k = 0; k < refs[r].size; ++k) { set_nth(sym[r], k, s); s = (uint8_t)(s ^ orc_nth(refs[r].seq, k)); }
+    set_nth(sym[r], k, s);
   }
-  fprintf(out, "@HD\tVN:1.4\tSO:unsorted\n");
+  return sym;
+}
+
+static void orc_sam_header(FILE *out, const char *sort_order, const orc_reads_t *db, const orc_ref_t *refs,
+                           int n_refs, const orc_params_t *p, int argc, char **argv) {
+  fprintf(out, "@HD\tVN:1.4\tSO:%s\n", sort_order);
   for (int r = 0; r < n_refs; ++r)
     fprintf(out, "@SQ\tSN:%s\tLN:%ld\n", refs[r].name ? refs[r].name : "*", (long)(refs[r].size + (p->color ? 1 : 0)));
   fprintf(out, "@RG\tID:%s\tSM:%s\n", db->name ? db->name : "*", db->name ? db->name : "*");
   fprintf(out, "@PG\tID:%s\tVN:%s\tCL:\"", "SToRM", "0.0101");
   for (int a = 0; a < argc; ++a) fprintf(out, "%s ", argv[a]);
   fprintf(out, "\"\n");
+}
+
+/* the fields of one mapped record from FLAG to QUAL (hit_map.c:401-448, genome_map.c:727-771) */
+static void orc_sam_record(FILE *out, const orc_read_t *rd, const orc_hit_t *h, int flag, int aq, const orc_ref_t *refs,
+                           uint8_t **sym, const orc_params_t *p, int L, char *cigar, char *seq) {
+  static const short UB[4] = {2, 7, 10, 93};
+  int rid = h->ref_id;
+  long rs = h->ref_start;
+  int ed = orc_traceback_to_cigar(h->traceback, h->traceback_len, refs[rid].seq, rs, p->color,
+                                  p->color ? (uint8_t)orc_nth(sym[rid], rs) : 0, cigar, seq);
+  fprintf(out, "%s\t%d\t%s\t%ld\t%d\t%s\t", rd->info, flag, refs[rid].name ? refs[rid].name : "*", rs + 1, aq, cigar);
+  fprintf(out, "*\t0\t0\t");
+  fprintf(out, "%s\t", seq);
+  if (rd->qual) {
+    if (h->sense & ORC_SENSE_REV) {
+      for (int q = L - 1; q >= 0; --q) {
+        int v = p->qual_base + UB[orc_nth(rd->qual, q)];
+        fputc(v < 126 ? v : 126, out);
+      }
+      if (p->color) { int v = p->qual_base + UB[rd->first_qual]; fputc(v < 126 ? v : 126, out); }
+    } else {
+      if (p->color) { int v = p->qual_base + UB[rd->first_qual]; fputc(v < 126 ? v : 126, out); }
+      for (int q = 0; q < L; ++q) {
+        int v = p->qual_base + UB[orc_nth(rd->qual, q)];
+        fputc(v < 126 ? v : 126, out);
+      }
+    }
+  } else {
+    fputc('*', out);
+  }
+  fprintf(out, "\tAS:i:%d\tNM:i:%d", h->score, ed);
+}
+
+/* hit_map.c:197-504 */
+void orc_write_sam(FILE *out, const orc_result_t *res, const orc_reads_t *db, const orc_ref_t *refs, int n_refs,
+                   const orc_params_t *p, int argc, char **argv) {
+  static const char BASES[4] = {'A', 'C', 'G', 'T'};
+  static const short UB[4] = {2, 7, 10, 93};
+  const int L = db->read_len;
+  uint8_t **sym = p->color ? orc_ref_symbols(refs, n_refs) : NULL;
+  orc_sam_header(out, "unsorted", db, refs, n_refs, p, argc, argv);
   char *cigar = xmalloc((size_t)L * 4 + 64);
   char *seq = xmalloc((size_t)L + 64);
-  const double lg = log(.3);
-  const int delta_m = p->match - p->mismatch;
   for (long id = 0; id < res->n_reads; ++id) {
     const orc_hit_t *row = res->map[id];
     const orc_read_t *rd = &db->reads[id];
@@ -631,42 +682,8 @@ void orc_write_sam(FILE *out, const orc_result_t *res, const orc_reads_t *db, co
         int flag = 0;
         if (row[k].sense & ORC_SENSE_REV) flag |= 0x10;
         if (k) flag |= 0x100;
-        int aq;
-        {
-          int s1 = row[0].score;
-          int s2 = (k == 0) ? (row[1].score > p->t ? row[1].score : p->t) : row[k].score;
-          double n = (k == 0) ? (double)(s1 - s2) / (double)delta_m : (double)(s2 - s1) / (double)delta_m;
-          int mq = -4.343 * (n * lg);
-          mq = (int)(mq - 4.343 * lg);
-          mq = mq >= 0 ? mq : 0;
-          mq = mq < 254 ? mq : 254;
-          aq = mq;
-        }
-        int rid = row[k].ref_id;
-        long rs = row[k].ref_start;
-        int ed = orc_traceback_to_cigar(row[k].traceback, row[k].traceback_len, refs[rid].seq, rs, p->color,
-                                        p->color ? (uint8_t)orc_nth(sym[rid], rs) : 0, cigar, seq);
-        fprintf(out, "%s\t%d\t%s\t%ld\t%d\t%s\t", rd->info, flag, refs[rid].name ? refs[rid].name : "*", rs + 1, aq, cigar);
-        fprintf(out, "*\t0\t0\t");
-        fprintf(out, "%s\t", seq);
-        if (rd->qual) {
-          if (row[k].sense & ORC_SENSE_REV) {
-            for (int q = L - 1; q >= 0; --q) {
-              int v = p->qual_base + UB[orc_nth(rd->qual, q)];
-              fputc(v < 126 ? v : 126, out);
-            }
-            if (p->color) { int v = p->qual_base + UB[rd->first_qual]; fputc(v < 126 ? v : 126, out); }
-          } else {
-            if (p->color) { int v = p->qual_base + UB[rd->first_qual]; fputc(v < 126 ? v : 126, out); }
-            for (int q = 0; q < L; ++q) {
-              int v = p->qual_base + UB[orc_nth(rd->qual, q)];
-              fputc(v < 126 ? v : 126, out);
-            }
-          }
-        } else {
-          fputc('*', out);
-        }
-        fprintf(out, "\tAS:i:%d\tNM:i:%d\tNH:i:%d\n", row[k].score, ed, nrep);
+        orc_sam_record(out, rd, &row[k], flag, orc_mapq(row, k, p), refs, sym, p, L, cigar, seq);
+        fprintf(out, "\tNH:i:%d\n", nrep);
       }
     } else if (p->print_all) {
       if (p->color) {
@@ -682,6 +699,208 @@ void orc_write_sam(FILE *out, const orc_result_t *res, const orc_reads_t *db, co
       }
     }
   }
+  fflush(out);
+  free(cigar); free(seq);
+  if (sym) { for (int r = 0; r < n_refs; ++r) free(sym[r]); free(sym); }
+}
+
+/* ------------------------------------------------------------ ordered mode (genome_map.c) */
+
+#define GM_NO_CODE 4   /* GENOME_MAP_NO_CODE = CODE_COUNT (genome_map.c:26) */
+
+/* per mapped position (GenomeMapPositionType, genome_map.h:31-48, COMPRESSED_COUNTERS): the reads
+ * placed here in placement order, and 6-bit counters of codes 0..3 and the gap (4) packed in an
+ * unsigned, for the aligned code and for each insertion offset 0..I-1 */
+typedef struct {
+  int *read, *rank;
+  int n;
+  unsigned aligned;
+  unsigned *inserted;
+} orc_gpos_t;
+
+/* COUNTER_GET / COUNTER_INC (genome_map.c:38-47): the increment wraps at 64 */
+static unsigned gm_get(unsigned c, int code) { return (c >> (code * 6)) & 0x3Fu; }
+static void gm_inc(unsigned *c, int code) {
+  unsigned u = (gm_get(*c, code) + 1) & 0x3Fu;
+  *c = (*c & ~(0x3Fu << (code * 6))) | (u << (code * 6));
+}
+
+/* genome_map__create_position_data (genome_map.c:68-99); positions 0..size exist (GENOME_MAP_SIZE) */
+static orc_gpos_t *gm_at(orc_gpos_t **g, long size, long pos, int indels, int greedy) {
+  if (pos < 0 || pos > size) {
+    fprintf(stderr, "oracle: genome map position %ld outside 0..%ld\n", pos, size);
+    exit(2);
+  }
+  if (!g[pos]) {
+    g[pos] = xcalloc(1, sizeof(orc_gpos_t));
+    if (!greedy && indels) g[pos]->inserted = xcalloc((size_t)indels, sizeof(unsigned));
+  }
+  return g[pos];
+}
+
+/* genome_map__set_read_idx_at (genome_map.c:123-138) */
+static void gm_place(orc_gpos_t **g, long size, long pos, int read, int rank, int indels, int greedy) {
+  orc_gpos_t *x = gm_at(g, size, pos, indels, greedy);
+  x->read = realloc(x->read, sizeof(int) * (size_t)(x->n + 1));
+  x->rank = realloc(x->rank, sizeof(int) * (size_t)(x->n + 1));
+  x->read[x->n] = read;
+  x->rank[x->n] = rank;
+  x->n++;
+}
+
+/* genome_map__get_direct_consensus_code (genome_map.c:308-330) */
+static int gm_consensus(orc_gpos_t **g, long size, long pos, int offset, int reference_code) {
+  if (pos < 0 || pos > size) {
+    fprintf(stderr, "oracle: genome map position %ld outside 0..%ld\n", pos, size);
+    exit(2);
+  }
+  if (!g[pos]) return reference_code;
+  unsigned c = offset == -1 ? g[pos]->aligned : g[pos]->inserted[offset];
+  int best = reference_code;
+  for (int i = 0; i <= GM_NO_CODE; ++i)
+    if (gm_get(c, i) > gm_get(c, best)) best = i;
+  if (gm_get(c, best) == 0) best = GM_NO_CODE;
+  return best;
+}
+
+/* genome_map__update_consensus_code (genome_map.c:340-349) */
+static void gm_count(orc_gpos_t **g, long size, long pos, int offset, int code, int indels) {
+  orc_gpos_t *x = gm_at(g, size, pos, indels, 0);
+  if (offset >= 0) gm_inc(&x->inserted[offset], code);
+  else gm_inc(&x->aligned, code);
+}
+
+static int tb_fwd(const uint8_t *tb, int i) { return (tb[i >> 1] >> ((i & 1) ? 0 : 4)) & 0xF; }   /* alignment.h:82 */
+
+/* GET_PAIR_TYPE (alignment.h:87-94) */
+static int pair_type(int cell, int color) {
+  if ((cell & 0xC) == 0x4) return 0x4;
+  if (color && (cell & 0xC) == 0x8) return 0x8;
+  if ((cell & 0xC) == 0xC) return 0xC;
+  return cell;
+}
+
+/* genome_map__compute_adjusted_score (genome_map.c:359-425) */
+static int gm_adjusted(orc_gpos_t **g, const orc_hit_t *h, const orc_ref_t *ref, const orc_params_t *p) {
+  if (h->ref_start == ORC_UNMAPPED) return ORC_MIN_SCORE;
+  int score = 0, indel = 0;
+  long pos = h->ref_start;
+  for (int i = 0; i < h->traceback_len; ++i) {
+    int cell = tb_fwd(h->traceback, i);
+    switch (pair_type(cell, p->color)) {
+      case 0x3: {
+        int rc = orc_nth(ref->seq, pos);
+        score += (rc == gm_consensus(g, ref->size, pos, -1, rc)) ? p->match : p->mismatch;
+        ++pos; indel = 0;
+      } break;
+      case 0x1:
+        score += (gm_consensus(g, ref->size, pos, -1, GM_NO_CODE) == GM_NO_CODE) ? p->match : p->mismatch;
+        ++pos; indel = 0;
+        break;
+      case 0x4: case 0x8: {
+        int c = cell & 3;
+        score += (c == gm_consensus(g, ref->size, pos, -1, c)) ? p->match : p->mismatch;
+        ++pos; indel = 0;
+      } break;
+      case 0xC: {
+        int c = cell & 3;
+        score += (c == gm_consensus(g, ref->size, pos, indel, c)) ? p->match : p->mismatch;
+        ++indel;
+      } break;
+      default: break;
+    }
+  }
+  return score;
+}
+
+/* genome_map__radix_sort_reads (genome_map.c:252-298): LSD radix over the 8 hex digits of the
+ * int score, each digit's bucket reversed, so the order is descending in the unsigned 32-bit
+ * value of the score, ties in the incoming order */
+static void gm_sort(int *ids, long n, orc_hit_t **map) {
+  int *tmp = xmalloc(sizeof(int) * (size_t)(n ? n : 1));
+  for (int d = 0; d < 8; ++d) {
+    long count[16] = {0}, index[16];
+    for (long j = 0; j < n; ++j) count[15 - (((unsigned)map[ids[j]][0].score >> (4 * d)) & 15u)]++;
+    index[0] = 0;
+    for (int b = 1; b < 16; ++b) index[b] = index[b - 1] + count[b - 1];
+    for (long j = 0; j < n; ++j) tmp[index[15 - (((unsigned)map[ids[j]][0].score >> (4 * d)) & 15u)]++] = ids[j];
+    memcpy(ids, tmp, sizeof(int) * (size_t)n);
+  }
+  free(tmp);
+}
+
+void orc_write_sam_ordered(FILE *out, const orc_result_t *res, const orc_reads_t *db, const orc_ref_t *refs,
+                           int n_refs, const orc_params_t *p, int argc, char **argv) {
+  const int I = p->allowed_indels;   /* hit_map__create(size, allowed_indels), run.c:818 */
+  orc_gpos_t ***g = xcalloc((size_t)n_refs, sizeof(orc_gpos_t **));
+  for (int r = 0; r < n_refs; ++r) g[r] = xcalloc((size_t)refs[r].size + 1, sizeof(orc_gpos_t *));
+  if (p->greedy) {   /* genome_map__build_greedy / __map_read_greedy (genome_map.c:186-232) */
+    for (long id = 0; id < res->n_reads; ++id) {
+      const orc_hit_t *row = res->map[id];
+      if (!row || row[0].ref_start == ORC_UNMAPPED) continue;
+      if (p->uniqueness == 0 || row[0].score - row[1].score >= p->uniqueness_threshold)
+        gm_place(g[row[0].ref_id], refs[row[0].ref_id].size, row[0].ref_start, (int)id, 0, I, 1);
+    }
+  } else {   /* genome_map__build_contextual (genome_map.c:432-536) */
+    int *ids = xmalloc(sizeof(int) * (size_t)(res->n_reads ? res->n_reads : 1));
+    long nm = 0;
+    for (long id = 0; id < res->n_reads; ++id)
+      if (res->map[id]) ids[nm++] = (int)id;
+    gm_sort(ids, nm, res->map);
+    for (long s = 0; s < nm; ++s) {
+      const int id = ids[s];
+      const orc_hit_t *row = res->map[id];
+      int adj[ORC_MAP_DETAIL];
+      for (int k = 0; k < ORC_MAP_DETAIL; ++k) adj[k] = ORC_MIN_SCORE;
+      int best_rank = ORC_UNMAPPED, best = ORC_MIN_SCORE;
+      for (int k = 0; k < ORC_MAP_DETAIL && row[k].ref_start != ORC_UNMAPPED; ++k) {
+        adj[k] = gm_adjusted(g[row[k].ref_id], &row[k], &refs[row[k].ref_id], p);
+        if (adj[k] > best) { best_rank = k; best = adj[k]; }
+      }
+      if (best_rank == ORC_UNMAPPED || !(p->uniqueness == 0 || adj[0] - adj[1] > p->uniqueness_threshold)) continue;
+      const orc_hit_t *h = &row[best_rank];
+      orc_gpos_t **gr = g[h->ref_id];
+      const orc_ref_t *ref = &refs[h->ref_id];
+      long pos = h->ref_start;
+      gm_place(gr, ref->size, pos, id, best_rank, I, 0);
+      int ins = -1;
+      for (int k = 0; k < h->traceback_len; ++k) {
+        int cell = tb_fwd(h->traceback, k);
+        switch (pair_type(cell, p->color)) {
+          case 0xC: ins++; gm_count(gr, ref->size, pos, ins, cell & 3, I); break;
+          case 0x3: ins = -1; gm_count(gr, ref->size, pos, -1, orc_nth(ref->seq, pos), I); break;
+          case 0x4: case 0x8: ins = -1; gm_count(gr, ref->size, pos, -1, cell & 3, I); break;
+          case 0x1: ins = -1; gm_count(gr, ref->size, pos, -1, GM_NO_CODE, I); break;
+          default: break;
+        }
+        for (int q = ins + 1; q < I; ++q) gm_count(gr, ref->size, pos, q, GM_NO_CODE, I);
+        if (ins < 0) pos++;
+      }
+    }
+    free(ids);
+  }
+  /* genome_map__generate_SAM_output (genome_map.c:551-800) */
+  const int L = db->read_len;
+  uint8_t **sym = p->color ? orc_ref_symbols(refs, n_refs) : NULL;
+  orc_sam_header(out, "coordinate", db, refs, n_refs, p, argc, argv);
+  char *cigar = xmalloc((size_t)L * 4 + 64);
+  char *seq = xmalloc((size_t)L + 64);
+  for (int r = 0; r < n_refs; ++r) {
+    for (long pos = 0; pos <= refs[r].size; ++pos) {
+      orc_gpos_t *x = g[r][pos];
+      if (!x) continue;
+      for (int k = 0; k < x->n; ++k) {
+        const orc_hit_t *row = res->map[x->read[k]];
+        const orc_hit_t *h = &row[x->rank[k]];
+        int flag = (h->sense & ORC_SENSE_REV) ? 0x10 : 0;
+        orc_sam_record(out, &db->reads[x->read[k]], h, flag, orc_mapq(row, x->rank[k], p), refs, sym, p, L, cigar, seq);
+        fputc('\n', out);
+      }
+      free(x->read); free(x->rank); free(x->inserted); free(x);
+    }
+    free(g[r]);
+  }
+  free(g);
   fflush(out);
   free(cigar); free(seq);
   if (sym) { for (int r = 0; r < n_refs; ++r) free(sym[r]); free(sym); }

@@ -1,6 +1,7 @@
 /*
  * oracle_main.c -- command line of the CPU restatement (test infrastructure only).
- * Takes the reference's flags (main.c:188) for the -M mapping path and writes the same SAM.
+ * Takes the reference's flags (main.c:188) and writes the same SAM: the -M hit map output, or
+ * without -M the ordered (genome map) output, contextual or greedy (-G).
  * Built twice: -DORC_COLOR=0 (nucleotide) and -DORC_COLOR=1 (color space).
  */
 #include "storm_oracle.h"
@@ -21,7 +22,7 @@ int main(int argc, char **argv) {
                                 : "####--#-#--##--####;#####--###-#-###;###-##-#######";   /* seed.h:29-32 */
   int opt;
   p.unordered_M = 0;
-  while ((opt = getopt(argc, argv, "g:r:q:s:b:m:x:d:e:t:z:i:v:o:l:u:M:N:jFRA")) != -1) {
+  while ((opt = getopt(argc, argv, "g:r:q:s:b:m:x:d:e:t:z:i:v:o:l:u:M:N:jFRAG")) != -1) {
     switch (opt) {
       case 'g': genome = optarg; break;
       case 'r': reads = optarg; break;
@@ -48,13 +49,17 @@ int main(int argc, char **argv) {
       case 'j': p.ignore_lowercase = 1; break;
       case 'F': p.sense_mask = ORC_SENSE_FWD; break;
       case 'R': p.sense_mask = ORC_SENSE_REV; break;
-      case 'A': p.print_all = 1; break;
+      case 'A':
+        if (!p.unordered_M) { fprintf(stderr, "-A needs the unordered mapping (-M <number>) first\n"); return 1; }   /* main.c:377-382 */
+        p.print_all = 1;
+        break;
+      case 'G': p.greedy = 1; break;
       case 'v': case 'N': break;
       default: return 1;
     }
   }
-  if (!genome || !reads || p.unordered_M < 1 || p.unordered_M > 4) {
-    fprintf(stderr, "usage: %s -g genome.fa -r reads.fq -M <1..4> [reference flags]\n", argv[0]);
+  if (!genome || !reads || p.unordered_M < 0 || p.unordered_M > 4) {
+    fprintf(stderr, "usage: %s -g genome.fa -r reads.fq [-M <1..4>] [reference flags]\n", argv[0]);
     return 1;
   }
   p.n_bytes = orc_n_bytes(p.simd_diags);
@@ -71,7 +76,8 @@ int main(int argc, char **argv) {
   orc_map(&db, refs, nr, sd, ns, &p, &res);
   FILE *out = outpath ? fopen(outpath, "w") : stdout;
   if (!out) { fprintf(stderr, "cannot open %s\n", outpath); return 1; }
-  orc_write_sam(out, &res, &db, refs, nr, &p, argc, argv);
+  if (p.unordered_M) orc_write_sam(out, &res, &db, refs, nr, &p, argc, argv);
+  else orc_write_sam_ordered(out, &res, &db, refs, nr, &p, argc, argv);
   if (out != stdout) fclose(out);
   fprintf(stderr, "oracle: %ld of %ld reads mapped; candidates %ld, filter passes %ld, accepted %ld\n",
           res.mapped, res.n_reads, res.n_candidates, res.n_filter_pass, res.n_accepted);

@@ -111,6 +111,7 @@ typedef struct {
   int unordered_M;        /* -M */
   int uniqueness, uniqueness_threshold;   /* -u */
   int print_all;          /* -A */
+  int greedy;             /* -G: greedy ordered mapping (genome_map.c:210-232) */
 } orc_params_t;
 
 /* defaults of the reference (alignment.h:24-28,51-52, main.c:37-38) */
@@ -198,6 +199,11 @@ void orc_result_free(orc_result_t *res);
 
 int orc_traceback_to_cigar(const uint8_t *tb, int tb_len, const uint8_t *ref_seq, long ref_start,
                            int color, uint8_t ref_start_symbol, char *cigar, char *seq);
+/* ordered mode (no -M, the reference's default): the genome map built from the hit map, greedy
+ * (-G) or contextual (genome_map.c:210-536), and its coordinate-ordered SAM (genome_map.c:551-800) */
+void orc_write_sam_ordered(FILE *out, const orc_result_t *res, const orc_reads_t *db,
+                           const orc_ref_t *refs, int n_refs, const orc_params_t *p,
+                           int argc, char **argv);
 void orc_write_sam(FILE *out, const orc_result_t *res, const orc_reads_t *db,
                    const orc_ref_t *refs, int n_refs, const orc_params_t *p,
                    int argc, char **argv);

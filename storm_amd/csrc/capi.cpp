@@ -40,6 +40,8 @@ double now_s() {
 
 int filter_lanes(int d) { return d <= 1 ? 2 : d <= 3 ? 4 : d <= 7 ? 8 : 16; }   // run.c:109-127
 
+int tb_ranks(const storm_params& p) { return p.max_hits == 0 ? 4 : p.max_hits; }
+
 void fill_score(const storm_params& p, ScoreDev& sc) {
   sc.fmatch = std::abs(p.match) & 0xff;
   sc.fmismatch = std::abs(p.mismatch) & 0xff;
@@ -482,7 +484,7 @@ int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int pass_id
   HIPOK(P.mark_slot.ensure(rb * 4));
   HIPOK(P.mark_count.ensure(4));
   HIPOK(hipMemsetAsync(P.mark_count.p, 0, 4, st));
-  HIPOK(launch_mark_referenced(slots + 4 * r0, 4 * (r1 - r0), pass_id, m->p.max_hits, P.mark_rec.as<int32_t>(),
+  HIPOK(launch_mark_referenced(slots + 4 * r0, 4 * (r1 - r0), pass_id, tb_ranks(m->p), P.mark_rec.as<int32_t>(),
                                P.mark_slot.as<int32_t>(), P.mark_count.as<int32_t>(), st));
   (void)n_slots;
   int32_t n_mark = 0;
@@ -533,7 +535,7 @@ static int assemble_slots(storm_mapper* m, const std::vector<Slot>& hs, const st
       h.score = s.score; h.ref_id = s.ref_id; h.ref_start = s.ref_start; h.ref_end = s.ref_end;
       h.read_start = s.read_start; h.read_end = s.read_end; h.sense = s.sense;
       h.tb_len = s.tb_len;
-      if (s.ref_id == -1 || k >= m->p.max_hits || !m->want_tb) continue;
+      if (s.ref_id == -1 || k >= tb_ranks(m->p) || !m->want_tb) continue;
       if (s.pass == PASS_FUSED) {
         if (!cells) return fail(STORM_ERR_DEVICE, "traceback of a fused hit was not gathered");
         auto it = cells->find(4 * i + k);
@@ -884,7 +886,7 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
   std::vector<Slot> hs((size_t)(4 * n));
   if (m->want_tb) {
     HIPOK(launch_fused_reset(F.gcount.as<int32_t>(), nullptr, m->st));
-    HIPOK(launch_gather_cells(F.slots.as<Slot>(), 4 * n, m->p.max_hits, F.cells.as<uint8_t>(), F.gids.as<int32_t>(),
+    HIPOK(launch_gather_cells(F.slots.as<Slot>(), 4 * n, tb_ranks(m->p), F.cells.as<uint8_t>(), F.gids.as<int32_t>(),
                               F.gtb.as<uint8_t>(), F.gcount.as<int32_t>(), m->st));
     HIPOK(hipMemcpyAsync(&ng, F.gcount.p, 4, hipMemcpyDeviceToHost, m->st));
   }
@@ -969,7 +971,9 @@ int storm::check_params(const storm_params* p) {
   if (!p) return fail(STORM_ERR_INVALID, "no parameters");
   if (p->allowed_indels < 0 || p->allowed_indels > MAX_I)
     return fail(STORM_ERR_INVALID, "-i %d outside 0..%d", p->allowed_indels, MAX_I);
-  if (p->max_hits < 1 || p->max_hits > 4) return fail(STORM_ERR_INVALID, "-M must be in 1..4");
+  if (p->max_hits < 0 || p->max_hits > 4) return fail(STORM_ERR_INVALID, "-M must be in 1..4 (0: ordered mode)");
+  if (p->max_hits == 0 && p->print_all)   // main.c:377-382
+    return fail(STORM_ERR_INVALID, "-A (all reads in the SAM) needs the unordered mapping (-M <number>)");
   if (p->match <= 0 || p->mismatch > 0 || p->gap_open > 0 || p->gap_extend > 0)
     return fail(STORM_ERR_INVALID, "scores: match > 0, mismatch/gap penalties <= 0 (main.c:262-291)");
   if (p->sense_mask < 1 || p->sense_mask > 3) return fail(STORM_ERR_INVALID, "sense_mask must be 1, 2 or 3");

@@ -608,14 +608,8 @@ static int traceback_to_cigar(const HostSlot& s, const HostRef& ref, const std::
   return ed;
 }
 
-int write_sam(const char* path, const std::vector<HostRef>& refs, const HostReads& reads,
-              const std::vector<HostSlot>& slots, const storm_params& p, int argc, char** argv, std::string& err) {
-  FILE* out = fopen(path, "wb");
-  if (!out) { err = std::string("cannot write ") + path; return STORM_ERR_IO; }
-  static const char BASES[4] = {'A', 'C', 'G', 'T'};
-  static const short UB[4] = {2, 7, 10, 93};
-  const bool color = p.color != 0;
-  std::vector<std::vector<uint8_t>> sym;
+SamFormat::SamFormat(const std::vector<HostRef>& refs_, const HostReads& reads_, const storm_params& p_)
+    : refs(refs_), reads(reads_), p(p_), color(p_.color != 0) {
   if (color) {   // hit_map.c:236-249, base-space symbols of each color reference (zeroed buffer)
     sym.resize(refs.size());
     for (size_t r = 0; r < refs.size(); ++r) {
@@ -625,7 +619,10 @@ int write_sam(const char* path, const std::vector<HostRef>& refs, const HostRead
       sym[r][(size_t)refs[r].size] = s;
     }
   }
-  fprintf(out, "@HD\tVN:1.4\tSO:unsorted\n");
+}
+
+void SamFormat::header(FILE* out, const char* sort_order, int argc, char** argv) const {
+  fprintf(out, "@HD\tVN:1.4\tSO:%s\n", sort_order);
   for (auto& r : refs)
     fprintf(out, "@SQ\tSN:%s\tLN:%ld\n", r.name.empty() ? "*" : r.name.c_str(), (long)(r.size + (color ? 1 : 0)));
   const char* rg = reads.db_name.empty() ? "*" : reads.db_name.c_str();
@@ -633,8 +630,86 @@ int write_sam(const char* path, const std::vector<HostRef>& refs, const HostRead
   fprintf(out, "@PG\tID:SToRM\tVN:0.0101\tCL:\"");
   for (int a = 0; a < argc; ++a) fprintf(out, "%s ", argv[a]);
   fprintf(out, "\"\n");
+}
+
+int SamFormat::mapq(const HostSlot* row, int k) const {   // hit_map.c:374-399
   const double lg = log(.3);
   const int delta_m = p.match - p.mismatch;
+  const int s1 = row[0].score, s2 = (k == 0) ? std::max(row[1].score, p.t) : row[k].score;
+  const double nn = (k == 0) ? (double)(s1 - s2) / (double)delta_m : (double)(s2 - s1) / (double)delta_m;
+  int mq = -4.343 * (nn * lg);
+  mq = (int)(mq - 4.343 * lg);
+  mq = mq >= 0 ? mq : 0;
+  return mq < 254 ? mq : 254;
+}
+
+void SamFormat::record(std::string& line, int64_t id, const HostSlot* row, int k, int flag, std::string& cigar,
+                       std::string& seq) const {   // hit_map.c:401-448
+  static const short UB[4] = {2, 7, 10, 93};
+  char buf[96];
+  const int L = reads.len[id];
+  const bool rq = reads.has_qual && !(!reads.no_qual.empty() && reads.no_qual[(size_t)id]);
+  const uint8_t* qs = rq ? &reads.qual[(size_t)(id * reads.qual_stride)] : nullptr;
+  const HostRef& ref = refs[(size_t)row[k].ref_id];
+  const int ed = traceback_to_cigar(row[k], ref, color ? &sym[(size_t)row[k].ref_id] : nullptr, color, cigar, seq);
+  line += reads.names[(size_t)id];
+  snprintf(buf, sizeof buf, "\t%d\t", flag);
+  line += buf;
+  line += ref.name.empty() ? "*" : ref.name;
+  snprintf(buf, sizeof buf, "\t%d\t%d\t", row[k].ref_start + 1, mapq(row, k));
+  line += buf;
+  line += cigar;
+  line += "\t*\t0\t0\t";
+  line += seq;
+  line += '\t';
+  if (qs) {
+    auto qc = [&](int lv) { int v = p.qual_base + UB[lv]; return (char)(v < 126 ? v : 126); };
+    if (row[k].sense == -1) {
+      for (int q = L - 1; q >= 0; --q) line += qc(get2(qs, q));
+      if (color) line += qc(reads.first_qual[(size_t)id]);
+    } else {
+      if (color) line += qc(reads.first_qual[(size_t)id]);
+      for (int q = 0; q < L; ++q) line += qc(get2(qs, q));
+    }
+  } else {
+    line += '*';
+  }
+  snprintf(buf, sizeof buf, "\tAS:i:%d\tNM:i:%d", row[k].score, ed);
+  line += buf;
+}
+
+void write_blocks(FILE* out, int64_t n, const std::function<void(int64_t, std::string&, std::string&, std::string&)>& fmt) {
+  const int nt = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+  const int64_t per_thread = 32768, block = per_thread * nt;
+  std::vector<std::string> part((size_t)nt);
+  for (int64_t b0 = 0; b0 < n; b0 += block) {
+    const int64_t b1 = std::min(n, b0 + block);
+    auto work = [&](int t) {
+      std::string& out_t = part[(size_t)t];
+      out_t.clear();
+      std::string cigar, seq;
+      const int64_t r0 = b0 + t * per_thread, r1 = std::min(b1, r0 + per_thread);
+      for (int64_t id = r0; id < r1; ++id) fmt(id, out_t, cigar, seq);
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt && b0 + t * per_thread < b1; ++t) th.emplace_back(work, t);
+    work(0);
+    for (auto& x : th) x.join();
+    for (int t = 0; t < nt; ++t)
+      if (!part[(size_t)t].empty()) fwrite(part[(size_t)t].data(), 1, part[(size_t)t].size(), out);
+  }
+}
+
+int write_sam(const char* path, const std::vector<HostRef>& refs, const HostReads& reads,
+              const std::vector<HostSlot>& slots, const storm_params& p, int argc, char** argv, std::string& err) {
+  if (p.max_hits == 0) return write_sam_ordered(path, refs, reads, slots, p, argc, argv, err);
+  FILE* out = fopen(path, "wb");
+  if (!out) { err = std::string("cannot write ") + path; return STORM_ERR_IO; }
+  static const char BASES[4] = {'A', 'C', 'G', 'T'};
+  static const short UB[4] = {2, 7, 10, 93};
+  const bool color = p.color != 0;
+  const SamFormat sf(refs, reads, p);
+  sf.header(out, "unsorted", argc, argv);
   const int M = p.max_hits;
   // the records of read `id` appended to `line` (hit_map.c:325-504); reads are formatted in
   // parallel chunks and written in read order
@@ -647,45 +722,13 @@ int write_sam(const char* path, const std::vector<HostRef>& refs, const HostRead
       return k < M && row[k].ref_start != -1 &&
              (p.uniqueness == 0 || ((k + 1 < M) && (row[k].score - row[k + 1].score) >= p.uniqueness) || (k + 1 == M));
     };
-    char buf[96];
+    char buf[32];
     if (row[0].ref_id != -1) {
       int nrep = 0;
       for (int k = 0; reportable(k); ++k) nrep++;
       for (int k = 0; reportable(k); ++k) {
-        int flag = 0;
-        if (row[k].sense == -1) flag |= 0x10;
-        if (k) flag |= 0x100;
-        int s1 = row[0].score, s2 = (k == 0) ? std::max(row[1].score, p.t) : row[k].score;
-        double nn = (k == 0) ? (double)(s1 - s2) / (double)delta_m : (double)(s2 - s1) / (double)delta_m;
-        int mq = -4.343 * (nn * lg);
-        mq = (int)(mq - 4.343 * lg);
-        mq = mq >= 0 ? mq : 0;
-        mq = mq < 254 ? mq : 254;
-        const HostRef& ref = refs[(size_t)row[k].ref_id];
-        int ed = traceback_to_cigar(row[k], ref, color ? &sym[(size_t)row[k].ref_id] : nullptr, color, cigar, seq);
-        line += reads.names[(size_t)id];
-        snprintf(buf, sizeof buf, "\t%d\t", flag);
-        line += buf;
-        line += ref.name.empty() ? "*" : ref.name;
-        snprintf(buf, sizeof buf, "\t%d\t%d\t", row[k].ref_start + 1, mq);
-        line += buf;
-        line += cigar;
-        line += "\t*\t0\t0\t";
-        line += seq;
-        line += '\t';
-        if (qs) {
-          auto qc = [&](int lv) { int v = p.qual_base + UB[lv]; return (char)(v < 126 ? v : 126); };
-          if (row[k].sense == -1) {
-            for (int q = L - 1; q >= 0; --q) line += qc(get2(qs, q));
-            if (color) line += qc(reads.first_qual[(size_t)id]);
-          } else {
-            if (color) line += qc(reads.first_qual[(size_t)id]);
-            for (int q = 0; q < L; ++q) line += qc(get2(qs, q));
-          }
-        } else {
-          line += '*';
-        }
-        snprintf(buf, sizeof buf, "\tAS:i:%d\tNM:i:%d\tNH:i:%d\n", row[k].score, ed, nrep);
+        sf.record(line, id, row, k, (row[k].sense == -1 ? 0x10 : 0) | (k ? 0x100 : 0), cigar, seq);
+        snprintf(buf, sizeof buf, "\tNH:i:%d\n", nrep);
         line += buf;
       }
     } else if (p.print_all) {
@@ -706,25 +749,7 @@ int write_sam(const char* path, const std::vector<HostRef>& refs, const HostRead
       }
     }
   };
-  const int nt = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
-  const int64_t per_thread = 32768, block = per_thread * nt;
-  std::vector<std::string> part((size_t)nt);
-  for (int64_t b0 = 0; b0 < reads.n; b0 += block) {
-    const int64_t b1 = std::min(reads.n, b0 + block);
-    auto work = [&](int t) {
-      std::string& out_t = part[(size_t)t];
-      out_t.clear();
-      std::string cigar, seq;
-      const int64_t r0 = b0 + t * per_thread, r1 = std::min(b1, r0 + per_thread);
-      for (int64_t id = r0; id < r1; ++id) format_read(id, out_t, cigar, seq);
-    };
-    std::vector<std::thread> th;
-    for (int t = 1; t < nt && b0 + t * per_thread < b1; ++t) th.emplace_back(work, t);
-    work(0);
-    for (auto& x : th) x.join();
-    for (int t = 0; t < nt; ++t)
-      if (!part[(size_t)t].empty()) fwrite(part[(size_t)t].data(), 1, part[(size_t)t].size(), out);
-  }
+  write_blocks(out, reads.n, format_read);
   if (fclose(out) != 0) { err = std::string("error writing ") + path; return STORM_ERR_IO; }
   return STORM_OK;
 }

@@ -2,6 +2,7 @@
 #pragma once
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -56,9 +57,35 @@ struct HostSlot {
   int tb_len;
 };
 
-// hit_map__generate_SAM_output (hit_map.c:197-504) for reads in order; slots[4*r + k]
+// hit_map__generate_SAM_output (hit_map.c:197-504) for reads in order; slots[4*r + k].  With
+// p.max_hits == 0 (no -M) the ordered output, write_sam_ordered.
 int write_sam(const char* path, const std::vector<HostRef>& refs, const HostReads& reads,
               const std::vector<HostSlot>& slots, const storm_params& p, int argc, char** argv, std::string& err);
+
+// the ordered mode (no -M, the reference's default): the genome map built from the hit map,
+// greedy (p.greedy, -G) or contextual, and its coordinate-ordered SAM (genome_map.c:186-800)
+int write_sam_ordered(const char* path, const std::vector<HostRef>& refs, const HostReads& reads,
+                      const std::vector<HostSlot>& slots, const storm_params& p, int argc, char** argv,
+                      std::string& err);
+
+// SAM text shared by the two writers: header lines, MAPQ, and the fields of a mapped record
+struct SamFormat {
+  SamFormat(const std::vector<HostRef>& refs, const HostReads& reads, const storm_params& p);
+  void header(FILE* out, const char* sort_order, int argc, char** argv) const;
+  int mapq(const HostSlot* row, int k) const;
+  // QNAME .. QUAL and the AS / NM tags of rank k of read id, no line end
+  void record(std::string& line, int64_t id, const HostSlot* row, int k, int flag, std::string& cigar,
+              std::string& seq) const;
+  const std::vector<HostRef>& refs;
+  const HostReads& reads;
+  const storm_params& p;
+  bool color;
+  std::vector<std::vector<uint8_t>> sym;   // color: base before each position of each reference
+};
+
+// items [0, n) formatted by fmt(item, text, cigar scratch, seq scratch) on up to 16 threads in
+// blocks, written to out in item order
+void write_blocks(FILE* out, int64_t n, const std::function<void(int64_t, std::string&, std::string&, std::string&)>& fmt);
 
 uint32_t glibc_rand_next(uint32_t* state /* 35+ words, state[34] = position */);
 

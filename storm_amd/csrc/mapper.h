@@ -200,6 +200,8 @@ namespace storm {
 
 int filter_lanes(int d);
 void fill_score(const storm_params& p, ScoreDev& sc);
+// hit map ranks whose tracebacks the SAM needs: the -M reported ranks, all four in ordered mode
+int tb_ranks(const storm_params& p);
 const char* default_seeds(int color);
 int check_params(const storm_params* p);
 

@@ -539,6 +539,9 @@ int storm_server_submit(storm_server* s, const storm_reads_view* v, const storm_
   if (!s || !v || !params || !id) return fail(STORM_ERR_INVALID, "null argument");
   int rc = check_params(params);
   if (rc) return rc;
+  if (params->max_hits == 0)
+    return fail(STORM_ERR_INVALID, "the ordered mode (-M 0) places the reads of a whole run together; a request "
+                                   "needs -M 1..4");
   const int color = params->color ? 1 : 0;
   if (!s->res[color]) return fail(STORM_ERR_INVALID, "the server holds no %s index", color ? "color" : "nucleotide");
   if (v->n_reads <= 0) return fail(STORM_ERR_INVALID, "no reads");

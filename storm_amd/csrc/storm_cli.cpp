@@ -1,5 +1,6 @@
-// storm_cli.cpp -- command line with the reference's flags for the -M mapping path (main.c:188-433),
-// running the gfx950 pipeline through the C ABI.  Built as storm-gpu-nucleotide / storm-gpu-color.
+// storm_cli.cpp -- command line with the reference's flags (main.c:188-433), running the gfx950
+// pipeline through the C ABI: -M <1..4> writes the hit map SAM, no -M the ordered (genome map)
+// SAM, contextual or greedy (-G).  Built as storm-gpu-nucleotide / storm-gpu-color.
 #include <unistd.h>
 
 #include <cstdio>
@@ -17,7 +18,7 @@ int main(int argc, char** argv) {
   const char *genome = nullptr, *reads = nullptr, *quals = nullptr, *out = "/dev/stdout";
   int mode = STORM_MODE_BATCHED, device = 0, opt;
   p.max_hits = 0;
-  while ((opt = getopt(argc, argv, "g:r:q:s:b:m:x:d:e:t:z:i:v:o:l:u:M:N:jFRAPD:")) != -1) {
+  while ((opt = getopt(argc, argv, "g:r:q:s:b:m:x:d:e:t:z:i:v:o:l:u:M:N:jFRAGPD:")) != -1) {
     switch (opt) {
       case 'g': genome = optarg; break;
       case 'r': reads = optarg; break;
@@ -38,15 +39,22 @@ int main(int argc, char** argv) {
       case 'j': p.ignore_lowercase = 1; break;
       case 'F': p.sense_mask = 1; break;
       case 'R': p.sense_mask = 2; break;
-      case 'A': p.print_all = 1; break;
+      case 'A':
+        if (!p.max_hits) {   // main.c:377-382
+          fprintf(stderr, "%s: -A (output all reads) needs the unordered mapping (-M <number>) first\n", argv[0]);
+          return 1;
+        }
+        p.print_all = 1;
+        break;
+      case 'G': p.greedy = 1; break;
       case 'P': mode = STORM_MODE_PLAIN; break;     // plain per-batch path
       case 'D': device = atoi(optarg); break;
       case 'v': case 'N': break;
       default: return 1;
     }
   }
-  if (!genome || !reads || p.max_hits < 1) {
-    fprintf(stderr, "usage: %s -g genome.fa -r reads.{fq,fa,csfasta} [-q reads.qual] -M <1..4> [reference flags] [-P]\n", argv[0]);
+  if (!genome || !reads || p.max_hits < 0 || p.max_hits > 4) {
+    fprintf(stderr, "usage: %s -g genome.fa -r reads.{fq,fa,csfasta} [-q reads.qual] [-M <1..4>] [-G] [reference flags] [-P]\n", argv[0]);
     return 1;
   }
   storm_map_stats st{};

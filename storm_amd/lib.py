@@ -26,7 +26,7 @@ class Params(C.Structure):
     _fields_ = [(n, C.c_int) for n in (
         "color", "match", "mismatch", "gap_open", "gap_extend", "allowed_indels", "t", "z", "qual_base",
         "stdev_distance", "sense_mask", "max_hits", "uniqueness", "print_all", "ignore_lowercase",
-        "strict_length")] + [("seeds", C.c_char_p)]
+        "strict_length")] + [("seeds", C.c_char_p), ("greedy", C.c_int)]
 
 
 class MapStats(C.Structure):

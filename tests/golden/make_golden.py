@@ -5,7 +5,8 @@ Run in the container that holds the reference sources (it builds oracle/_ref fro
     python tests/golden/make_golden.py
 
 Outputs, all small and committed:
-  sam_<case>/   genome.fa.gz, reads.*.gz, ref.sam.gz (the reference's -M SAM), args.json
+  sam_<case>/   genome.fa.gz, reads.*.gz, ref.sam.gz (the reference's SAM: -M, or the ordered
+                genome-map output when the case has no -M), args.json
   filter_W<w>_L<l>.npz   packed genome/reads, window starts and the u8 score of every candidate
                          from the reference SSE2 routine (threshold sweep over t = 0..254)
 
@@ -43,6 +44,10 @@ SAM_CASES = [
     ("c_i3_M1", "c", "plain", 60000, 1500, 50, 3, ["-M", "1", "-z", "85", "-t", "100"], 104),
     ("c_i7_M4_rep", "c", "repeat", 60000, 1500, 60, 7, ["-M", "4", "-l", "3"], 105),
     ("c_i2_M3_u", "c", "masked", 40000, 1000, 35, 2, ["-M", "3", "-u", "4", "-A"], 106),
+    # ordered mode (no -M): the genome map output, contextual by default, greedy with -G
+    ("n_i7_ordered_rep", "n", "repeat", 60000, 1500, 75, 7, ["-l", "3"], 107),
+    ("n_i3_ordered_G_u", "n", "masked", 40000, 1000, 50, 3, ["-G", "-u", "3"], 108),
+    ("c_i3_ordered_rep", "c", "repeat", 60000, 1500, 50, 3, ["-z", "85"], 109),
 ]
 
 FILTER_CASES = [(d, L) for d in (1, 3, 7, 15) for L in (35, 50, 75, 100, 150)]
@@ -145,13 +150,19 @@ def make_filter_case(d, L, lib):
 
 
 def main():
+    """With arguments, only the SAM cases of those names are written."""
     if not os.path.isdir("/root/reference"):
         raise SystemExit("the reference sources are needed to write golden fixtures")
     sh(["make", "-C", os.path.join(ROOT, "oracle"), "-j8", "ref"])
     tmp = os.path.join(ROOT, "_work", "golden_tmp")
     os.makedirs(tmp, exist_ok=True)
+    only = set(sys.argv[1:])
     for c in SAM_CASES:
-        make_sam_case(*c, tmp)
+        if not only or c[0] in only:
+            make_sam_case(*c, tmp)
+    if only:
+        shutil.rmtree(tmp)
+        return
     lib = C.CDLL(os.path.join(REF, "libref_filter.so"))
     lib.ref_filter_scores.argtypes = [C.c_int] * 7 + [C.c_void_p, C.c_long, C.c_void_p, C.c_void_p, C.c_void_p,
                                                       C.c_long, C.c_void_p, C.c_int]

@@ -30,7 +30,7 @@ class OrcParams(C.Structure):
     _fields_ = [(n, C.c_int) for n in (
         "color", "match", "mismatch", "gap_open", "gap_extend", "allowed_indels", "simd_diags", "t", "z",
         "qual_base", "filter_repeats", "stdev_distance", "n_bytes", "sense_mask", "ignore_lowercase",
-        "unordered_M", "uniqueness", "uniqueness_threshold", "print_all")]
+        "unordered_M", "uniqueness", "uniqueness_threshold", "print_all", "greedy")]
 
 
 class OrcAlignOut(C.Structure):

@@ -2,14 +2,15 @@
 binaries (test_oracle_vs_reference_binary.py): -i 0 .. 20 (every band class of the filter and the
 DP, the two-records-per-lane DP and the single-record one), -M 1-4, -u, -A, -F / -R, custom scores,
 explicit seeds and position lists, N runs, lowercase, multi-FASTA, repeat filter firing, in
-nucleotide and color space.  Integer and text output: the SAM must be identical."""
+nucleotide and color space; and the ordered mode (no -M: contextual genome map, greedy -G, -u).
+Integer and text output: the SAM must be identical."""
 import os
 import subprocess
 
 import pytest
 
 from conftest import ROOT, sam_body
-from test_oracle_vs_reference_binary import CASES, SEEDS_C, SEEDS_N, make_inputs
+from test_oracle_vs_reference_binary import CASES, ORDERED_CASES, SEEDS_C, SEEDS_N, make_inputs
 
 pytestmark = pytest.mark.gpu
 
@@ -26,7 +27,7 @@ FLAGS = {"-M": ("max_hits", int), "-z": ("z", int), "-t": ("t", int), "-l": ("st
 
 
 def params_of(mode, i, extra):
-    kw = dict(color=1 if mode == "c" else 0, allowed_indels=i, seeds=SEEDS_C if mode == "c" else SEEDS_N)
+    kw = dict(color=1 if mode == "c" else 0, allowed_indels=i, seeds=SEEDS_C if mode == "c" else SEEDS_N, max_hits=0)
     k = 0
     while k < len(extra):
         f = extra[k]
@@ -35,6 +36,9 @@ def params_of(mode, i, extra):
             k += 1
         elif f == "-F":
             kw["sense_mask"] = 1
+            k += 1
+        elif f == "-G":
+            kw["greedy"] = 1
             k += 1
         elif f == "-R":
             kw["sense_mask"] = 2
@@ -46,8 +50,8 @@ def params_of(mode, i, extra):
     return kw
 
 
-@pytest.mark.parametrize("mode,kind,nreads,L,i,extra,seed", CASES + EXTRA,
-                         ids=[f"{c[0]}-{c[1]}-i{c[4]}-{c[6]}" for c in CASES + EXTRA])
+@pytest.mark.parametrize("mode,kind,nreads,L,i,extra,seed", CASES + EXTRA + ORDERED_CASES,
+                         ids=[f"{c[0]}-{c[1]}-i{c[4]}-{c[6]}" for c in CASES + EXTRA + ORDERED_CASES])
 def test_gpu_paths_equal_oracle(mode, kind, nreads, L, i, extra, seed, tmp_path, oracle_built):
     from storm_amd import lib
     tmp = str(tmp_path)

@@ -1,4 +1,5 @@
-"""Whole -M mapping runs on the GPU against the reference's SAM (golden fixtures)."""
+"""Whole mapping runs on the GPU against the reference's SAM (golden fixtures): -M hit map output,
+and the ordered (genome map) output of the cases without -M, contextual or greedy (-G)."""
 import os
 
 import pytest
@@ -12,7 +13,7 @@ def _run(meta, paths, out, mode):
     from storm_amd import lib
     color = 1 if meta["mode"] == "c" else 0
     a = meta["args"]
-    kw = dict(color=color)
+    kw = dict(color=color, max_hits=0)
     i = 0
     while i < len(a):
         f, v = a[i], a[i + 1] if i + 1 < len(a) else None
@@ -24,6 +25,7 @@ def _run(meta, paths, out, mode):
         elif f == "-l": kw["stdev_distance"] = int(v)
         elif f == "-u": kw["uniqueness"] = int(v)
         elif f == "-A": kw["print_all"] = 1; i -= 1
+        elif f == "-G": kw["greedy"] = 1; i -= 1
         else: raise AssertionError(f)
         i += 2
     if color:

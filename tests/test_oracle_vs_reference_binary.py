@@ -102,6 +102,31 @@ def test_oracle_equals_reference_binary(mode, kind, nreads, L, i, extra, seed, t
     assert a == b
 
 
+# ordered mode: no -M, the reference's default (genome_map.c), contextual or greedy (-G)
+ORDERED_CASES = [
+    ("n", "multi", 1200, 48, 3, [], 41),
+    ("n", "repeat", 1500, 60, 7, ["-l", "3"], 42),
+    ("n", "repeat", 1200, 60, 7, ["-G"], 43),
+    ("n", "plain", 1000, 50, 2, ["-u", "5"], 44),
+    ("n", "plain", 800, 64, 12, ["-z", "60", "-t", "90", "-G", "-u", "3"], 45),
+    ("n", "plain", 800, 50, 0, [], 46),
+    ("c", "multi", 1200, 48, 3, ["-z", "85"], 51),
+    ("c", "repeat", 1500, 60, 7, ["-l", "3"], 52),
+    ("c", "plain", 1000, 50, 3, ["-G"], 53),
+    ("c", "plain", 800, 35, 2, ["-u", "4"], 54),
+]
+
+
+@pytest.mark.parametrize("mode,kind,nreads,L,i,extra,seed", ORDERED_CASES,
+                         ids=[f"{c[0]}-{c[1]}-i{c[4]}-{c[6]}" for c in ORDERED_CASES])
+def test_oracle_ordered_equals_reference_binary(mode, kind, nreads, L, i, extra, seed, tmp_path, oracle_built):
+    gpath, rargs = make_inputs(str(tmp_path), mode, kind, nreads, L, seed)
+    a, b = run_both(str(tmp_path), mode, gpath, rargs, ["-i", str(i)] + extra)
+    assert a[0].startswith("@HD\tVN:1.4\tSO:coordinate")
+    assert len(a) > 5
+    assert a == b
+
+
 @pytest.mark.parametrize("mode", ["n", "c"])
 def test_oracle_equals_reference_binary_without_qualities(mode, tmp_path, oracle_built):
     gpath, rargs = make_inputs(str(tmp_path), mode, "plain", 600, 50, 31, fasta_reads=True)

@@ -23,7 +23,7 @@ class OrcParams(C.Structure):
     _fields_ = [(f, C.c_int) for f in ("color", "match", "mismatch", "gap_open", "gap_extend", "allowed_indels",
                                        "simd_diags", "t", "z", "qual_base", "filter_repeats", "stdev_distance",
                                        "n_bytes", "sense_mask", "ignore_lowercase", "unordered_M", "uniqueness",
-                                       "uniqueness_threshold", "print_all")]
+                                       "uniqueness_threshold", "print_all", "greedy")]
 
 
 def oracle_reads(oracle_built, path, color):
