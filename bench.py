@@ -178,7 +178,9 @@ def ref_binary(color: int):
 
 
 def cli_args(kw) -> list:
-    out = ["-i", str(kw["allowed_indels"]), "-z", str(kw["z"]), "-t", str(kw["t"]), "-M", str(kw["max_hits"])]
+    out = ["-i", str(kw["allowed_indels"]), "-z", str(kw["z"]), "-t", str(kw["t"])]
+    if kw["max_hits"]:
+        out += ["-M", str(kw["max_hits"])]   # no -M: the ordered (genome map) output
     return out
 
 
@@ -205,7 +207,7 @@ def run_reference(color, gpath, rargs, kw, sam, cores):
     return time.perf_counter() - t0
 
 
-def cpu_mapping_baseline(lib, d, gpath, pr, kw, color, n_sample):
+def cpu_mapping_baseline(lib, d, gpath, pr, kw, color, n_sample, ordered=False):
     """The reference binary (oracle/_ref, default Makefile flags, OpenMP) on a sample of the
     workload's reads, timed on this host: one run on 16 reads gives the fixed cost (genome load,
     index build), one on the sample; reads/s = sample / (difference).  The GPU maps the same files
@@ -232,13 +234,29 @@ def cpu_mapping_baseline(lib, d, gpath, pr, kw, color, n_sample):
     lib.map_files(gpath, files[0], gsam, qual=files[1], mode=1, color=color, **kw)
     same = sam_body(gsam) == sam_body(os.path.join(d, "ref_sample.sam"))
     dt = max(t_all - t_fixed, 1e-6)
-    return {"value": round(n_sample / dt, 1), "unit": "reads/s", "cores": cores, "kind": "reference",
-            "sample": f"{n_sample} reads of the workload{length_note}; reference {os.path.basename(ref_binary(color))} -N {cores}: "
-                      f"{t_all:.1f}s total minus {t_fixed:.1f}s fixed cost (load + index, measured on 16 reads)",
-            "gpu_sam_identical_to_reference": bool(same)}
+    out = {"value": round(n_sample / dt, 1), "unit": "reads/s", "cores": cores, "kind": "reference",
+           "sample": f"{n_sample} reads of the workload{length_note}; reference {os.path.basename(ref_binary(color))} -N {cores}: "
+                     f"{t_all:.1f}s total minus {t_fixed:.1f}s fixed cost (load + index, measured on 16 reads)",
+           "gpu_sam_identical_to_reference": bool(same)}
+    if ordered:
+        # the same sample without -M: the ordered output (genome map placement, coordinate-sorted SAM)
+        ko = dict(kw, max_hits=0)
+        t_ref = run_reference(color, gpath, r_samp, ko, os.path.join(d, "ref_ordered.sam"), cores)
+        osam = os.path.join(d, "gpu_ordered.sam")
+        t0 = time.perf_counter()
+        so = lib.map_files(gpath, files[0], osam, qual=files[1], mode=1, color=color, **ko)
+        t_gpu = time.perf_counter() - t0
+        same_o = sam_body(osam) == sam_body(os.path.join(d, "ref_ordered.sam"))
+        out["ordered"] = {"reads": n_sample, "gpu_sam_identical_to_reference": bool(same_o),
+                          "gpu_call_s": round(t_gpu, 3), "gpu_map_s": round(so["seconds_map"], 3),
+                          "genome_map_and_sam_s": round(so["seconds_sam"], 3),
+                          "reference_s": round(t_ref, 2), "reference_minus_fixed_s": round(t_ref - t_fixed, 2)}
+        log(f"ordered mode on {n_sample} reads: GPU call {t_gpu:.2f}s (genome map + SAM {so['seconds_sam']:.2f}s), "
+            f"reference {t_ref:.2f}s; SAM identical: {same_o}")
+    return out
 
 
-def map_config(lib, name, gpath, pr, kw, color, n_sample_cpu, sam_reads, skip_plain=False):
+def map_config(lib, name, gpath, pr, kw, color, n_sample_cpu, sam_reads, skip_plain=False, ordered=False):
     """Batched (persistent fused kernel in a hipGraph) and plain (staged kernels, one batch per
     read length) on the same resident mapper; hits returned to host memory; whole-call wall time."""
     t0 = time.perf_counter()
@@ -285,7 +303,7 @@ def map_config(lib, name, gpath, pr, kw, color, n_sample_cpu, sam_reads, skip_pl
         res["sam_writer_reads_per_s"] = round(k / max(st["seconds_sam"], 1e-9), 1)
     m.close()
     if n_sample_cpu:
-        cb = cpu_mapping_baseline(lib, os.path.dirname(gpath), gpath, pr, kw, color, n_sample_cpu)
+        cb = cpu_mapping_baseline(lib, os.path.dirname(gpath), gpath, pr, kw, color, n_sample_cpu, ordered)
         res["cpu_baseline"] = cb
         if cb:
             log(f"{name}: reference CPU {cb['value']} reads/s on {cb['cores']} cores; GPU SAM identical on the sample: "
@@ -303,7 +321,7 @@ def config1(lib, cpu=True):
     synth.write_fasta_fast(gpath, "chr1", g)
     pr = synth.fast_nucleotide_reads(g, 100_000, 50, 101, sub=0.02, indel=0.001, qlo=20, qhi=40)
     kw = dict(allowed_indels=3, z=100, t=100, max_hits=1)
-    return map_config(lib, "config1", gpath, pr, kw, 0, 100_000 if cpu else 0, 100_000)
+    return map_config(lib, "config1", gpath, pr, kw, 0, 100_000 if cpu else 0, 100_000, ordered=True)
 
 
 def config3(lib, cpu=True, n_reads=5_000_000):
