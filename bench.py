@@ -34,6 +34,10 @@ VALU_PEAK_PK16 = 78.6        # T 16-bit ops/s with packed VALU instructions (2 p
 # VALU lane-instructions k_filter<8> issues per cell at L=75 (SQ_INSTS_VALU x 64 / cells, rocprofv3 --pmc,
 # profiles/k_filter_W8_L75_10Mcand.md); the kernel's own work measure for the VALU-issue roofline
 VALU_INSTR_PER_CELL_W8_L75 = 5.447
+# VALU lane-instructions k_align<7,false> issues per DP cell (L x 15 band columns) on the DP stage
+# workload (score pass of every record + traceback pass of the accepted ones), measured the same way
+# (profiles/k_align_I7_L75_2M_records.md)
+VALU_INSTR_PER_DP_CELL_I7_L75 = 78.52
 # HBM bytes per candidate of the same dispatch: (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00) x 1024 /
 # candidates, the factors calibrated on a device-to-device copy of known size in the same run
 # (profiles/k_filter_W8_L75_10Mcand.md)
@@ -154,6 +158,95 @@ def cpu_baseline(g, reads, pos, stride, L, d, n_sample, target_s=10.0):
                                 P(rd.ctypes.data), stride, P(bits.ctypes.data), cores, reps)
         out[name] = (2 * n * reps, t, cores)
     return out
+
+
+def gpu_align_stage(lib, g, reads, pos, stride, L, I, d, n_rec, reps):
+    """Stage (d) alone through storm_align_plan_*: the DP + traceback of the filter's true-origin
+    passes (read strand 0 at diagonal origin), qualities Q20-40 (all level 3, so no quality
+    array), default scores and -t 115; HIP-event time per run over `reps` runs."""
+    from storm_amd import synth
+    P = C.c_void_p
+    so = lib.load()
+    gp = synth.pack_codes(g)
+    n = min(n_rec, reads.shape[0])
+    rl = np.full(n, L, np.int32)
+    rs = (2 * np.arange(n, dtype=np.int64)).astype(np.int32)
+    diag = np.ascontiguousarray(pos[:n, 0] + d, dtype=np.int32)
+    p = lib.params(0, allowed_indels=I)
+    h = P()
+    lib.check(so.storm_align_plan_create(0, P(gp.ctypes.data), None, len(g), P(reads.ctypes.data), stride, None, 0,
+                                         None, P(rl.ctypes.data), n, P(rs.ctypes.data), P(diag.ctypes.data), n,
+                                         C.byref(p), C.byref(h)))
+    ms = C.c_double()
+    lib.check(so.storm_align_run(h, 1, C.byref(ms)))
+    lib.check(so.storm_align_run(h, reps, C.byref(ms)))
+    res = np.zeros(n, lib.ALIGN_RESULT)
+    tb = np.zeros((n, lib.TB_BYTES), np.uint8)
+    lib.check(so.storm_align_fetch(h, P(res.ctypes.data), P(tb.ctypes.data)))
+    so.storm_align_plan_destroy(h)
+    return ms.value, res, tb, rs, diag
+
+
+def cpu_align_baseline(g, reads, L, I, rs, diag, res, tb, n_sample=20000):
+    """The oracle's restatement of alignment__align + alignment__traceback (orc_align, one
+    thread, kind "port") on the first n_sample records of the DP workload, timed on this host;
+    the GPU records of the sample are compared with it."""
+    path = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+    if not os.path.exists(path):
+        return None
+
+    class OrcParams(C.Structure):
+        _fields_ = [(f, C.c_int) for f in (
+            "color", "match", "mismatch", "gap_open", "gap_extend", "allowed_indels", "simd_diags", "t", "z",
+            "qual_base", "filter_repeats", "stdev_distance", "n_bytes", "sense_mask", "ignore_lowercase",
+            "unordered_M", "uniqueness", "uniqueness_threshold", "print_all", "greedy")]
+
+    class OrcAlignOut(C.Structure):
+        _fields_ = [("score", C.c_int), ("best_i", C.c_int), ("best_j", C.c_int), ("best_i0", C.c_int),
+                    ("best_j0", C.c_int), ("tb", C.c_uint8 * 1024), ("tb_len", C.c_int)]
+
+    o = C.CDLL(path)
+    P = C.c_void_p
+    o.orc_params_default.argtypes = [C.POINTER(OrcParams), C.c_int]
+    o.orc_align.argtypes = [P, P, C.c_int, P, P, C.c_int, C.POINTER(OrcParams), C.c_int, C.POINTER(OrcAlignOut)]
+    o.orc_align.restype = C.c_int
+    op = OrcParams()
+    o.orc_params_default(C.byref(op), 0)
+    op.allowed_indels, op.simd_diags = I, min(I, 15)
+    n = min(n_sample, len(rs))
+    pad = 64
+    gpad = np.zeros(pad + len(g) + pad, np.uint8)
+    gpad[pad:pad + len(g)] = g
+    mpad = np.ones_like(gpad)
+    mpad[pad:pad + len(g)] = 0
+    qlev = np.full(L, 3, np.uint8)
+    codes = np.stack([(reads[:n, :(L + 3) // 4] >> s) & 3 for s in (0, 2, 4, 6)], axis=2).reshape(n, -1)[:, :L]
+    codes = np.ascontiguousarray(codes, dtype=np.uint8)
+    d = min(I, 15)
+    wins = [int(h - I) if h - I > -d else -d for h in diag[:n]]
+    wref = [np.ascontiguousarray(gpad[pad + w:pad + w + L + 2 * I + 1]) for w in wins]
+    wmask = [np.ascontiguousarray(mpad[pad + w:pad + w + L + 2 * I + 1]) for w in wins]
+    outs = [OrcAlignOut() for _ in range(n)]
+    t0 = time.perf_counter()
+    for k in range(n):
+        o.orc_align(P(codes[k].ctypes.data), P(qlev.ctypes.data), L, P(wref[k].ctypes.data), P(wmask[k].ctypes.data),
+                    I, C.byref(op), op.t, C.byref(outs[k]))
+    dt = time.perf_counter() - t0
+    same = True
+    for k in range(n):
+        a = outs[k]
+        if (int(res["score"][k]), int(res["best_j"][k]), int(res["tb_len"][k])) != (a.score, a.best_j, a.tb_len if a.score > op.t else 0):
+            same = False
+            break
+        if a.score > op.t:
+            got = bytes(((tb[k][i >> 1] >> (0 if i & 1 else 4)) & 0xF) for i in range(a.tb_len))
+            if got != bytes(a.tb[:a.tb_len]):
+                same = False
+                break
+    return {"value": round(n * L * (2 * I + 1) / dt / 1e9, 4), "unit": "GCUPS", "cores": 1, "kind": "port",
+            "sample": f"orc_align (the oracle's restatement of alignment__align + alignment__traceback) on the first "
+                      f"{n} records, one thread, {dt:.1f}s", "records_per_s": round(n / dt, 1),
+            "gpu_records_identical": same}
 
 
 SEEDS_N = "####--#-#--##--####;#####--###-#-###;###-##-#######"   # seed.h:29 (nucleotide defaults)
@@ -489,6 +582,7 @@ def main():
     ap.add_argument("--no-map", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--percall", type=int, default=20000)
+    ap.add_argument("--dp-records", type=int, default=2_000_000, help="records of the DP stage measurement")
     ap.add_argument("--config", default="default", choices=["default", "1", "3", "4", "5"],
                     help="default: filter headline + config-1 mapping; 1/3/4: that mapping workload alone")
     ap.add_argument("--reads-c3", type=int, default=5_000_000)
@@ -569,6 +663,14 @@ def main():
                          for c in sel])
         spot = bool(np.array_equal(scores[sel].astype(np.int64), want))
         log(f"spot check vs oracle on 2000 candidates: {'identical' if spot else 'MISMATCH'}")
+    # stage (d): the DP + traceback of the true-origin passes
+    n_dp = min(a.dp_records, a.reads)
+    dp_ms, dp_res, dp_tb, dp_rs, dp_diag = gpu_align_stage(lib, g, reads, pos, stride, L, 7, d, n_dp, max(1, a.steps // 4))
+    dp_cells = n_dp * L * 15
+    dp_gcups = dp_cells / (dp_ms * 1e-3) / 1e9
+    n_tb = int((dp_res["tb_len"] > 0).sum())
+    log(f"align stage: {n_dp} records in {dp_ms:.2f} ms = {dp_gcups:.1f} GCUPS ({n_dp / dp_ms / 1e3:.1f} M records/s), "
+        f"{n_tb} tracebacks")
     pc_dt, _ = percall_filter(lib, g, reads, pos, stride, L, W, min(a.percall, a.reads))
     pc_gcups = 2 * min(a.percall, a.reads) * L * 2 * W / pc_dt / 1e9
     log(f"per-call drop-in row: {pc_dt:.2f}s for {min(a.percall, a.reads)} calls, {pc_gcups:.3f} GCUPS")
@@ -586,6 +688,22 @@ def main():
                 nc2, t2, _ = cb["avx2"]
                 cpu["avx2_gcups"] = round(nc2 * L * 2 * W / t2 / 1e9, 3)
             log(f"cpu baseline: {cpu['value']} GCUPS on {cores} threads")
+    dp_cpu = None
+    if not a.no_cpu:
+        dp_cpu = cpu_align_baseline(g, reads, L, 7, dp_rs, dp_diag, dp_res, dp_tb)
+        if dp_cpu:
+            log(f"align stage cpu baseline: {dp_cpu['value']} GCUPS on 1 thread; GPU records identical on the sample: "
+                f"{dp_cpu['gpu_records_identical']}")
+    dp_achieved = dp_gcups * VALU_INSTR_PER_DP_CELL_I7_L75 / 1e3
+    align_stage = {
+        "workload": f"DP + traceback of {n_dp} filter passes (the true-origin windows of the filter workload), "
+                    f"L={L}, -i 7 (15 band columns), -t 115, levels 3",
+        "kernel": "k_align<7,false>", "records": n_dp, "tracebacks": n_tb, "ms": round(dp_ms, 3),
+        "gcups": round(dp_gcups, 1), "records_per_s": round(n_dp / (dp_ms * 1e-3), 1),
+        "roofline": {"bound": "valu", "achieved": round(dp_achieved, 3), "peak": VALU_PEAK_INT32,
+                     "unit": "T VALU lane-instr/s", "frac": round(dp_achieved / VALU_PEAK_INT32, 4),
+                     "work_per_cell": VALU_INSTR_PER_DP_CELL_I7_L75},
+        "cpu_baseline": dp_cpu}
     achieved = gcups * VALU_INSTR_PER_CELL_W8_L75 / 1e3          # T lane-instructions/s
     algorithmic = gcups * OPS_PER_CELL / 1e3                       # T reference-ops/s
     line = {
@@ -612,6 +730,7 @@ def main():
         "percall_gcups": round(pc_gcups, 4),
         "batched_over_percall": round(gcups / pc_gcups, 1),
         "spot_check_identical": spot,
+        "align_stage": align_stage,
         "mapping": mapping,
     }
     print(json.dumps(line), flush=True)
