@@ -646,10 +646,11 @@ def main():
     gcups = cells / (ms * 1e-3) / 1e9
     log(f"filter: {ms:.3f} ms/launch (events), {wall * 1e3:.3f} ms/step (wall), {gcups:.1f} GCUPS, "
         f"pass rate {passf.mean():.3f}")
-    # spot check against the CPU restatement (the full parity tests live in tests/)
+    # part of the CPU-baseline leg: a spot check against the CPU restatement (the full parity tests
+    # live in tests/)
     spot = None
     orc = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
-    if os.path.exists(orc):
+    if not a.no_cpu and os.path.exists(orc):
         from storm_amd import synth
         ol = C.CDLL(orc)
         ol.orc_filter_score.restype = C.c_int
