@@ -82,7 +82,7 @@ int ensure_ref_resident(storm_mapper* m, size_t r) {
 
 static void release_ref(storm_mapper* m, size_t r) {
   RefState& rs = *m->res->rstate[r];
-  rs.g.lo.release(); rs.g.hi.release(); rs.g.mask.release();
+  rs.g.lh.release(); rs.g.mask.release();
   rs.idx.clear();
   rs.resident = false;
 }
@@ -797,7 +797,7 @@ int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback,
     // one graph per reference: reset + fused kernel, replayed for every batch with the same
     // buffers and scoring (the kernel arguments are part of the captured graph)
     FusedArgs a = fused_args(m, r, tune.group);
-    std::vector<const void*> sig = {a.g.lo, a.r.planes, a.r.qual, a.r.len, a.slots, a.cells, a.scratch, a.tbbuf, a.work,
+    std::vector<const void*> sig = {a.g.lh, a.r.planes, a.r.qual, a.r.len, a.slots, a.cells, a.scratch, a.tbbuf, a.work,
                                     a.seeds.s[0].first, (const void*)(intptr_t)tune.group, (const void*)(intptr_t)tune.grid,
                                     (const void*)(intptr_t)a.scratch_stride, (const void*)(intptr_t)a.sense_mask};
     const int* scw = (const int*)&a.sc;

@@ -67,7 +67,10 @@ struct DPLane {
     wi = wbase >> 5;
     sh = (int)(wbase & 31);
 #pragma unroll
-    for (int q = 0; q < NW; ++q) { gl[q] = g.lo[wi + q]; gh[q] = g.hi[wi + q]; gm[q] = g.mask[wi + q]; }
+    for (int q = 0; q < NW; ++q) {
+      const uint2 p = g.lh[wi + q];
+      gl[q] = p.x; gh[q] = p.y; gm[q] = g.mask[wi + q];
+    }
     rlo = pl[0];
     rhi = pl[RW];
     rq = has_qual ? qp[0] : 0xffffffffu;
@@ -124,8 +127,9 @@ struct DPLane {
       ++wi;
 #pragma unroll
       for (int q = 0; q + 1 < NW; ++q) { gl[q] = gl[q + 1]; gh[q] = gh[q + 1]; gm[q] = gm[q + 1]; }
-      gl[NW - 1] = g.lo[wi + NW - 1];
-      gh[NW - 1] = g.hi[wi + NW - 1];
+      const uint2 p = g.lh[wi + NW - 1];
+      gl[NW - 1] = p.x;
+      gh[NW - 1] = p.y;
       gm[NW - 1] = g.mask[wi + NW - 1];
     }
     if ((i & 31) == 0) { rlo = pl[i >> 5]; rhi = pl[RW + (i >> 5)]; }

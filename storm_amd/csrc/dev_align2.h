@@ -81,7 +81,10 @@ struct DPHalf {
     wi = wbase >> 5;
     sh = (int)(wbase & 31);
 #pragma unroll
-    for (int q = 0; q < 3; ++q) { gl[q] = g.lo[wi + q]; gh[q] = g.hi[wi + q]; gm[q] = g.mask[wi + q]; }
+    for (int q = 0; q < 3; ++q) {
+      const uint2 p = g.lh[wi + q];
+      gl[q] = p.x; gh[q] = p.y; gm[q] = g.mask[wi + q];
+    }
     rlo = pl[0];
     rhi = pl[RW];
     rq = r.has_qual ? qp[0] : 0xffffffffu;
@@ -93,8 +96,9 @@ struct DPHalf {
       ++wi;
       gl[0] = gl[1]; gh[0] = gh[1]; gm[0] = gm[1];
       gl[1] = gl[2]; gh[1] = gh[2]; gm[1] = gm[2];
-      gl[2] = g.lo[wi + 2];
-      gh[2] = g.hi[wi + 2];
+      const uint2 p = g.lh[wi + 2];
+      gl[2] = p.x;
+      gh[2] = p.y;
       gm[2] = g.mask[wi + 2];
     }
     if ((i & 31) == 0) { rlo = pl[i >> 5]; rhi = pl[RW + (i >> 5)]; }

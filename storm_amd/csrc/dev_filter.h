@@ -62,10 +62,11 @@ __device__ __forceinline__ void filter_pair(const GenomeDev& g, const uint32_t* 
   for (int blk = 0; blk * 32 < Lmax; ++blk) {
     // genome planes: 64 bits starting at window code 32*blk of each candidate
     const int64_t ba = xa + 32 * blk, bb = xb + 32 * blk;
-    const uint32_t A0l = gbit_word(g.lo, ba), A1l = gbit_word(g.lo, ba + 32);
-    const uint32_t A0h = gbit_word(g.hi, ba), A1h = gbit_word(g.hi, ba + 32);
-    const uint32_t B0l = gbit_word(g.lo, bb), B1l = gbit_word(g.lo, bb + 32);
-    const uint32_t B0h = gbit_word(g.hi, bb), B1h = gbit_word(g.hi, bb + 32);
+    uint32_t A0l, A0h, A1l, A1h, B0l, B0h, B1l, B1h;
+    gbit_pair(g.lh, ba, A0l, A0h);
+    gbit_pair(g.lh, ba + 32, A1l, A1h);
+    gbit_pair(g.lh, bb, B0l, B0h);
+    gbit_pair(g.lh, bb + 32, B1l, B1h);
     const uint32_t ral = pa[blk], rah = pa[RW + blk], rbl = pb[blk], rbh = pb[RW + blk];
     const int rows = (Lmax - 32 * blk) < 32 ? (Lmax - 32 * blk) : 32;
     for (int t = 0; t < rows; ++t) {

@@ -36,11 +36,10 @@ int upload_genome(const uint8_t* code, const uint8_t* mask, int64_t size, DevGen
     HIPOK(dmask.ensure((size_t)size));
     HIPOK(hipMemcpyAsync(dmask.p, mask, (size_t)size, hipMemcpyHostToDevice, st));
   }
-  HIPOK(g.lo.ensure(nw * 4));
-  HIPOK(g.hi.ensure(nw * 4));
+  HIPOK(g.lh.ensure(nw * 8));
   HIPOK(g.mask.ensure(nw * 4));
-  HIPOK(launch_genome_planes(dcode.as<uint8_t>(), mask ? dmask.as<uint8_t>() : nullptr, size, nw, g.lo.as<uint32_t>(),
-                             g.hi.as<uint32_t>(), g.mask.as<uint32_t>(), st));
+  HIPOK(launch_genome_planes(dcode.as<uint8_t>(), mask ? dmask.as<uint8_t>() : nullptr, size, nw, g.lh.as<uint2>(),
+                             g.mask.as<uint32_t>(), st));
   HIPOK(hipStreamSynchronize(st));
   g.size = size;
   if (keep_code) *keep_code = std::move(dcode);

@@ -30,7 +30,7 @@ __global__ void __launch_bounds__(256) k_unpack_codes(const uint8_t* __restrict_
 // [32 w - GPAD, 32 w - GPAD + 32).  Codes outside [0, size) are 0 and masked.
 __global__ void __launch_bounds__(256) k_genome_planes(const uint8_t* __restrict__ code,
                                                        const uint8_t* __restrict__ mask, int64_t size,
-                                                       int64_t nw, uint32_t* lo, uint32_t* hi, uint32_t* mk) {
+                                                       int64_t nw, uint2* lh, uint32_t* mk) {
   const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nw) return;
   const int64_t p0 = w * 32 - GPAD;
@@ -47,8 +47,7 @@ __global__ void __launch_bounds__(256) k_genome_planes(const uint8_t* __restrict
       m |= 1u << b;
     }
   }
-  lo[w] = l;
-  hi[w] = h;
+  lh[w] = make_uint2(l, h);
   mk[w] = m;
 }
 
@@ -123,10 +122,10 @@ hipError_t launch_unpack_codes(const uint8_t* packed, int64_t size, uint8_t* cod
   return hipGetLastError();
 }
 
-hipError_t launch_genome_planes(const uint8_t* code, const uint8_t* mask, int64_t size, int64_t nw, uint32_t* lo,
-                                uint32_t* hi, uint32_t* mk, hipStream_t st) {
+hipError_t launch_genome_planes(const uint8_t* code, const uint8_t* mask, int64_t size, int64_t nw, uint2* lh,
+                                uint32_t* mk, hipStream_t st) {
   if (nw <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_genome_planes, blocks_for(nw), dim3(256), 0, st, code, mask, size, nw, lo, hi, mk);
+  hipLaunchKernelGGL(k_genome_planes, blocks_for(nw), dim3(256), 0, st, code, mask, size, nw, lh, mk);
   return hipGetLastError();
 }
 

@@ -60,9 +60,9 @@ double now_s();
 
 // genome bit planes of one reference sequence
 struct DevGenome {
-  DevMem lo, hi, mask;
+  DevMem lh, mask;   // {lo, hi} word pairs and the mask plane
   int64_t size = 0;
-  GenomeDev view() const { return GenomeDev{lo.as<uint32_t>(), hi.as<uint32_t>(), mask.as<uint32_t>(), size}; }
+  GenomeDev view() const { return GenomeDev{lh.as<uint2>(), mask.as<uint32_t>(), size}; }
 };
 
 struct DevIndex {

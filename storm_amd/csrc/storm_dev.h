@@ -1,9 +1,12 @@
 // storm_dev.h -- device-side data layout and helpers shared by the gfx950 kernels.
 //
 // HBM layout (all little-endian 32-bit words):
-//   genome   : three bit planes (lo = code bit 0, hi = code bit 1, mask = 1 when masked), each
-//              with GPAD codes of padding before code 0 and after the last code.  Padding codes
-//              are 0 (A) and masked, as in the reference buffers (reference_data.c:75-88,184-188).
+//   genome   : bit planes of 32 codes per word (lo = code bit 0, hi = code bit 1, mask = 1 when
+//              masked), with GPAD codes of padding before code 0 and after the last code.  lo and
+//              hi are interleaved ({lo, hi} as one 8-byte pair per 32 codes), so the window of a
+//              filter candidate is one contiguous run of pairs; mask is a plane of its own (only the
+//              DP reads it).  Padding codes are 0 (A) and masked, as in the reference buffers
+//              (reference_data.c:75-88,184-188).
 //   reads    : per read and strand, RW words of the lo plane then RW words of the hi plane
 //              (row r, strand s at (2r+s)*2*RW), plus QW words of 2-bit quality levels.
 //   index    : per seed, first[4^w + 1] and map[] (CSR, index.c:157-188), int32.
@@ -23,10 +26,9 @@ constexpr int MAX_SEED_WEIGHT = 14; // first[] holds 4^w + 1 entries
 constexpr int MAX_I = 31;           // largest -i the DP kernel accepts
 
 struct GenomeDev {
-  const uint32_t* lo;
-  const uint32_t* hi;
+  const uint2* lh;          // {lo, hi} word pairs
   const uint32_t* mask;
-  int64_t size;       // codes
+  int64_t size;             // codes
 };
 
 struct ReadsDev {
@@ -63,17 +65,20 @@ struct ScoreDev {
   int t;                                // -t
 };
 
-__device__ __forceinline__ uint32_t gbit_word(const uint32_t* plane, int64_t bit) {
-  // 32 bits of `plane` starting at bit index `bit` (bit >= 0)
-  int64_t w = bit >> 5;
-  uint32_t s = (uint32_t)(bit & 31);
-  return __builtin_amdgcn_alignbit(plane[w + 1], plane[w], s);
+// 32 bits of the lo and hi planes starting at bit index `bit` (bit >= 0)
+__device__ __forceinline__ void gbit_pair(const uint2* lh, int64_t bit, uint32_t& lo, uint32_t& hi) {
+  const int64_t w = bit >> 5;
+  const uint32_t s = (uint32_t)(bit & 31);
+  const uint2 a = lh[w], b = lh[w + 1];
+  lo = __builtin_amdgcn_alignbit(b.x, a.x, s);
+  hi = __builtin_amdgcn_alignbit(b.y, a.y, s);
 }
 
 __device__ __forceinline__ int gcode(const GenomeDev& g, int64_t pos) {
   int64_t b = pos + GPAD;
-  uint32_t l = (g.lo[b >> 5] >> (b & 31)) & 1u;
-  uint32_t h = (g.hi[b >> 5] >> (b & 31)) & 1u;
+  const uint2 p = g.lh[b >> 5];
+  uint32_t l = (p.x >> (b & 31)) & 1u;
+  uint32_t h = (p.y >> (b & 31)) & 1u;
   return (int)(l | (h << 1));
 }
 

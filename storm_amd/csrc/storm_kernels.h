@@ -206,8 +206,8 @@ hipError_t sort_tags(uint64_t* k0, uint64_t* k1, int32_t* v0, int32_t* v1, int64
 
 // genome planes and the device index build (k_index.hip)
 hipError_t launch_unpack_codes(const uint8_t* packed, int64_t size, uint8_t* code, hipStream_t st);
-hipError_t launch_genome_planes(const uint8_t* code, const uint8_t* mask, int64_t size, int64_t nw, uint32_t* lo,
-                                uint32_t* hi, uint32_t* mk, hipStream_t st);
+hipError_t launch_genome_planes(const uint8_t* code, const uint8_t* mask, int64_t size, int64_t nw, uint2* lh,
+                                uint32_t* mk, hipStream_t st);
 hipError_t launch_index_keys(const uint8_t* code, int64_t map_size, const int2* regions, int n_regions,
                              const SeedDev& seed, uint32_t ks, uint32_t* keys, int32_t* vals, hipStream_t st);
 hipError_t launch_index_runs(const uint32_t* keys, int64_t n, int32_t* st_, int32_t* en, hipStream_t st);
