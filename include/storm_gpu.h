@@ -56,6 +56,9 @@ typedef struct storm_params {
   const char *seeds;      /* -s, ';'-separated spaced seeds; NULL = reference defaults (seed.h:29-32) */
   int greedy;             /* -G: ordered mode places each read at its best hit (genome_map.c:186-232)
                            * instead of the contextual placement (genome_map.c:432-536) */
+  const char *unmapped_path;   /* -O: FASTQ of the reads without any hit, in read order
+                                * (hit_map__generate_unmapped_FASTQ_output, hit_map.c:514-578,
+                                * run.c:1078-1086); NULL = none */
 } storm_params;
 
 void storm_params_default(storm_params *p, int color);

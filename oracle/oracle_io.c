@@ -704,6 +704,35 @@ void orc_write_sam(FILE *out, const orc_result_t *res, const orc_reads_t *db, co
   if (sym) { for (int r = 0; r < n_refs; ++r) free(sym[r]); free(sym); }
 }
 
+/* hit_map__generate_unmapped_FASTQ_output (hit_map.c:514-578): the reads without any hit */
+void orc_write_unmapped_fastq(FILE *out, const orc_result_t *res, const orc_reads_t *db, const orc_params_t *p) {
+  static const char BASES[4] = {'A', 'C', 'G', 'T'};
+  static const short UB[4] = {2, 7, 10, 93};
+  const int L = db->read_len;
+  for (long id = 0; id < res->n_reads; ++id) {
+    if (res->map[id]) continue;
+    const orc_read_t *rd = &db->reads[id];
+    if (rd->info) fprintf(out, "@%s\n", rd->info); else fprintf(out, "@%ld\n", id);
+    if (p->color) {
+      fputc(BASES[rd->first_base & 3], out);
+      for (int k = 0; k < L; ++k) fputc('0' + orc_nth(rd->seq, k), out);
+    } else {
+      for (int k = 0; k < L; ++k) fputc(BASES[orc_nth(rd->seq, k)], out);
+    }
+    fputc('\n', out);
+    if (rd->info) fprintf(out, "+%s\n", rd->info); else fprintf(out, "+%ld\n", id);
+    if (rd->qual) {
+      if (p->color) { int v = p->qual_base + UB[rd->first_qual]; fputc(v < 126 ? v : 126, out); }
+      for (int q = 0; q < L; ++q) { int v = p->qual_base + UB[orc_nth(rd->qual, q)]; fputc(v < 126 ? v : 126, out); }
+    } else {
+      const int v = p->qual_base + 93 < 126 ? p->qual_base + 93 : 126;   /* MAX_READ_QUALITY */
+      for (int q = 0; q < L + (p->color ? 1 : 0); ++q) fputc(v, out);
+    }
+    fputc('\n', out);
+  }
+  fflush(out);
+}
+
 /* ------------------------------------------------------------ ordered mode (genome_map.c) */
 
 #define GM_NO_CODE 4   /* GENOME_MAP_NO_CODE = CODE_COUNT (genome_map.c:26) */

@@ -17,12 +17,12 @@
 int main(int argc, char **argv) {
   orc_params_t p;
   orc_params_default(&p, ORC_COLOR);
-  const char *genome = NULL, *reads = NULL, *quals = NULL, *outpath = NULL;
+  const char *genome = NULL, *reads = NULL, *quals = NULL, *outpath = NULL, *unmapped = NULL;
   const char *seeds = ORC_COLOR ? "####----##--##-####;####-###--#----####;####-####-####"
                                 : "####--#-#--##--####;#####--###-#-###;###-##-#######";   /* seed.h:29-32 */
   int opt;
   p.unordered_M = 0;
-  while ((opt = getopt(argc, argv, "g:r:q:s:b:m:x:d:e:t:z:i:v:o:l:u:M:N:jFRAG")) != -1) {
+  while ((opt = getopt(argc, argv, "g:r:q:s:b:m:x:d:e:t:z:i:v:o:O:l:u:M:N:jFRAG")) != -1) {
     switch (opt) {
       case 'g': genome = optarg; break;
       case 'r': reads = optarg; break;
@@ -40,6 +40,7 @@ int main(int argc, char **argv) {
         p.simd_diags = p.allowed_indels >= 16 ? 15 : p.allowed_indels;   /* main.c:302-313 */
         break;
       case 'o': outpath = optarg; break;
+      case 'O': unmapped = (!strcmp(optarg, "NULL") || !strcmp(optarg, "null")) ? NULL : optarg; break;   /* main.c:345-355 */
       case 'l':
         p.filter_repeats = 1; p.stdev_distance = atoi(optarg);
         if (p.stdev_distance <= 0) { p.filter_repeats = 0; p.stdev_distance = 0; }
@@ -79,6 +80,12 @@ int main(int argc, char **argv) {
   if (p.unordered_M) orc_write_sam(out, &res, &db, refs, nr, &p, argc, argv);
   else orc_write_sam_ordered(out, &res, &db, refs, nr, &p, argc, argv);
   if (out != stdout) fclose(out);
+  if (unmapped) {   /* run.c:1078-1086 */
+    FILE *uf = fopen(unmapped, "w");
+    if (!uf) { fprintf(stderr, "cannot open %s\n", unmapped); return 1; }
+    orc_write_unmapped_fastq(uf, &res, &db, &p);
+    fclose(uf);
+  }
   fprintf(stderr, "oracle: %ld of %ld reads mapped; candidates %ld, filter passes %ld, accepted %ld\n",
           res.mapped, res.n_reads, res.n_candidates, res.n_filter_pass, res.n_accepted);
   orc_result_free(&res);

@@ -204,6 +204,8 @@ int orc_traceback_to_cigar(const uint8_t *tb, int tb_len, const uint8_t *ref_seq
 void orc_write_sam_ordered(FILE *out, const orc_result_t *res, const orc_reads_t *db,
                            const orc_ref_t *refs, int n_refs, const orc_params_t *p,
                            int argc, char **argv);
+/* -O: FASTQ of the reads without any hit (hit_map.c:514-578) */
+void orc_write_unmapped_fastq(FILE *out, const orc_result_t *res, const orc_reads_t *db, const orc_params_t *p);
 void orc_write_sam(FILE *out, const orc_result_t *res, const orc_reads_t *db,
                    const orc_ref_t *refs, int n_refs, const orc_params_t *p,
                    int argc, char **argv);

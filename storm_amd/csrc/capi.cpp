@@ -1010,6 +1010,8 @@ int storm_mapper_create(int device, const char* genome_path, const storm_params*
   rc = load_resident(genome_path, *params, m->res);
   if (rc) return rc;
   m->p.seeds = m->res->seeds_str.c_str();
+  m->unmapped_str = params->unmapped_path ? params->unmapped_path : "";
+  m->p.unmapped_path = params->unmapped_path ? m->unmapped_str.c_str() : nullptr;
   HIPOK(hipSetDevice(device));
   HIPOK(hipStreamCreateWithFlags(&m->st, hipStreamNonBlocking));
   if (m->res->refs.size() == 1) {   // one sequence: keep genome + indexes resident in HBM
@@ -1075,6 +1077,10 @@ int storm_mapper_map(storm_mapper* m, const storm_reads_view* reads, int mode, c
   std::string err;
   rc = write_sam(sam_path, m->res->refs, hr, slots, m->p, argc, argv, err);
   if (rc) return fail(rc, "%s", err.c_str());
+  if (m->p.unmapped_path) {   // run.c:1078-1086
+    rc = write_unmapped_fastq(m->p.unmapped_path, hr, slots, m->p, err);
+    if (rc) return fail(rc, "%s", err.c_str());
+  }
   local.seconds_sam = now_s() - t0;
   if (stats) *stats = local;
   return STORM_OK;

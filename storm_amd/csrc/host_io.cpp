@@ -700,6 +700,46 @@ void write_blocks(FILE* out, int64_t n, const std::function<void(int64_t, std::s
   }
 }
 
+int write_unmapped_fastq(const char* path, const HostReads& reads, const std::vector<HostSlot>& slots,
+                         const storm_params& p, std::string& err) {
+  FILE* out = fopen(path, "wb");
+  if (!out) { err = std::string("cannot write ") + path; return STORM_ERR_IO; }
+  static const char BASES[4] = {'A', 'C', 'G', 'T'};
+  static const short UB[4] = {2, 7, 10, 93};
+  const bool color = p.color != 0;
+  const char qmax = (char)std::min(p.qual_base + 93, 126);   // MAX_READ_QUALITY (read_quality.h:8)
+  write_blocks(out, reads.n, [&](int64_t id, std::string& line, std::string&, std::string&) {
+    if (slots[(size_t)(4 * id)].ref_id != -1) return;
+    const int L = reads.len[id];
+    const std::string& nm = reads.names[(size_t)id];
+    line += '@';
+    line += nm;
+    line += '\n';
+    const uint8_t* cs = &reads.codes[(size_t)(id * reads.code_stride)];
+    if (color) {
+      line += BASES[reads.first_base[(size_t)id] & 3];
+      for (int k = 0; k < L; ++k) line += (char)('0' + get2(cs, k));
+    } else {
+      for (int k = 0; k < L; ++k) line += BASES[get2(cs, k)];
+    }
+    line += "\n+";
+    line += nm;
+    line += '\n';
+    const bool rq = reads.has_qual && !(!reads.no_qual.empty() && reads.no_qual[(size_t)id]);
+    if (rq) {
+      const uint8_t* qs = &reads.qual[(size_t)(id * reads.qual_stride)];
+      auto qc = [&](int lv) { int v = p.qual_base + UB[lv]; return (char)(v < 126 ? v : 126); };
+      if (color) line += qc(reads.first_qual[(size_t)id]);
+      for (int q = 0; q < L; ++q) line += qc(get2(qs, q));
+    } else {
+      line.append((size_t)(L + (color ? 1 : 0)), qmax);
+    }
+    line += '\n';
+  });
+  if (fclose(out) != 0) { err = std::string("error writing ") + path; return STORM_ERR_IO; }
+  return STORM_OK;
+}
+
 int write_sam(const char* path, const std::vector<HostRef>& refs, const HostReads& reads,
               const std::vector<HostSlot>& slots, const storm_params& p, int argc, char** argv, std::string& err) {
   if (p.max_hits == 0) return write_sam_ordered(path, refs, reads, slots, p, argc, argv, err);

@@ -68,6 +68,11 @@ int write_sam_ordered(const char* path, const std::vector<HostRef>& refs, const 
                       const std::vector<HostSlot>& slots, const storm_params& p, int argc, char** argv,
                       std::string& err);
 
+// -O: the reads whose hit map row is empty (slots[4*r].ref_id == -1), as FASTQ in read order
+// (hit_map__generate_unmapped_FASTQ_output, hit_map.c:514-578)
+int write_unmapped_fastq(const char* path, const HostReads& reads, const std::vector<HostSlot>& slots,
+                         const storm_params& p, std::string& err);
+
 // SAM text shared by the two writers: header lines, MAPQ, and the fields of a mapped record
 struct SamFormat {
   SamFormat(const std::vector<HostRef>& refs, const HostReads& reads, const storm_params& p);

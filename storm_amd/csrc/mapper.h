@@ -185,6 +185,7 @@ struct storm_mapper {
   int device = 0;
   hipStream_t st = nullptr;
   storm_params p{};
+  std::string unmapped_str;    // the -O path p.unmapped_path points to (empty: none)
   std::shared_ptr<storm::Resident> res;
   storm::PassScratch ps;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;

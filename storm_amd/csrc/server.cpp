@@ -549,6 +549,7 @@ int storm_server_submit(storm_server* s, const storm_reads_view* v, const storm_
   std::unique_ptr<Request> r(new Request());
   r->p = *params;
   r->p.seeds = nullptr;
+  r->p.unmapped_path = nullptr;   // a request returns hits, it writes no files
   r->cls = class_of(r->p);
   r->n = v->n_reads;
   r->cs = v->code_stride;

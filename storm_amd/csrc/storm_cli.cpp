@@ -5,6 +5,9 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <string>
 
 #include "../../include/storm_gpu.h"
 
@@ -12,18 +15,45 @@
 #define STORM_CLI_COLOR 0
 #endif
 
+// seed__string_from_file (seed.c:140-179): one or more seeds per line, lines starting with '%'
+// ignored, trailing blanks removed, a ';' appended to a line that does not end with one
+static bool seeds_from_file(const char* path, std::string& out) {
+  std::ifstream f(path);
+  if (!f) return false;
+  out.clear();
+  std::string line;
+  while (std::getline(f, line)) {
+    if (!line.empty() && line[0] == '%') continue;
+    while (!line.empty() && (line.back() == '\r' || line.back() == ' ' || line.back() == '\t')) line.pop_back();
+    out += line;
+    if (!out.empty() && out.back() != ';') out += ';';
+  }
+  return true;
+}
+
 int main(int argc, char** argv) {
   storm_params p;
   storm_params_default(&p, STORM_CLI_COLOR);
   const char *genome = nullptr, *reads = nullptr, *quals = nullptr, *out = "/dev/stdout";
   int mode = STORM_MODE_BATCHED, device = 0, opt;
+  std::string seed_file_text;
   p.max_hits = 0;
-  while ((opt = getopt(argc, argv, "g:r:q:s:b:m:x:d:e:t:z:i:v:o:l:u:M:N:jFRAGPD:")) != -1) {
+  while ((opt = getopt(argc, argv, "g:r:q:s:S:b:m:x:d:e:t:z:i:v:o:O:l:u:M:N:jFRAGPD:")) != -1) {
     switch (opt) {
       case 'g': genome = optarg; break;
       case 'r': reads = optarg; break;
       case 'q': quals = optarg; break;
       case 's': p.seeds = optarg; break;
+      case 'S':   // main.c:252-266 (the reference then asks whether to go on with the default seeds)
+        if (!seeds_from_file(optarg, seed_file_text)) {
+          fprintf(stderr, "%s: unable to read seeds input file %s\n", argv[0], optarg);
+          return 1;
+        }
+        p.seeds = seed_file_text.c_str();
+        break;
+      case 'O':   // main.c:345-355
+        p.unmapped_path = (!strcmp(optarg, "NULL") || !strcmp(optarg, "null")) ? nullptr : optarg;
+        break;
       case 'b': p.qual_base = atoi(optarg); break;
       case 'm': p.match = atoi(optarg); break;
       case 'x': p.mismatch = atoi(optarg); break;

@@ -26,7 +26,7 @@ class Params(C.Structure):
     _fields_ = [(n, C.c_int) for n in (
         "color", "match", "mismatch", "gap_open", "gap_extend", "allowed_indels", "t", "z", "qual_base",
         "stdev_distance", "sense_mask", "max_hits", "uniqueness", "print_all", "ignore_lowercase",
-        "strict_length")] + [("seeds", C.c_char_p), ("greedy", C.c_int)]
+        "strict_length")] + [("seeds", C.c_char_p), ("greedy", C.c_int), ("unmapped_path", C.c_char_p)]
 
 
 class MapStats(C.Structure):
@@ -136,7 +136,7 @@ def params(color: int = 0, **kw) -> Params:
     p = Params()
     load().storm_params_default(C.byref(p), color)
     for k, v in kw.items():
-        if k == "seeds":
+        if k in ("seeds", "unmapped_path"):
             v = v.encode() if isinstance(v, str) else v
         setattr(p, k, v)
     return p

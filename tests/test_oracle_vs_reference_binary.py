@@ -149,3 +149,32 @@ def test_oracle_equals_reference_binary_hash_quality_lines(tmp_path, oracle_buil
     recs = [line.split("\t") for line in a if not line.startswith("@")]
     assert len(recs) > 5 and any(f[10] == "*" for f in recs) and any(f[10] != "*" for f in recs)
     assert a == b
+
+
+# -O: the FASTQ of the reads without any hit (hit_map.c:514-578), with -M and in ordered mode;
+# a high -t leaves many reads unmapped
+UNMAPPED_CASES = [
+    ("n", "plain", 800, 50, 3, ["-M", "1", "-t", "200"], 61, False),
+    ("n", "plain", 600, 50, 3, ["-t", "190"], 62, True),
+    ("c", "plain", 800, 48, 3, ["-M", "2", "-t", "170"], 63, False),
+    ("c", "plain", 600, 48, 3, ["-t", "160", "-G"], 64, True),
+]
+
+
+@pytest.mark.parametrize("mode,kind,nreads,L,i,extra,seed,fasta", UNMAPPED_CASES,
+                         ids=[f"{c[0]}-{c[6]}{'-fasta' if c[7] else ''}" for c in UNMAPPED_CASES])
+def test_oracle_unmapped_fastq_equals_reference_binary(mode, kind, nreads, L, i, extra, seed, fasta, tmp_path,
+                                                        oracle_built):
+    tmp = str(tmp_path)
+    gpath, rargs = make_inputs(tmp, mode, kind, nreads, L, seed, fasta_reads=fasta)
+    ru, ou = os.path.join(tmp, "ref_un.fq"), os.path.join(tmp, "orc_un.fq")
+    ref = os.path.join(REF, "storm-nucleotide" if mode == "n" else "storm-color")
+    orc = os.path.join(ROOT, "oracle", "_build", "oracle-nucleotide" if mode == "n" else "oracle-color")
+    common = ["-g", gpath] + rargs + ["-i", str(i), "-s", SEEDS_N if mode == "n" else SEEDS_C] + extra
+    subprocess.run([ref] + common + ["-o", os.path.join(tmp, "ref.sam"), "-O", ru, "-v", "0"], check=True,
+                   env=dict(os.environ, MALLOC_PERTURB_="255"), stderr=subprocess.DEVNULL)
+    subprocess.run([orc] + common + ["-o", os.path.join(tmp, "orc.sam"), "-O", ou], check=True,
+                   stderr=subprocess.DEVNULL)
+    a, b = open(ru, "rb").read(), open(ou, "rb").read()
+    assert a.count(b"\n") >= 40 and a.count(b"\n") % 4 == 0
+    assert a == b
