@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <unordered_map>
 
 #include "host_io.h"
 
@@ -27,51 +28,83 @@ inline void bump(uint32_t& w, int code) {
   w = (w & ~(63u << (6 * code))) | (u << (6 * code));
 }
 
-// consensus counters of one reference sequence, positions 0..size (an insertion can follow the
-// last code): a position "exists" once a read was placed at it or counted over it; counters of a
-// position that does not exist are never read
+// Consensus counters of one reference sequence, positions 0..size (an insertion can follow the
+// last code).  A position "exists" once a read was placed at it or counted over it; counters of a
+// position that does not exist are never read.  Per position the reference keeps one counter word
+// for the aligned codes and one per insertion offset 0..I-1, and every non-insertion cell of a
+// placed read adds a gap (NO_CODE) to all I insertion counters (genome_map.c:512-515).  Each 6-bit
+// counter is independent modulo 64, so that full-range gap count is kept once per position
+// (`gaps`), and the few positions an insertion touches keep their per-offset counts in a side
+// table; an insertion counter's gap count is the sum of the two, modulo 64.  The aligned counters
+// use bits 0-29 of a word whose bit 31 marks the position as existing.
 struct Consensus {
+  static constexpr int P = 1 << PAGE_BITS;
+  static constexpr uint32_t EXISTS = 0x80000000u;
   struct Page {
-    std::vector<uint8_t> exists;
-    std::vector<uint32_t> aligned, inserted;   // inserted: I words per position
+    uint32_t word[P];
+    uint8_t gaps[P];
+  };
+  struct Ins {
+    uint8_t code[32][4];   // per insertion offset, counts of the inserted codes 0..3 (mod 64)
+    uint8_t gap[32];       // per insertion offset, gap counts beyond `gaps` (mod 64)
   };
   int64_t size = 0;
   int I = 0;
   std::vector<std::unique_ptr<Page>> pages;
+  std::unordered_map<int64_t, Ins> ins;
 
-  Consensus(int64_t size_, int I_) : size(size_), I(I_), pages((size_t)((size_ + 1 + (1 << PAGE_BITS) - 1) >> PAGE_BITS)) {}
+  Consensus(int64_t size_, int I_) : size(size_), I(I_), pages((size_t)((size_ + 1 + P - 1) >> PAGE_BITS)) {}
 
   bool in_range(int64_t pos) const { return pos >= 0 && pos <= size; }
-  Page* page(int64_t pos) const { return pages[(size_t)(pos >> PAGE_BITS)].get(); }
+  const Page* page(int64_t pos) const { return pages[(size_t)(pos >> PAGE_BITS)].get(); }
   // creates the position (genome_map__create_position_data, genome_map.c:68-99)
-  void touch(int64_t pos) {
+  Page* touch(int64_t pos) {
     auto& pg = pages[(size_t)(pos >> PAGE_BITS)];
-    if (!pg) {
-      pg.reset(new Page());
-      pg->exists.assign((size_t)1 << PAGE_BITS, 0);
-      pg->aligned.assign((size_t)1 << PAGE_BITS, 0);
-      pg->inserted.assign(((size_t)1 << PAGE_BITS) * (size_t)I, 0);
+    if (!pg) pg.reset(new Page());   // value-initialised: all zero
+    pg->word[pos & (P - 1)] |= EXISTS;
+    return pg.get();
+  }
+  // Whether the consensus code at pos (offset -1: aligned, else insertion offset), asked with x as
+  // the default code, is x (genome_map__get_direct_consensus_code, genome_map.c:308-330, as its
+  // callers use it, genome_map.c:373-420: every call passes the code it then compares with).  The
+  // answer is x when the position does not exist; otherwise the code with the largest count, x on
+  // a tie, and NO_CODE when every count is zero: x exactly when x's count is the largest and not
+  // zero, or every count is zero and x is NO_CODE.  pg = page(pos).
+  bool agrees(const Page* pg, int64_t pos, int offset, int x) const {
+    const size_t o = (size_t)(pos & (P - 1));
+    if (!pg || !(pg->word[o] & EXISTS)) return true;
+    unsigned n[NO_CODE + 1];
+    if (offset < 0) {
+      const uint32_t w = pg->word[o];
+      for (int c = 0; c <= NO_CODE; ++c) n[c] = cnt(w, c);
+    } else {
+      auto it = ins.find(pos);
+      for (int c = 0; c < NO_CODE; ++c) n[c] = it == ins.end() ? 0u : it->second.code[offset][c];
+      n[NO_CODE] = (pg->gaps[o] + (it == ins.end() ? 0u : it->second.gap[offset])) & 63u;
     }
-    pg->exists[(size_t)(pos & ((1 << PAGE_BITS) - 1))] = 1;
+    unsigned m = n[0];
+    for (int c = 1; c <= NO_CODE; ++c) m = n[c] > m ? n[c] : m;
+    return m == 0 ? x == NO_CODE : n[x] == m;
   }
-  // the consensus code at pos (offset -1: aligned, else insertion offset) with reference_code as
-  // the default and the tie winner (genome_map__get_direct_consensus_code, genome_map.c:308-330)
-  int code(int64_t pos, int offset, int reference_code) const {
-    const Page* pg = page(pos);
-    const size_t o = (size_t)(pos & ((1 << PAGE_BITS) - 1));
-    if (!pg || !pg->exists[o]) return reference_code;
-    const uint32_t w = offset < 0 ? pg->aligned[o] : pg->inserted[o * (size_t)I + (size_t)offset];
-    int best = reference_code;
-    for (int c = 0; c <= NO_CODE; ++c)
-      if (cnt(w, c) > cnt(w, best)) best = c;
-    return cnt(w, best) == 0 ? NO_CODE : best;
-  }
-  // genome_map__update_consensus_code (genome_map.c:340-349)
-  void count(int64_t pos, int offset, int code) {
+  // genome_map__update_consensus_code (genome_map.c:340-349) for an aligned code
+  void count_aligned(int64_t pos, int code) { bump(touch(pos)->word[pos & (P - 1)], code); }
+  // ... for an inserted code at offset q
+  void count_inserted(int64_t pos, int q, int code) {
     touch(pos);
-    Page* pg = page(pos);
-    const size_t o = (size_t)(pos & ((1 << PAGE_BITS) - 1));
-    bump(offset < 0 ? pg->aligned[o] : pg->inserted[o * (size_t)I + (size_t)offset], code);
+    uint8_t& c = ins[pos].code[q][code];
+    c = (uint8_t)((c + 1) & 63);
+  }
+  // the gap counts a cell adds at offsets q0..I-1 (genome_map.c:512-515)
+  void count_gaps(int64_t pos, int q0) {
+    if (q0 >= I) return;
+    Page* pg = touch(pos);
+    if (q0 == 0) {
+      uint8_t& g = pg->gaps[pos & (P - 1)];
+      g = (uint8_t)((g + 1) & 63);
+      return;
+    }
+    Ins& e = ins[pos];
+    for (int q = q0; q < I; ++q) e.gap[q] = (uint8_t)((e.gap[q] + 1) & 63);
   }
 };
 
@@ -108,6 +141,7 @@ int write_sam_ordered(const char* path, const std::vector<HostRef>& refs, const 
                       std::string& err) {
   const bool color = p.color != 0;
   const int I = p.allowed_indels;   // the hit map's indel_count (hit_map__create, run.c:818)
+  if (I < 0 || I > 31) { err = "-i outside 0..31"; return STORM_ERR_INVALID; }
   std::vector<Placement> placed;
   auto out_of_range = [&](int64_t id, int64_t pos) {
     err = "read " + std::to_string(id) + ": genome map position " + std::to_string(pos) + " outside the sequence";
@@ -130,13 +164,13 @@ int write_sam_ordered(const char* path, const std::vector<HostRef>& refs, const 
     // mapped reads by decreasing best score, ties in read order: the reference's LSD radix sort
     // over hex digits with reversed buckets (genome_map.c:252-298) orders by the score's unsigned
     // 32-bit value
-    std::vector<int32_t> order;
+    std::vector<uint64_t> keyed;
     for (int64_t id = 0; id < reads.n; ++id)
-      if (slots[(size_t)(4 * id)].ref_id != -1) order.push_back((int32_t)id);
-    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-      return (uint32_t)slots[(size_t)(4 * a)].score > (uint32_t)slots[(size_t)(4 * b)].score;
-    });
-    for (const int32_t id : order) {
+      if (slots[(size_t)(4 * id)].ref_id != -1)
+        keyed.push_back(((uint64_t)(0xffffffffu - (uint32_t)slots[(size_t)(4 * id)].score) << 32) | (uint64_t)id);
+    std::sort(keyed.begin(), keyed.end());   // keys are unique: this is the stable order
+    for (const uint64_t kv : keyed) {
+      const int32_t id = (int32_t)(kv & 0xffffffffu);
       const HostSlot* row = &slots[(size_t)(4 * id)];
       // adjusted score of each hit against the current consensus (genome_map.c:359-425)
       int adj[4] = {kMinScore, kMinScore, kMinScore, kMinScore};
@@ -146,25 +180,21 @@ int write_sam_ordered(const char* path, const std::vector<HostRef>& refs, const 
         const Consensus& cs = cons[(size_t)h.ref_id];
         const HostRef& ref = refs[(size_t)h.ref_id];
         int score = 0, indel = 0;
-        int64_t pos = h.ref_start;
+        int64_t pos = h.ref_start, pg_id = -1;
+        const Consensus::Page* pg = nullptr;
         for (int c = 0; c < h.tb_len; ++c) {
           const int code = tb_code(h, c);
           const int t = pair_type(code, color);
           if (t < 0) continue;
           if (!cs.in_range(pos)) return out_of_range(id, pos);
+          if ((pos >> PAGE_BITS) != pg_id) { pg_id = pos >> PAGE_BITS; pg = cs.page(pos); }
           if (t == 0xC) {
-            score += ((code & 3) == cs.code(pos, indel, code & 3)) ? p.match : p.mismatch;
+            score += cs.agrees(pg, pos, indel, code & 3) ? p.match : p.mismatch;
             ++indel;
             continue;
           }
-          if (t == 0x3) {
-            const int rc = ref_code(ref, pos);
-            score += (rc == cs.code(pos, -1, rc)) ? p.match : p.mismatch;
-          } else if (t == 0x1) {
-            score += (cs.code(pos, -1, NO_CODE) == NO_CODE) ? p.match : p.mismatch;
-          } else {
-            score += ((code & 3) == cs.code(pos, -1, code & 3)) ? p.match : p.mismatch;
-          }
+          const int x = t == 0x3 ? ref_code(ref, pos) : t == 0x1 ? NO_CODE : (code & 3);
+          score += cs.agrees(pg, pos, -1, x) ? p.match : p.mismatch;
           ++pos;
           indel = 0;
         }
@@ -185,13 +215,13 @@ int write_sam_ordered(const char* path, const std::vector<HostRef>& refs, const 
         const int code = tb_code(h, c);
         if (!cs.in_range(pos)) return out_of_range(id, pos);
         switch (pair_type(code, color)) {
-          case 0xC: ++ins; cs.count(pos, ins, code & 3); break;
-          case 0x3: ins = -1; cs.count(pos, -1, ref_code(ref, pos)); break;
-          case 0x4: case 0x8: ins = -1; cs.count(pos, -1, code & 3); break;
-          case 0x1: ins = -1; cs.count(pos, -1, NO_CODE); break;
+          case 0xC: ++ins; cs.count_inserted(pos, ins, code & 3); break;
+          case 0x3: ins = -1; cs.count_aligned(pos, ref_code(ref, pos)); break;
+          case 0x4: case 0x8: ins = -1; cs.count_aligned(pos, code & 3); break;
+          case 0x1: ins = -1; cs.count_aligned(pos, NO_CODE); break;
           default: break;
         }
-        for (int q = ins + 1; q < I; ++q) cs.count(pos, q, NO_CODE);
+        cs.count_gaps(pos, ins + 1);
         if (ins < 0) ++pos;
       }
     }
