@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <thread>
 #include <unordered_map>
 
 #include "host_io.h"
@@ -57,6 +58,12 @@ struct Consensus {
 
   bool in_range(int64_t pos) const { return pos >= 0 && pos <= size; }
   const Page* page(int64_t pos) const { return pages[(size_t)(pos >> PAGE_BITS)].get(); }
+  // the page holding pos, allocated on first use (positions in it do not exist until touched)
+  Page* page_w(int64_t pos) {
+    auto& pg = pages[(size_t)(pos >> PAGE_BITS)];
+    if (!pg) pg.reset(new Page());   // value-initialised: all zero
+    return pg.get();
+  }
   // creates the position (genome_map__create_position_data, genome_map.c:68-99)
   Page* touch(int64_t pos) {
     auto& pg = pages[(size_t)(pos >> PAGE_BITS)];
@@ -86,9 +93,7 @@ struct Consensus {
     for (int c = 1; c <= NO_CODE; ++c) m = n[c] > m ? n[c] : m;
     return m == 0 ? x == NO_CODE : n[x] == m;
   }
-  // genome_map__update_consensus_code (genome_map.c:340-349) for an aligned code
-  void count_aligned(int64_t pos, int code) { bump(touch(pos)->word[pos & (P - 1)], code); }
-  // ... for an inserted code at offset q
+  // genome_map__update_consensus_code (genome_map.c:340-349) for an inserted code at offset q
   void count_inserted(int64_t pos, int q, int code) {
     touch(pos);
     uint8_t& c = ins[pos].code[q][code];
@@ -169,12 +174,13 @@ int write_sam_ordered(const char* path, const std::vector<HostRef>& refs, const 
       if (slots[(size_t)(4 * id)].ref_id != -1)
         keyed.push_back(((uint64_t)(0xffffffffu - (uint32_t)slots[(size_t)(4 * id)].score) << 32) | (uint64_t)id);
     std::sort(keyed.begin(), keyed.end());   // keys are unique: this is the stable order
-    for (const uint64_t kv : keyed) {
-      const int32_t id = (int32_t)(kv & 0xffffffffu);
+    // adjusted score of each hit of read id against the current consensus (genome_map.c:359-425);
+    // false (bad = the position) when a hit leaves the sequence
+    auto score_read = [&](int32_t id, int* adj, int& best_rank, int64_t& bad) -> bool {
       const HostSlot* row = &slots[(size_t)(4 * id)];
-      // adjusted score of each hit against the current consensus (genome_map.c:359-425)
-      int adj[4] = {kMinScore, kMinScore, kMinScore, kMinScore};
-      int best_rank = -1, best = kMinScore;
+      for (int k = 0; k < 4; ++k) adj[k] = kMinScore;
+      int best = kMinScore;
+      best_rank = -1;
       for (int k = 0; k < 4 && row[k].ref_start != -1; ++k) {
         const HostSlot& h = row[k];
         const Consensus& cs = cons[(size_t)h.ref_id];
@@ -186,7 +192,7 @@ int write_sam_ordered(const char* path, const std::vector<HostRef>& refs, const 
           const int code = tb_code(h, c);
           const int t = pair_type(code, color);
           if (t < 0) continue;
-          if (!cs.in_range(pos)) return out_of_range(id, pos);
+          if (!cs.in_range(pos)) { bad = pos; return false; }
           if ((pos >> PAGE_BITS) != pg_id) { pg_id = pos >> PAGE_BITS; pg = cs.page(pos); }
           if (t == 0xC) {
             score += cs.agrees(pg, pos, indel, code & 3) ? p.match : p.mismatch;
@@ -201,29 +207,136 @@ int write_sam_ordered(const char* path, const std::vector<HostRef>& refs, const 
         adj[k] = score;
         if (score > best) { best = score; best_rank = k; }
       }
-      if (best_rank < 0 || !(p.uniqueness == 0 || adj[0] - adj[1] > p.uniqueness)) continue;
-      // place the read and add its alignment to the consensus (genome_map.c:468-520)
-      const HostSlot& h = row[best_rank];
-      Consensus& cs = cons[(size_t)h.ref_id];
-      const HostRef& ref = refs[(size_t)h.ref_id];
-      int64_t pos = h.ref_start;
-      if (!cs.in_range(pos)) return out_of_range(id, pos);
-      cs.touch(pos);
-      placed.push_back({h.ref_id, pos, id, best_rank});
-      int ins = -1;
+      return true;
+    };
+    // the positions [lo, hi] a hit's traceback reads or counts over: from its start, one position
+    // per aligned or deleted code, and the one after them (insertions sit at the next position)
+    auto span = [&](const HostSlot& h, int64_t& lo, int64_t& hi) {
+      int64_t n = 0;
       for (int c = 0; c < h.tb_len; ++c) {
-        const int code = tb_code(h, c);
-        if (!cs.in_range(pos)) return out_of_range(id, pos);
-        switch (pair_type(code, color)) {
-          case 0xC: ++ins; cs.count_inserted(pos, ins, code & 3); break;
-          case 0x3: ins = -1; cs.count_aligned(pos, ref_code(ref, pos)); break;
-          case 0x4: case 0x8: ins = -1; cs.count_aligned(pos, code & 3); break;
-          case 0x1: ins = -1; cs.count_aligned(pos, NO_CODE); break;
-          default: break;
-        }
-        cs.count_gaps(pos, ins + 1);
-        if (ins < 0) ++pos;
+        const int t = pair_type(tb_code(h, c), color);
+        n += (t >= 0 && t != 0xC);
       }
+      lo = h.ref_start;
+      hi = h.ref_start + n;
+    };
+    // Placement in chunks of reads: the adjusted scores of a chunk are computed in parallel
+    // against the consensus at the chunk's start, then the reads are placed in order; a read any
+    // of whose hits overlaps a position counted by an earlier read of the chunk is scored again
+    // first.  A score that reads no changed counter is the one the sequential order computes.
+    std::vector<std::vector<uint64_t>> dirty(refs.size());
+    for (size_t r = 0; r < refs.size(); ++r) dirty[r].assign((size_t)((refs[r].size + 1 + 64) >> 6) + 1, 0);
+    std::vector<std::pair<int32_t, std::pair<int64_t, int64_t>>> marked;
+    auto is_dirty = [&](int32_t rid, int64_t lo, int64_t hi) {
+      const std::vector<uint64_t>& d = dirty[(size_t)rid];
+      for (int64_t w = lo >> 6; w <= (hi >> 6); ++w) {
+        uint64_t m = d[(size_t)w];
+        if (w == (lo >> 6)) m &= ~0ull << (lo & 63);
+        if (w == (hi >> 6)) m &= (hi & 63) == 63 ? ~0ull : ((1ull << ((hi & 63) + 1)) - 1);
+        if (m) return true;
+      }
+      return false;
+    };
+    auto mark = [&](int32_t rid, int64_t lo, int64_t hi) {
+      std::vector<uint64_t>& d = dirty[(size_t)rid];
+      for (int64_t q = lo; q <= hi; ++q) d[(size_t)(q >> 6)] |= 1ull << (q & 63);
+      marked.push_back({rid, {lo, hi}});
+    };
+    const int64_t n_sorted = (int64_t)keyed.size();
+    const int64_t B = 1 << 15;
+    const int nt = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    std::vector<int> sadj((size_t)(4 * B)), sbest((size_t)B);
+    std::vector<uint8_t> sok((size_t)B);
+    std::vector<int64_t> sbad((size_t)B);
+    for (int64_t s0 = 0; s0 < n_sorted; s0 += B) {
+      const int64_t nb = std::min(B, n_sorted - s0);
+      auto work = [&](int t) {
+        const int64_t per = (nb + nt - 1) / nt, j0 = t * per, j1 = std::min(nb, j0 + per);
+        for (int64_t j = j0; j < j1; ++j)
+          sok[(size_t)j] = score_read((int32_t)(keyed[(size_t)(s0 + j)] & 0xffffffffu), &sadj[(size_t)(4 * j)],
+                                      sbest[(size_t)j], sbad[(size_t)j]);
+      };
+      if (nb >= 4096 && nt > 1) {
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
+        work(0);
+        for (auto& x : th) x.join();
+      } else {
+        for (int64_t j = 0; j < nb; ++j)
+          sok[(size_t)j] = score_read((int32_t)(keyed[(size_t)(s0 + j)] & 0xffffffffu), &sadj[(size_t)(4 * j)],
+                                      sbest[(size_t)j], sbad[(size_t)j]);
+      }
+      constexpr int64_t PD = 8;   // prefetch distance in reads: slot rows at 2 PD, the data they name at PD
+      for (int64_t j = 0; j < nb; ++j) {
+        if (j + 2 * PD < nb) __builtin_prefetch(&slots[(size_t)(4 * (keyed[(size_t)(s0 + j + 2 * PD)] & 0xffffffffu))]);
+        if (j + PD < nb && sbest[(size_t)(j + PD)] >= 0) {
+          const HostSlot& hp = slots[(size_t)(4 * (keyed[(size_t)(s0 + j + PD)] & 0xffffffffu)) + sbest[(size_t)(j + PD)]];
+          __builtin_prefetch(hp.tb.data());
+          const Consensus& cp = cons[(size_t)hp.ref_id];
+          if (hp.ref_start >= 0 && hp.ref_start + hp.tb_len <= cp.size) {
+            for (int64_t q = hp.ref_start; q < hp.ref_start + hp.tb_len; q += 16) {
+              const Consensus::Page* pp = cp.page(q);
+              if (pp) __builtin_prefetch(&pp->word[q & (Consensus::P - 1)], 1);
+            }
+            __builtin_prefetch(&refs[(size_t)hp.ref_id].code[(size_t)hp.ref_start]);
+          }
+        }
+        const int32_t id = (int32_t)(keyed[(size_t)(s0 + j)] & 0xffffffffu);
+        const HostSlot* row = &slots[(size_t)(4 * id)];
+        int* adj = &sadj[(size_t)(4 * j)];
+        int best_rank = sbest[(size_t)j];
+        bool ok = sok[(size_t)j] != 0;
+        int64_t bad = sbad[(size_t)j];
+        bool redo = false;
+        for (int k = 0; k < 4 && row[k].ref_start != -1 && !marked.empty() && !redo; ++k) {
+          int64_t lo, hi;
+          span(row[k], lo, hi);
+          const int64_t sz = refs[(size_t)row[k].ref_id].size;
+          lo = std::max<int64_t>(lo, 0);
+          hi = std::min<int64_t>(hi, sz);
+          redo = lo <= hi && is_dirty(row[k].ref_id, lo, hi);
+        }
+        if (redo) ok = score_read(id, adj, best_rank, bad);
+        if (!ok) return out_of_range(id, bad);
+        if (best_rank < 0 || !(p.uniqueness == 0 || adj[0] - adj[1] > p.uniqueness)) continue;
+        // place the read and add its alignment to the consensus (genome_map.c:468-520)
+        const HostSlot& h = row[best_rank];
+        Consensus& cs = cons[(size_t)h.ref_id];
+        const HostRef& ref = refs[(size_t)h.ref_id];
+        int64_t pos = h.ref_start, pg_id = -1;
+        if (!cs.in_range(pos)) return out_of_range(id, pos);
+        cs.touch(pos);
+        placed.push_back({h.ref_id, pos, id, best_rank});
+        Consensus::Page* pg = nullptr;
+        int ins = -1;
+        for (int c = 0; c < h.tb_len; ++c) {
+          const int code = tb_code(h, c);
+          if (!cs.in_range(pos)) return out_of_range(id, pos);
+          if ((pos >> PAGE_BITS) != pg_id) { pg_id = pos >> PAGE_BITS; pg = cs.page_w(pos); }
+          uint32_t& w = pg->word[pos & (Consensus::P - 1)];
+          switch (pair_type(code, color)) {   // the counts of genome_map.c:488-517, one cell
+            case 0xC: ++ins; cs.count_inserted(pos, ins, code & 3); break;
+            case 0x3: ins = -1; w |= Consensus::EXISTS; bump(w, ref_code(ref, pos)); break;
+            case 0x4: case 0x8: ins = -1; w |= Consensus::EXISTS; bump(w, code & 3); break;
+            case 0x1: ins = -1; w |= Consensus::EXISTS; bump(w, NO_CODE); break;
+            default: break;
+          }
+          if (ins < 0 && I > 0) {   // a gap at every insertion offset
+            w |= Consensus::EXISTS;
+            uint8_t& gp = pg->gaps[pos & (Consensus::P - 1)];
+            gp = (uint8_t)((gp + 1) & 63);
+          } else {
+            cs.count_gaps(pos, ins + 1);
+          }
+          if (ins < 0) ++pos;
+        }
+        mark(h.ref_id, h.ref_start, std::min<int64_t>(pos, cs.size));
+      }
+      for (auto& mr : marked) {
+        std::vector<uint64_t>& d = dirty[(size_t)mr.first];
+        for (int64_t w = mr.second.first >> 6; w <= (mr.second.second >> 6); ++w) d[(size_t)w] = 0;
+      }
+      marked.clear();
     }
   }
   // by reference, then position; reads at one position in placement order
