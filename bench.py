@@ -349,7 +349,8 @@ def cpu_mapping_baseline(lib, d, gpath, pr, kw, color, n_sample, ordered=False):
     return out
 
 
-def map_config(lib, name, gpath, pr, kw, color, n_sample_cpu, sam_reads, skip_plain=False, ordered=False):
+def map_config(lib, name, gpath, pr, kw, color, n_sample_cpu, sam_reads, skip_plain=False, ordered=False,
+               ordered_reads=0):
     """Batched (persistent fused kernel in a hipGraph) and plain (staged kernels, one batch per
     read length) on the same resident mapper; hits returned to host memory; whole-call wall time."""
     t0 = time.perf_counter()
@@ -395,6 +396,17 @@ def map_config(lib, name, gpath, pr, kw, color, n_sample_cpu, sam_reads, skip_pl
         st = m.map_sam(pr, "/dev/null", mode=1, lo=0, hi=k)
         res["sam_writer_reads_per_s"] = round(k / max(st["seconds_sam"], 1e-9), 1)
     m.close()
+    if ordered_reads:
+        # the reference's default output on the same reads: no -M, reads placed on the genome map
+        # (host, after the device hit map), coordinate-sorted SAM
+        k = min(n, ordered_reads)
+        mo = lib.Mapper(gpath, color=color, **dict(kw, max_hits=0))
+        st = mo.map_sam(pr, "/dev/null", mode=1, lo=0, hi=k)
+        mo.close()
+        res["ordered"] = {"reads": k, "gpu_map_s": round(st["seconds_map"], 3),
+                          "genome_map_and_sam_s": round(st["seconds_sam"], 3),
+                          "reads_per_s": round(k / max(st["seconds_map"] + st["seconds_sam"], 1e-9), 1)}
+        log(f"{name}: ordered mode on {k} reads: map {st['seconds_map']:.2f}s, genome map + SAM {st['seconds_sam']:.2f}s")
     if n_sample_cpu:
         cb = cpu_mapping_baseline(lib, os.path.dirname(gpath), gpath, pr, kw, color, n_sample_cpu, ordered)
         res["cpu_baseline"] = cb
@@ -446,7 +458,8 @@ def config4(lib, cpu=True, n_reads=20_000_000, skip_plain=False):
     pr = synth.fast_nucleotide_reads(g, n_reads, 100, 404, sub=0.015, indel=0.002, qlo=15, qhi=40)
     log(f"config4: {n_reads} reads generated in {time.perf_counter() - t0:.1f}s")
     kw = dict(allowed_indels=7, z=100, t=115, max_hits=4)
-    return map_config(lib, "config4", gpath, pr, kw, 0, 2_000 if cpu else 0, 200_000, skip_plain=skip_plain)
+    return map_config(lib, "config4", gpath, pr, kw, 0, 2_000 if cpu else 0, 200_000, skip_plain=skip_plain,
+                      ordered_reads=min(n_reads, 2_000_000))
 
 
 def serve_trace(g, n_req: int, seed: int = 5, lo: int = 1_000, hi: int = 1_000_000):
