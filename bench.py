@@ -442,7 +442,7 @@ def config3(lib, cpu=True, n_reads=5_000_000):
     pr = synth.fast_color_reads(g, n_reads, (34, 74), 303, snp=0.01, color_err=0.03, qlo=10, qhi=35)
     log(f"config3: {n_reads} reads generated in {time.perf_counter() - t0:.1f}s")
     kw = dict(allowed_indels=3, z=85, t=100, max_hits=1)
-    return map_config(lib, "config3", gpath, pr, kw, 1, 20_000 if cpu else 0, 500_000)
+    return map_config(lib, "config3", gpath, pr, kw, 1, 150_000 if cpu else 0, 500_000)
 
 
 def config4(lib, cpu=True, n_reads=20_000_000, skip_plain=False):
@@ -458,7 +458,7 @@ def config4(lib, cpu=True, n_reads=20_000_000, skip_plain=False):
     pr = synth.fast_nucleotide_reads(g, n_reads, 100, 404, sub=0.015, indel=0.002, qlo=15, qhi=40)
     log(f"config4: {n_reads} reads generated in {time.perf_counter() - t0:.1f}s")
     kw = dict(allowed_indels=7, z=100, t=115, max_hits=4)
-    return map_config(lib, "config4", gpath, pr, kw, 0, 2_000 if cpu else 0, 200_000, skip_plain=skip_plain,
+    return map_config(lib, "config4", gpath, pr, kw, 0, 10_000 if cpu else 0, 200_000, skip_plain=skip_plain,
                       ordered_reads=min(n_reads, 2_000_000))
 
 
