@@ -131,20 +131,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 3 :
       // two records per lane, one per 16-bit half (records 2l, 2l+1, then 2l+128, ...), directions
       // stored as they are computed and the traceback of a record that can be accepted walked at
       // once, so no record is aligned twice
-      for (int k0 = 2 * lane; k0 < npl; k0 += 128) {
-        int rs0 = 0, h0 = 0, rs1 = 0, h1 = 0;
-        const bool run0 = triage(k0, rs0, h0) == 0;
-        const bool run1 = k0 + 1 < npl && triage(k0 + 1, rs1, h1) == 0;
-        if (run0 || run1) {   // a lone record runs in both halves; the copy writes nothing
-          AlignRec x0, x1;
-          const int kA = run0 ? k0 : k0 + 1;
-          dp_score_pair<IM, COLOR, true>(a.g, a.r, a.sc, run0 ? rs0 : rs1, run0 ? h0 : h1, run1 ? rs1 : rs0,
-                                         run1 ? h1 : h0, x0, x1, tbw, tbbuf + (int64_t)kA * TBB,
-                                         (run0 && run1) ? tbbuf + (int64_t)(k0 + 1) * TBB : nullptr);
-          if (run0) recs[k0] = x0;
-          if (run1) recs[k0 + 1] = run0 ? x1 : x0;
-        }
+      // the records that run, compacted in record order into hbuf (free during a flush), so that
+      // skipped and overflowing records leave no lane idle
+      int nrun = 0;
+      for (int b0 = 0; b0 < npl; b0 += 64) {
+        const int k = b0 + lane;
+        int rs = 0, h = 0, r = 0;
+        if (k < npl) r = triage(k, rs, h) == 0;
+        const int incl = wave_scan_incl_i32(r);
+        if (r) hbuf[nrun + incl - 1] = k;
+        nrun += __shfl(incl, 63, 64);
       }
+      __syncthreads();
+      for (int j0 = 2 * lane; j0 < nrun; j0 += 128) {
+        const int k0 = hbuf[j0];
+        const bool two = j0 + 1 < nrun;   // a lone record runs in both halves; the copy writes nothing
+        const int k1 = two ? hbuf[j0 + 1] : k0;
+        const int c0 = rec2cand[k0], c1 = rec2cand[k1];
+        AlignRec x0, x1;
+        dp_score_pair<IM, COLOR, true>(a.g, a.r, a.sc, (int)(2 * base + crs[c0]), cdiag[c0], (int)(2 * base + crs[c1]),
+                                       cdiag[c1], x0, x1, tbw, tbbuf + (int64_t)k0 * TBB,
+                                       two ? tbbuf + (int64_t)k1 * TBB : nullptr);
+        recs[k0] = x0;
+        if (two) recs[k1] = x1;
+      }
+      __syncthreads();   // hbuf is rewritten below
     } else {
       for (int k = lane; k < npl; k += 64) {
         int rs = 0, h = 0;
