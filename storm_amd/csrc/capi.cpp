@@ -1265,12 +1265,21 @@ void storm_filter_plan_destroy(storm_filter_plan* pl) {
 // Process-global state like the reference filter's (alignment_simd.c:1032-1043): init sets the
 // constants and read length; the genome registered once is resident in HBM.
 namespace {
+// One call's read planes, windows and pass flags, in pinned, coherent host memory that the
+// kernel reads and writes directly: a call is one launch and one synchronisation, no copies.
+struct CallStage {
+  uint32_t planes[2 * RW];
+  int32_t len;
+  int32_t rd[64], ps[64];
+  uint8_t pass[64];
+};
 struct CallState {
   DevGenome g;
   const unsigned char* code0 = nullptr;
   int64_t genome_codes = 0;
   int64_t lead_codes = 0;
-  DevMem planes, len, cand_read, cand_pos, pass;
+  CallStage* stage = nullptr;       // host address
+  CallStage* stage_dev = nullptr;   // the same memory's device address
   hipStream_t st = nullptr;
   int W = 0, L = 0, thr = 0, mt = 5, mm = 4, op = 9, ex = 4;
   bool ready = false;
@@ -1293,11 +1302,10 @@ int alignment_gfx950__set_genome(const unsigned char* sequence_alloc, size_t all
   c.code0 = sequence_alloc + code0_offset_bytes;
   c.genome_codes = codes;
   c.lead_codes = (int64_t)code0_offset_bytes * 4;
-  HIPOK(c.planes.ensure(2 * RW * 4));
-  HIPOK(c.len.ensure(16));
-  HIPOK(c.cand_read.ensure(64 * 4));
-  HIPOK(c.cand_pos.ensure(64 * 4));
-  HIPOK(c.pass.ensure(64));
+  if (!c.stage) {
+    HIPOK(hipHostMalloc((void**)&c.stage, sizeof(CallStage), hipHostMallocCoherent | hipHostMallocMapped));
+    HIPOK(hipHostGetDevicePointer((void**)&c.stage_dev, c.stage, 0));
+  }
   return STORM_OK;
 }
 
@@ -1327,37 +1335,26 @@ static unsigned call_align(int W, int mul, unsigned char* genome, int* pos_genom
     return 0;
   }
   if (c.L < W || c.L > MAXL) { fail(STORM_ERR_INVALID, "read length %d not supported", c.L); return 0; }
-  uint32_t planes[2 * RW];
-  pack_read_planes(read, c.L, planes);
-  int32_t rd[64], ps[64];
-  for (int k = 0; k < mul; ++k) { rd[k] = 0; ps[k] = (int32_t)(pos_genome[k] + c.lead_codes); }
-  int32_t L = c.L;
-  uint8_t pass[64];
-  if (hipMemcpyAsync(c.planes.p, planes, sizeof planes, hipMemcpyHostToDevice, c.st) != hipSuccess ||
-      hipMemcpyAsync(c.len.p, &L, 4, hipMemcpyHostToDevice, c.st) != hipSuccess ||
-      hipMemcpyAsync(c.cand_read.p, rd, (size_t)mul * 4, hipMemcpyHostToDevice, c.st) != hipSuccess ||
-      hipMemcpyAsync(c.cand_pos.p, ps, (size_t)mul * 4, hipMemcpyHostToDevice, c.st) != hipSuccess) {
-    fail(STORM_ERR_DEVICE, "alignment_gfx950: copy failed");
-    return 0;
-  }
+  CallStage& in = *c.stage;   // the previous call has completed: its synchronisation returned
+  pack_read_planes(read, c.L, in.planes);
+  in.len = c.L;
+  for (int k = 0; k < mul; ++k) { in.rd[k] = 0; in.ps[k] = (int32_t)(pos_genome[k] + c.lead_codes); }
   FilterArgs a{};
   a.g = c.g.view();
-  a.rplanes = c.planes.as<uint32_t>();
-  a.rlen = c.len.as<int32_t>();
+  a.rplanes = c.stage_dev->planes;
+  a.rlen = &c.stage_dev->len;
   a.len_shift = 0;
-  a.cand_read = c.cand_read.as<int32_t>();
-  a.cand_pos = c.cand_pos.as<int32_t>();
+  a.cand_read = c.stage_dev->rd;
+  a.cand_pos = c.stage_dev->ps;
   a.n = mul;
-  a.pass = c.pass.as<uint8_t>();
+  a.pass = c.stage_dev->pass;
   a.mt = c.mt; a.mm = c.mm; a.op = c.op; a.ex = c.ex; a.thr = c.thr;
-  if (launch_filter(a, W, c.st) != hipSuccess ||
-      hipMemcpyAsync(pass, c.pass.p, (size_t)mul, hipMemcpyDeviceToHost, c.st) != hipSuccess ||
-      hipStreamSynchronize(c.st) != hipSuccess) {
+  if (launch_filter(a, W, c.st) != hipSuccess || hipStreamSynchronize(c.st) != hipSuccess) {
     fail(STORM_ERR_DEVICE, "alignment_gfx950: launch failed");
     return 0;
   }
   unsigned x = 0;
-  for (int k = 0; k < mul; ++k) x |= (unsigned)(pass[k] & 1) << k;
+  for (int k = 0; k < mul; ++k) x |= (unsigned)(in.pass[k] & 1) << k;
   return x;
 }
 
@@ -1370,8 +1367,8 @@ unsigned alignment_gfx950__align_mono(unsigned char* g, int* p, unsigned char* r
 void alignment_gfx950__clean(void) {
   CallState& c = g_call;
   c.g = DevGenome();
-  c.planes.release(); c.len.release(); c.cand_read.release(); c.cand_pos.release(); c.pass.release();
-  if (c.st) { (void)hipStreamDestroy(c.st); c.st = nullptr; }
+  if (c.st) { (void)hipStreamSynchronize(c.st); (void)hipStreamDestroy(c.st); c.st = nullptr; }
+  if (c.stage) { (void)hipHostFree(c.stage); c.stage = nullptr; c.stage_dev = nullptr; }
   c.code0 = nullptr;
   c.ready = false;
 }

@@ -1,0 +1,47 @@
+"""The drop-in rows of the reference's SIMD filter table (alignment_gfx950__init_<w> /
+__align_<w>, same signatures as alignment_simd.h:32-300, one GPU launch per call of simd_mul
+windows of one read, run.c:109,660) against the reference SSE2 routine's per-candidate u8 scores
+(golden fixtures): bit k of a call's result is score(window k) > min(threshold, 254)."""
+import ctypes as C
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+ROW = {2: ("octa", 8), 4: ("quad", 4), 8: ("pair", 2), 16: ("mono", 1)}
+
+
+@pytest.mark.parametrize("fx", sorted(glob.glob(os.path.join(GOLDEN, "filter_*_L75.npz")) +
+                                      glob.glob(os.path.join(GOLDEN, "filter_*_L35.npz"))), ids=os.path.basename)
+def test_percall_rows_match_reference_scores(fx):
+    from storm_amd import lib
+    so = lib.load()
+    z = np.load(fx)
+    gp, pad = np.ascontiguousarray(z["genome_packed"]), int(z["pad"])
+    L, W, stride = int(z["L"]), int(z["W"]), int(z["stride"])
+    rp = np.ascontiguousarray(z["reads_packed"])
+    pos, read_of, scores = z["pos"], z["read_of"], z["scores"].astype(np.int64)
+    name, mul = ROW[W]
+    P = C.c_void_p
+    lib.check(so.alignment_gfx950__set_genome(P(gp.ctypes.data), gp.nbytes, pad))
+    init = getattr(so, f"alignment_gfx950__init_{name}")
+    align = getattr(so, f"alignment_gfx950__align_{name}")
+    n_reads = int(read_of.max()) + 1
+    checked = 0
+    for thr in (60, 100, 300):
+        init(5, 4, 9, 4, thr, L)
+        for r in range(0, n_reads, 3):
+            cands = np.nonzero(read_of == r)[0]
+            sel = np.resize(cands, mul)            # simd_mul windows of this read, repeated to fill
+            ps = np.ascontiguousarray(pos[sel], dtype=np.int32)
+            bits = align(P(gp.ctypes.data + pad), P(ps.ctypes.data), P(rp.ctypes.data + r * stride))
+            want = sum(1 << k for k in range(mul) if scores[sel[k]] > min(thr, 254))
+            assert bits == want, (fx, thr, r, bits, want)
+            checked += mul
+    so.alignment_gfx950__clean()
+    assert checked > 100
