@@ -33,7 +33,7 @@ VALU_PEAK_INT32 = 39.3       # T VALU lane-instructions/s: 256 CU x 64 lanes x 2
 VALU_PEAK_PK16 = 78.6        # T 16-bit ops/s with packed VALU instructions (2 per lane-instruction)
 # VALU lane-instructions k_filter<8> issues per cell at L=75 (SQ_INSTS_VALU x 64 / cells, rocprofv3 --pmc,
 # profiles/k_filter_W8_L75_10Mcand.md); the kernel's own work measure for the VALU-issue roofline
-VALU_INSTR_PER_CELL_W8_L75 = 5.447
+VALU_INSTR_PER_CELL_W8_L75 = 5.438
 # VALU lane-instructions k_align<7,false> issues per DP cell (L x 15 band columns) on the DP stage
 # workload (score pass of every record + traceback pass of the accepted ones), measured the same way
 # (profiles/k_align_I7_L75_2M_records.md)
@@ -41,7 +41,7 @@ VALU_INSTR_PER_DP_CELL_I7_L75 = 78.16
 # HBM bytes per candidate of the same dispatch: (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00) x 1024 /
 # candidates, the factors calibrated on a device-to-device copy of known size in the same run
 # (profiles/k_filter_W8_L75_10Mcand.md)
-HBM_BYTES_PER_CAND_W8_L75 = 45.8
+HBM_BYTES_PER_CAND_W8_L75 = 45.6
 # algorithmic bytes per candidate: packed read (19 B at L = 75) + genome window (L + 2W - 1 = 90
 # codes, 23 B) + the u8 score and the pass byte
 ALG_BYTES_PER_CAND_W8_L75 = 44.0
