@@ -352,7 +352,8 @@ def cpu_mapping_baseline(lib, d, gpath, pr, kw, color, n_sample, ordered=False):
 def map_config(lib, name, gpath, pr, kw, color, n_sample_cpu, sam_reads, skip_plain=False, ordered=False,
                ordered_reads=0):
     """Batched (persistent fused kernel in a hipGraph) and plain (staged kernels, one batch per
-    read length) on the same resident mapper; hits returned to host memory; whole-call wall time."""
+    read length) on the same resident mapper; hits returned to host memory; whole-call wall time
+    (the mean of repeated calls when one call takes under 0.1 s)."""
     t0 = time.perf_counter()
     m = lib.Mapper(gpath, color=color, **kw)
     t_create = time.perf_counter() - t0
@@ -362,9 +363,21 @@ def map_config(lib, name, gpath, pr, kw, color, n_sample_cpu, sam_reads, skip_pl
     if not skip_plain:
         m.map_hits(pr, mode=0, lo=0, hi=w)
     res = {"reads": n, "genome_load_index_s": round(t_create, 2)}
-    t0 = time.perf_counter()
-    hb, sb = m.map_hits(pr, mode=1)
-    tb = time.perf_counter() - t0
+
+    def timed(mode):
+        """One whole call; a call shorter than 0.1 s is repeated for about 0.3 s and averaged."""
+        t0 = time.perf_counter()
+        h, st = m.map_hits(pr, mode=mode)
+        t = time.perf_counter() - t0
+        if t < 0.1:
+            reps = max(1, min(50, int(0.3 / max(t, 1e-4))))
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                m.map_hits(pr, mode=mode)
+            t = (time.perf_counter() - t0) / reps
+        return h, st, t
+
+    hb, sb, tb = timed(1)
     res["batched"] = {"reads_per_s": round(n / tb, 1), "seconds": round(tb, 3), "mapped": sb["n_mapped"],
                       "gpu_ms_fused": round(sb["gpu_ms_filter"], 2),
                       "staged_fallback_gpu_ms": round(sb["gpu_ms_seed"] + sb["gpu_ms_align"] + sb["gpu_ms_replay"], 2),
@@ -374,9 +387,7 @@ def map_config(lib, name, gpath, pr, kw, color, n_sample_cpu, sam_reads, skip_pl
     log(f"{name}: batched {n} reads in {tb:.2f}s = {n / tb:.0f} reads/s, mapped {sb['n_mapped']}, "
         f"candidates {sb['n_candidates']}, fused kernel {sb['gpu_ms_filter']:.1f} ms")
     if not skip_plain:
-        t0 = time.perf_counter()
-        hp, sp = m.map_hits(pr, mode=0)
-        tp = time.perf_counter() - t0
+        hp, sp, tp = timed(0)
         same = all(np.array_equal(hb[k], hp[k]) for k in hb)
         lk_bytes = 8 * sp["n_lookups"]
         res["plain"] = {"reads_per_s": round(n / tp, 1), "seconds": round(tp, 3), "mapped": sp["n_mapped"],

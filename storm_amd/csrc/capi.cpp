@@ -353,11 +353,10 @@ int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int pass_id
                                  P.seg_end.as<int64_t>(), st));
     size_t sb = 0;
     HIPOK(segsort_i32(nullptr, nullptr, total, n_rs, nullptr, nullptr, nullptr, sb, st));
-    DevMem tmp;
-    HIPOK(tmp.ensure(sb + 256));
-    sb = tmp.n;
+    HIPOK(P.cub_tmp.ensure(sb + 256));
+    sb = P.cub_tmp.n;
     HIPOK(segsort_i32(P.hits.as<int32_t>(), P.sorted.as<int32_t>(), total, n_rs, P.seg_begin.as<int64_t>(),
-                      P.seg_end.as<int64_t>(), tmp.p, sb, st));
+                      P.seg_end.as<int64_t>(), P.cub_tmp.p, sb, st));
     SeedArgs su = sa;
     su.hits = P.sorted.as<int32_t>();
     HIPOK(launch_unique_segments(su, st));
@@ -569,10 +568,10 @@ static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode,
     batches.emplace_back(0, n);
   }
   const double t0 = now_s();
-  DevReads dr;
+  DevReads& dr = m->plain_reads;   // kept across calls: a call allocates only when it outgrows them
   int rc = upload_reads(m, v, order, dr);
   if (rc) return rc;
-  DevMem slots;
+  DevMem& slots = m->plain_slots;
   HIPOK(slots.ensure((size_t)(4 * n) * sizeof(Slot)));
   HIPOK(launch_slots_init(slots.as<Slot>(), 4 * n, m->st));
   TbStore store;

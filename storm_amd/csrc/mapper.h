@@ -191,6 +191,8 @@ struct storm_mapper {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   storm::FusedState fs;
   storm::DevReads stage;       // the caller's reads as uploaded, before the device reordering
+  storm::DevReads plain_reads; // the plain path's reads in device order, reused across calls
+  storm::DevMem plain_slots;   // the plain path's hit map slots, reused across calls
   storm::DevMem order_dev;     // device row -> caller's read index
   storm::DevMem hits_dev;      // per-slot results on their way to the caller
   storm::SharedTune* shared_tune = nullptr;   // tuner results of sibling contexts, or null
