@@ -46,6 +46,7 @@ __device__ __forceinline__ uint32_t padd(uint32_t a, uint32_t b) {
   return pu(__builtin_elementwise_add_sat(ps(a), ps(b)));
 }
 __device__ __forceinline__ uint32_t psub16(uint32_t a, uint32_t b) { return pwu(pw(a) - pw(b)); }   // per half
+__device__ __forceinline__ uint32_t pmad(uint32_t a, uint32_t b, uint32_t c) { return pwu(pw(a) * pw(b) + pw(c)); }
 __device__ __forceinline__ uint32_t pmax(uint32_t a, uint32_t b) { return pu(__builtin_elementwise_max(ps(a), ps(b))); }
 __device__ __forceinline__ uint32_t pmin(uint32_t a, uint32_t b) { return pu(__builtin_elementwise_min(ps(a), ps(b))); }
 // bit c of each half as a mask
@@ -172,6 +173,7 @@ __device__ __forceinline__ void dp_score_pair(const GenomeDev& g, const ReadsDev
   const uint32_t tx23 = (uint32_t)(uint16_t)k.px2 | ((uint32_t)(uint16_t)k.px3 << 16);
   const uint32_t MIN2 = pk(MIN_SCORE), OPEN2 = pk(k.gap_open), EXT2 = pk(k.gap_ext), I4_2 = pk(k.I4);
   const uint32_t BORDER2 = pk(m_make(3, k.I, 0, 0, 0));
+  const uint32_t DGAP2 = psub16(OPEN2, EXT2);
   uint32_t sc[NCM], meta[NCM];
   // row 0 (alignment.c:326-360)
   {
@@ -279,10 +281,12 @@ __device__ __forceinline__ void dp_score_pair(const GenomeDev& g, const ReadsDev
         const uint32_t diag = padd(csc, ms);
         const uint32_t t_lo8 = COLOR ? (tmeta & 0x00ff00ffu) : tmeta;
         const uint32_t l_lo8 = COLOR ? (left_meta & 0x00ff00ffu) : left_meta;
-        const uint32_t t_try = padd(tsc, psel(m_zero((t_lo8 & 0x00030003u) ^ 0x00020002u), EXT2, OPEN2));
+        // a cell's `from` is 1 (left), 2 (top) or 3 (corner; also row 0 and the band border): a gap
+        // extends from the top cell iff its from is 2 (bit 0 clear), from the left iff 1 (bit 1 clear)
+        const uint32_t t_try = padd(tsc, pmad(t_lo8 & 0x00010001u, DGAP2, EXT2));
         const uint32_t take_t = m_gt(I4_2, t_lo8) & m_gt(t_try, diag);
         const uint32_t after_t = psel(take_t, t_try, diag);
-        const uint32_t l_try = padd(left_sc, psel(m_zero((l_lo8 & 0x00030003u) ^ 0x00010001u), EXT2, OPEN2));
+        const uint32_t l_try = padd(left_sc, pmad((l_lo8 >> 1) & 0x00010001u, DGAP2, EXT2));
         const uint32_t take_l = m_gt(I4_2, l_lo8) & m_gt(l_try, after_t);
         const uint32_t nsc = psel(take_l, l_try, after_t);
         const uint32_t d_lo = (COLOR ? (cmeta & 0x00ff00ffu) : cmeta) | 0x00030003u;
