@@ -181,11 +181,13 @@ struct DPLane {
         }
         const int diag = csc + match_score;
         const int t_lo8 = COLOR ? (tmeta & 0xff) : tmeta, l_lo8 = COLOR ? (left_meta & 0xff) : left_meta;
-        const int t_try = tsc + msel((t_lo8 & 3) == 2, gap_ext, gap_open);
+        // a cell's from is 1 (left), 2 (top) or 3 (corner; also row 0 and the band border): a gap
+        // extends from the top cell iff its from is 2 (bit 0 clear), from the left iff 1 (bit 1 clear)
+        const int t_try = tsc + gap_ext + (t_lo8 & 1) * (gap_open - gap_ext);
         const bool take_t = (t_lo8 < I4) & (t_try > diag);   // indels < I
 
         const int after_t = msel(take_t, t_try, diag);
-        const int l_try = left_sc + msel((l_lo8 & 3) == 1, gap_ext, gap_open);
+        const int l_try = left_sc + gap_ext + ((l_lo8 >> 1) & 1) * (gap_open - gap_ext);
         const bool take_l = (l_lo8 < I4) & (l_try > after_t);
         const int nsc = msel(take_l, l_try, after_t);
         // low byte of the new meta word (from | indels << 2): the corner's indels with from = 3, or a
