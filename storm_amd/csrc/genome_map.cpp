@@ -239,7 +239,12 @@ int write_sam_ordered(const char* path, const std::vector<HostRef>& refs, const 
     };
     auto mark = [&](int32_t rid, int64_t lo, int64_t hi) {
       std::vector<uint64_t>& d = dirty[(size_t)rid];
-      for (int64_t q = lo; q <= hi; ++q) d[(size_t)(q >> 6)] |= 1ull << (q & 63);
+      for (int64_t w = lo >> 6; w <= (hi >> 6); ++w) {   // the bits lo..hi, a word at a time
+        uint64_t m = ~0ull;
+        if (w == (lo >> 6)) m &= ~0ull << (lo & 63);
+        if (w == (hi >> 6)) m &= (hi & 63) == 63 ? ~0ull : ((1ull << ((hi & 63) + 1)) - 1);
+        d[(size_t)w] |= m;
+      }
       marked.push_back({rid, {lo, hi}});
     };
     const int64_t n_sorted = (int64_t)keyed.size();
@@ -247,14 +252,25 @@ int write_sam_ordered(const char* path, const std::vector<HostRef>& refs, const 
     const int nt = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
     std::vector<int> sadj((size_t)(4 * B)), sbest((size_t)B);
     std::vector<uint8_t> sok((size_t)B);
-    std::vector<int64_t> sbad((size_t)B);
+    std::vector<int64_t> sbad((size_t)B), slo((size_t)(4 * B)), shi((size_t)(4 * B));
     for (int64_t s0 = 0; s0 < n_sorted; s0 += B) {
       const int64_t nb = std::min(B, n_sorted - s0);
+      // per read of the chunk: its speculative scores, and each hit's span clipped to its sequence
+      // (a span depends on the hit alone, so the ordered pass below only reads it)
+      auto prep = [&](int64_t j) {
+        const int32_t id = (int32_t)(keyed[(size_t)(s0 + j)] & 0xffffffffu);
+        sok[(size_t)j] = score_read(id, &sadj[(size_t)(4 * j)], sbest[(size_t)j], sbad[(size_t)j]);
+        const HostSlot* row = &slots[(size_t)(4 * id)];
+        for (int k = 0; k < 4 && row[k].ref_start != -1; ++k) {
+          int64_t lo, hi;
+          span(row[k], lo, hi);
+          slo[(size_t)(4 * j + k)] = std::max<int64_t>(lo, 0);
+          shi[(size_t)(4 * j + k)] = std::min<int64_t>(hi, refs[(size_t)row[k].ref_id].size);
+        }
+      };
       auto work = [&](int t) {
         const int64_t per = (nb + nt - 1) / nt, j0 = t * per, j1 = std::min(nb, j0 + per);
-        for (int64_t j = j0; j < j1; ++j)
-          sok[(size_t)j] = score_read((int32_t)(keyed[(size_t)(s0 + j)] & 0xffffffffu), &sadj[(size_t)(4 * j)],
-                                      sbest[(size_t)j], sbad[(size_t)j]);
+        for (int64_t j = j0; j < j1; ++j) prep(j);
       };
       if (nb >= 4096 && nt > 1) {
         std::vector<std::thread> th;
@@ -262,13 +278,17 @@ int write_sam_ordered(const char* path, const std::vector<HostRef>& refs, const 
         work(0);
         for (auto& x : th) x.join();
       } else {
-        for (int64_t j = 0; j < nb; ++j)
-          sok[(size_t)j] = score_read((int32_t)(keyed[(size_t)(s0 + j)] & 0xffffffffu), &sadj[(size_t)(4 * j)],
-                                      sbest[(size_t)j], sbad[(size_t)j]);
+        for (int64_t j = 0; j < nb; ++j) prep(j);
       }
       constexpr int64_t PD = 8;   // prefetch distance in reads: slot rows at 2 PD, the data they name at PD
       for (int64_t j = 0; j < nb; ++j) {
         if (j + 2 * PD < nb) __builtin_prefetch(&slots[(size_t)(4 * (keyed[(size_t)(s0 + j + 2 * PD)] & 0xffffffffu))]);
+        if (j + PD < nb) {
+          const HostSlot* rp = &slots[(size_t)(4 * (keyed[(size_t)(s0 + j + PD)] & 0xffffffffu))];
+          for (int k = 0; k < 4 && rp[k].ref_start != -1; ++k)   // the overlap check's bitmap words
+            __builtin_prefetch(&dirty[(size_t)rp[k].ref_id][(size_t)(std::min<int64_t>(slo[(size_t)(4 * (j + PD) + k)],
+                                                                                   refs[(size_t)rp[k].ref_id].size) >> 6)]);
+        }
         if (j + PD < nb && sbest[(size_t)(j + PD)] >= 0) {
           const HostSlot& hp = slots[(size_t)(4 * (keyed[(size_t)(s0 + j + PD)] & 0xffffffffu)) + sbest[(size_t)(j + PD)]];
           __builtin_prefetch(hp.tb.data());
@@ -276,7 +296,9 @@ int write_sam_ordered(const char* path, const std::vector<HostRef>& refs, const 
           if (hp.ref_start >= 0 && hp.ref_start + hp.tb_len <= cp.size) {
             for (int64_t q = hp.ref_start; q < hp.ref_start + hp.tb_len; q += 16) {
               const Consensus::Page* pp = cp.page(q);
-              if (pp) __builtin_prefetch(&pp->word[q & (Consensus::P - 1)], 1);
+              if (!pp) continue;
+              __builtin_prefetch(&pp->word[q & (Consensus::P - 1)], 1);
+              if ((q - hp.ref_start) % 64 == 0) __builtin_prefetch(&pp->gaps[q & (Consensus::P - 1)], 1);
             }
             __builtin_prefetch(&refs[(size_t)hp.ref_id].code[(size_t)hp.ref_start]);
           }
@@ -289,11 +311,7 @@ int write_sam_ordered(const char* path, const std::vector<HostRef>& refs, const 
         int64_t bad = sbad[(size_t)j];
         bool redo = false;
         for (int k = 0; k < 4 && row[k].ref_start != -1 && !marked.empty() && !redo; ++k) {
-          int64_t lo, hi;
-          span(row[k], lo, hi);
-          const int64_t sz = refs[(size_t)row[k].ref_id].size;
-          lo = std::max<int64_t>(lo, 0);
-          hi = std::min<int64_t>(hi, sz);
+          const int64_t lo = slo[(size_t)(4 * j + k)], hi = shi[(size_t)(4 * j + k)];
           redo = lo <= hi && is_dirty(row[k].ref_id, lo, hi);
         }
         if (redo) ok = score_read(id, adj, best_rank, bad);
