@@ -1262,7 +1262,9 @@ void storm_filter_plan_destroy(storm_filter_plan* pl) {
 
 // ------------------------------------------------------------------------------ per-call rows
 // Process-global state like the reference filter's (alignment_simd.c:1032-1043): init sets the
-// constants and read length; the genome registered once is resident in HBM.
+// constants and read length; the genome registered once is resident in HBM.  The align calls are
+// reentrant, as the reference's are (run.c:900 calls them from an OpenMP loop): each call takes a
+// staging slot of its own from a pool.
 namespace {
 // One call's read planes, windows and pass flags, in pinned, coherent host memory that the
 // kernel reads and writes directly: a call is one launch and one synchronisation, no copies.
@@ -1272,39 +1274,73 @@ struct CallStage {
   int32_t rd[64], ps[64];
   uint8_t pass[64];
 };
+struct CallSlot {
+  CallStage* stage = nullptr;       // host address
+  CallStage* stage_dev = nullptr;   // the same memory's device address
+  hipStream_t st = nullptr;
+};
 struct CallState {
   DevGenome g;
   const unsigned char* code0 = nullptr;
   int64_t genome_codes = 0;
   int64_t lead_codes = 0;
-  CallStage* stage = nullptr;       // host address
-  CallStage* stage_dev = nullptr;   // the same memory's device address
-  hipStream_t st = nullptr;
   int W = 0, L = 0, thr = 0, mt = 5, mm = 4, op = 9, ex = 4;
   bool ready = false;
+  std::mutex mu;                    // guards the slot pool
+  std::vector<CallSlot> idle, all;
 };
 CallState g_call;
+thread_local int t_call_status = STORM_OK;
+
+int take_slot(CallSlot& out) {
+  CallState& c = g_call;
+  {
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (!c.idle.empty()) {
+      out = c.idle.back();
+      c.idle.pop_back();
+      return STORM_OK;
+    }
+  }
+  CallSlot s;
+  HIPOK(hipSetDevice(0));
+  HIPOK(hipStreamCreateWithFlags(&s.st, hipStreamNonBlocking));
+  if (hipHostMalloc((void**)&s.stage, sizeof(CallStage), hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void**)&s.stage_dev, s.stage, 0) != hipSuccess) {
+    if (s.stage) (void)hipHostFree(s.stage);
+    (void)hipStreamDestroy(s.st);
+    return fail(STORM_ERR_MEMORY, "alignment_gfx950: pinned staging memory");
+  }
+  std::lock_guard<std::mutex> lk(c.mu);
+  c.all.push_back(s);
+  out = s;
+  return STORM_OK;
+}
+
+void give_slot(const CallSlot& s) {
+  std::lock_guard<std::mutex> lk(g_call.mu);
+  g_call.idle.push_back(s);
+}
 }  // namespace
 
 int alignment_gfx950__set_genome(const unsigned char* sequence_alloc, size_t alloc_bytes, size_t code0_offset_bytes) {
   if (!sequence_alloc || code0_offset_bytes > alloc_bytes) return fail(STORM_ERR_INVALID, "bad genome buffer");
   CallState& c = g_call;
   HIPOK(hipSetDevice(0));
-  if (!c.st) HIPOK(hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking));
+  hipStream_t st = nullptr;
+  HIPOK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   // the whole allocation (padding included) becomes the device genome; code 0 sits at
   // code0_offset_bytes * 4 codes into it
   const int64_t codes = (int64_t)alloc_bytes * 4;
   std::vector<uint8_t> cv((size_t)codes);
   for (int64_t i = 0; i < codes; ++i) cv[(size_t)i] = (sequence_alloc[i >> 2] >> ((i & 3) << 1)) & 3;
-  int rc = upload_genome(cv.data(), nullptr, codes, c.g, c.st);
+  int rc = upload_genome(cv.data(), nullptr, codes, c.g, st);
+  (void)hipStreamSynchronize(st);
+  (void)hipStreamDestroy(st);
   if (rc) return rc;
   c.code0 = sequence_alloc + code0_offset_bytes;
   c.genome_codes = codes;
   c.lead_codes = (int64_t)code0_offset_bytes * 4;
-  if (!c.stage) {
-    HIPOK(hipHostMalloc((void**)&c.stage, sizeof(CallStage), hipHostMallocCoherent | hipHostMallocMapped));
-    HIPOK(hipHostGetDevicePointer((void**)&c.stage_dev, c.stage, 0));
-  }
   return STORM_OK;
 }
 
@@ -1326,34 +1362,41 @@ void alignment_gfx950__init_quad(unsigned a, unsigned b, unsigned c, unsigned d,
 void alignment_gfx950__init_pair(unsigned a, unsigned b, unsigned c, unsigned d, unsigned e, unsigned f) { call_init(8, a, b, c, d, e, f); }
 void alignment_gfx950__init_mono(unsigned a, unsigned b, unsigned c, unsigned d, unsigned e, unsigned f) { call_init(16, a, b, c, d, e, f); }
 
-// One call filters simd_mul windows of one read (run.c:660); returns bit k for window k.
+// One call filters simd_mul windows of one read (run.c:660); returns bit k for window k.  On an
+// error it returns 0 and the calling thread's alignment_gfx950__status() is the error code.
 static unsigned call_align(int W, int mul, unsigned char* genome, int* pos_genome, unsigned char* read) {
   CallState& c = g_call;
+  t_call_status = STORM_OK;
   if (!c.ready || c.W != W || !c.code0 || genome != c.code0) {
-    fail(STORM_ERR_INVALID, "alignment_gfx950: init/set_genome missing or genome pointer not registered");
+    t_call_status = fail(STORM_ERR_INVALID, "alignment_gfx950: init/set_genome missing or genome pointer not registered");
     return 0;
   }
-  if (c.L < W || c.L > MAXL) { fail(STORM_ERR_INVALID, "read length %d not supported", c.L); return 0; }
-  CallStage& in = *c.stage;   // the previous call has completed: its synchronisation returned
+  if (c.L < W || c.L > MAXL) { t_call_status = fail(STORM_ERR_INVALID, "read length %d not supported", c.L); return 0; }
+  CallSlot slot;
+  if (int rc = take_slot(slot)) { t_call_status = rc; return 0; }
+  CallStage& in = *slot.stage;   // this call's own slot: no other call touches it until it is returned
   pack_read_planes(read, c.L, in.planes);
   in.len = c.L;
   for (int k = 0; k < mul; ++k) { in.rd[k] = 0; in.ps[k] = (int32_t)(pos_genome[k] + c.lead_codes); }
   FilterArgs a{};
   a.g = c.g.view();
-  a.rplanes = c.stage_dev->planes;
-  a.rlen = &c.stage_dev->len;
+  a.rplanes = slot.stage_dev->planes;
+  a.rlen = &slot.stage_dev->len;
   a.len_shift = 0;
-  a.cand_read = c.stage_dev->rd;
-  a.cand_pos = c.stage_dev->ps;
+  a.cand_read = slot.stage_dev->rd;
+  a.cand_pos = slot.stage_dev->ps;
   a.n = mul;
-  a.pass = c.stage_dev->pass;
+  a.pass = slot.stage_dev->pass;
   a.mt = c.mt; a.mm = c.mm; a.op = c.op; a.ex = c.ex; a.thr = c.thr;
-  if (launch_filter(a, W, c.st) != hipSuccess || hipStreamSynchronize(c.st) != hipSuccess) {
-    fail(STORM_ERR_DEVICE, "alignment_gfx950: launch failed");
+  if (hipSetDevice(0) != hipSuccess || launch_filter(a, W, slot.st) != hipSuccess ||
+      hipStreamSynchronize(slot.st) != hipSuccess) {
+    t_call_status = fail(STORM_ERR_DEVICE, "alignment_gfx950: launch failed");
+    give_slot(slot);
     return 0;
   }
   unsigned x = 0;
   for (int k = 0; k < mul; ++k) x |= (unsigned)(in.pass[k] & 1) << k;
+  give_slot(slot);
   return x;
 }
 
@@ -1365,12 +1408,21 @@ unsigned alignment_gfx950__align_mono(unsigned char* g, int* p, unsigned char* r
 
 void alignment_gfx950__clean(void) {
   CallState& c = g_call;
+  std::lock_guard<std::mutex> lk(c.mu);
+  (void)hipSetDevice(0);
+  for (CallSlot& s : c.all) {
+    (void)hipStreamSynchronize(s.st);
+    (void)hipStreamDestroy(s.st);
+    (void)hipHostFree(s.stage);
+  }
+  c.all.clear();
+  c.idle.clear();
   c.g = DevGenome();
-  if (c.st) { (void)hipStreamSynchronize(c.st); (void)hipStreamDestroy(c.st); c.st = nullptr; }
-  if (c.stage) { (void)hipHostFree(c.stage); c.stage = nullptr; c.stage_dev = nullptr; }
   c.code0 = nullptr;
   c.ready = false;
 }
+
+int alignment_gfx950__status(void) { return t_call_status; }
 
 int alignment_gfx950__compatible_proc(void) {
   int n = 0;

@@ -365,13 +365,13 @@ int write_sam_ordered(const char* path, const std::vector<HostRef>& refs, const 
   if (!out) { err = std::string("cannot write ") + path; return STORM_ERR_IO; }
   const SamFormat sf(refs, reads, p);
   sf.header(out, "coordinate", argc, argv);
-  write_blocks(out, (int64_t)placed.size(), [&](int64_t i, std::string& line, std::string& cigar, std::string& seq) {
+  const bool ok = write_blocks(out, (int64_t)placed.size(), [&](int64_t i, std::string& line, std::string& cigar, std::string& seq) {
     const Placement& pl = placed[(size_t)i];
     const HostSlot* row = &slots[(size_t)(4 * (int64_t)pl.read)];
     sf.record(line, pl.read, row, pl.rank, row[pl.rank].sense == -1 ? 0x10 : 0, cigar, seq);   // genome_map.c:688-776
     line += '\n';
   });
-  if (fclose(out) != 0) { err = std::string("error writing ") + path; return STORM_ERR_IO; }
+  if (fclose(out) != 0 || !ok) { err = std::string("error writing ") + path; return STORM_ERR_IO; }
   return STORM_OK;
 }
 

@@ -678,7 +678,8 @@ void SamFormat::record(std::string& line, int64_t id, const HostSlot* row, int k
   line += buf;
 }
 
-void write_blocks(FILE* out, int64_t n, const std::function<void(int64_t, std::string&, std::string&, std::string&)>& fmt) {
+bool write_blocks(FILE* out, int64_t n, const std::function<void(int64_t, std::string&, std::string&, std::string&)>& fmt) {
+  bool ok = true;
   const int nt = std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
   const int64_t per_thread = 32768, block = per_thread * nt;
   std::vector<std::string> part((size_t)nt);
@@ -696,8 +697,10 @@ void write_blocks(FILE* out, int64_t n, const std::function<void(int64_t, std::s
     work(0);
     for (auto& x : th) x.join();
     for (int t = 0; t < nt; ++t)
-      if (!part[(size_t)t].empty()) fwrite(part[(size_t)t].data(), 1, part[(size_t)t].size(), out);
+      if (!part[(size_t)t].empty() && fwrite(part[(size_t)t].data(), 1, part[(size_t)t].size(), out) != part[(size_t)t].size())
+        ok = false;
   }
+  return ok && !ferror(out);
 }
 
 int write_unmapped_fastq(const char* path, const HostReads& reads, const std::vector<HostSlot>& slots,
@@ -708,7 +711,7 @@ int write_unmapped_fastq(const char* path, const HostReads& reads, const std::ve
   static const short UB[4] = {2, 7, 10, 93};
   const bool color = p.color != 0;
   const char qmax = (char)std::min(p.qual_base + 93, 126);   // MAX_READ_QUALITY (read_quality.h:8)
-  write_blocks(out, reads.n, [&](int64_t id, std::string& line, std::string&, std::string&) {
+  const bool ok = write_blocks(out, reads.n, [&](int64_t id, std::string& line, std::string&, std::string&) {
     if (slots[(size_t)(4 * id)].ref_id != -1) return;
     const int L = reads.len[id];
     const std::string& nm = reads.names[(size_t)id];
@@ -736,7 +739,7 @@ int write_unmapped_fastq(const char* path, const HostReads& reads, const std::ve
     }
     line += '\n';
   });
-  if (fclose(out) != 0) { err = std::string("error writing ") + path; return STORM_ERR_IO; }
+  if (fclose(out) != 0 || !ok) { err = std::string("error writing ") + path; return STORM_ERR_IO; }
   return STORM_OK;
 }
 
@@ -789,8 +792,8 @@ int write_sam(const char* path, const std::vector<HostRef>& refs, const HostRead
       }
     }
   };
-  write_blocks(out, reads.n, format_read);
-  if (fclose(out) != 0) { err = std::string("error writing ") + path; return STORM_ERR_IO; }
+  const bool ok = write_blocks(out, reads.n, format_read);
+  if (fclose(out) != 0 || !ok) { err = std::string("error writing ") + path; return STORM_ERR_IO; }
   return STORM_OK;
 }
 

@@ -89,8 +89,8 @@ struct SamFormat {
 };
 
 // items [0, n) formatted by fmt(item, text, cigar scratch, seq scratch) on up to 16 threads in
-// blocks, written to out in item order
-void write_blocks(FILE* out, int64_t n, const std::function<void(int64_t, std::string&, std::string&, std::string&)>& fmt);
+// blocks, written to out in item order; false when a write failed
+bool write_blocks(FILE* out, int64_t n, const std::function<void(int64_t, std::string&, std::string&, std::string&)>& fmt);
 
 uint32_t glibc_rand_next(uint32_t* state /* 35+ words, state[34] = position */);
 

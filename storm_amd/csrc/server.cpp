@@ -22,6 +22,7 @@
 // staged kernels (one launch per stage and read length), as storm_mapper_map_hits does.
 #include <algorithm>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <list>
@@ -167,6 +168,11 @@ struct storm_server {
   };
   std::vector<std::unique_ptr<Lane>> lanes;
   SharedTune tune;   // the lanes share autotuner results
+  // fault injection for the error-path tests: the upload of the k-th admitted region (1-based,
+  // STORM_SERVER_FAIL_UPLOAD=k in the environment at creation) reports a failure without
+  // touching the device; 0 = off
+  int64_t fail_upload = 0;
+  int64_t n_uploads = 0;   // under mu
 };
 
 namespace storm {
@@ -196,6 +202,10 @@ static ServeRegion region_view(storm_server* s, const Request& r) {
 }
 
 static int upload_region(storm_server* s, storm_mapper* m, Request& r) {
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (++s->n_uploads == s->fail_upload) return fail(STORM_ERR_DEVICE, "injected upload failure (request %lld)", (long long)r.id);
+  }
   uint8_t* b = s->arena.base() + r.region_off;
   HIPOK(hipMemcpyAsync(b + r.o_codes, r.codes.data(), r.codes.size(), hipMemcpyHostToDevice, m->st));
   if (r.has_qual) HIPOK(hipMemcpyAsync(b + r.o_qual, r.qual.data(), r.qual.size(), hipMemcpyHostToDevice, m->st));
@@ -331,6 +341,18 @@ static void admit(storm_server* s, std::vector<Request*>& to_upload) {
   s->st.arena_peak_bytes = (int64_t)s->arena.peak();
 }
 
+// under s->mu: a failed request (its upload failed, or a batch holding its reads failed) is never
+// batched again: its unbatched reads are counted as done, and it goes to `retire` once no batch
+// holds any of its reads
+static void retire_failed(storm_server* s, std::vector<Request*>& retire) {
+  for (Request* r : s->active) {
+    if (r->status == STORM_OK || r->next_row >= r->n) continue;
+    r->rows_done += r->n - r->next_row;
+    r->next_row = r->n;
+    if (r->rows_done == r->n) retire.push_back(r);
+  }
+}
+
 // under s->mu: the next batch (empty when nothing is ready)
 static void form_batch(storm_server* s, const std::vector<Request*>& own_uploads, std::vector<Segment>& segs) {
   segs.clear();
@@ -339,7 +361,7 @@ static void form_batch(storm_server* s, const std::vector<Request*>& own_uploads
   };
   Request* head = nullptr;
   for (Request* r : s->active)
-    if (r->next_row < r->n && ready(r)) { head = r; break; }
+    if (r->next_row < r->n && r->status == STORM_OK && ready(r)) { head = r; break; }
   if (!head) return;
   const double now = now_s();
   if (s->cfg.mode == STORM_MODE_PLAIN) {   // one whole request
@@ -351,7 +373,7 @@ static void form_batch(storm_server* s, const std::vector<Request*>& own_uploads
   int64_t room = s->cfg.max_batch_reads;
   for (Request* r : s->active) {
     if (room == 0) break;
-    if (r->next_row >= r->n || !ready(r) || !(r->cls == head->cls)) continue;
+    if (r->next_row >= r->n || r->status != STORM_OK || !ready(r) || !(r->cls == head->cls)) continue;
     const int64_t k = std::min(room, r->n - r->next_row);
     segs.push_back(Segment{r, r->next_row, r->next_row + k});
     if (r->next_row == 0) r->t_first = now;
@@ -365,7 +387,7 @@ static bool has_work(storm_server* s) {
   if (!s->pending.empty() && (s->pending.front()->region_bytes == 0 || s->arena.fits(s->pending.front()->region_bytes)))
     return true;
   for (Request* r : s->active)
-    if (r->next_row < r->n && r->uploaded) return true;
+    if (r->next_row < r->n && r->uploaded) return true;   // failed ones included: they are retired
   return false;
 }
 
@@ -395,21 +417,24 @@ static void complete(storm_server* s, storm_server::Lane& ln, Request* r) {
 
 static void lane_loop(storm_server* s, storm_server::Lane* ln) {
   (void)hipSetDevice(s->device);
-  std::vector<Request*> uploads;
-  std::vector<Segment> segs;
+  std::vector<Request*> uploads, retire;
+  std::vector<Segment> segs, run;
   for (;;) {
+    retire.clear();
     {
       std::unique_lock<std::mutex> lk(s->mu);
       s->cv_work.wait(lk, [&] { return s->stop || has_work(s); });
       if (s->stop) return;
       uploads.clear();
       admit(s, uploads);
+      retire_failed(s, retire);
       form_batch(s, uploads, segs);
       if (!segs.empty()) {
         s->busy++;
         s->st.max_lanes_busy = std::max<int64_t>(s->st.max_lanes_busy, s->busy);
       }
     }
+    for (Request* r : retire) complete(s, *ln, r);
     // uploads of the regions this lane admitted; other lanes wait on each region's event
     for (Request* r : uploads) {
       int rc = upload_region(s, ln->m.get(), *r);
@@ -419,11 +444,17 @@ static void lane_loop(storm_server* s, storm_server::Lane* ln) {
       s->cv_work.notify_all();
     }
     if (segs.empty()) continue;
+    // the segments of requests that failed meanwhile (an upload of this lane) are left out: their
+    // regions hold no reads; they count as done and their requests complete as failed
+    run.clear();
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      for (const Segment& g : segs)
+        if (g.r->status == STORM_OK) run.push_back(g);
+    }
     const double tb = now_s();
     int rc = STORM_OK;
-    bool skip = false;
-    for (Segment& g : segs) skip = skip || g.r->status != STORM_OK;
-    if (!skip) rc = (s->cfg.mode == STORM_MODE_PLAIN) ? run_plain(s, *ln, *segs[0].r) : run_batch(s, *ln, segs);
+    if (!run.empty()) rc = (s->cfg.mode == STORM_MODE_PLAIN) ? run_plain(s, *ln, *run[0].r) : run_batch(s, *ln, run);
     std::string err = rc ? std::string(storm_last_error()) : std::string();
     std::vector<Request*> finished;
     {
@@ -438,6 +469,7 @@ static void lane_loop(storm_server* s, storm_server::Lane* ln) {
         if (std::find(seen.begin(), seen.end(), g.r) == seen.end()) seen.push_back(g.r);
         g.r->batches++;
         g.r->rows_done += g.row1 - g.row0;
+        // a failed batch fails every request it held (their results were not computed)
         if (rc != STORM_OK && g.r->status == STORM_OK) { g.r->status = rc; g.r->err = err; }
         if (g.r->rows_done == g.r->n) finished.push_back(g.r);
       }
@@ -528,6 +560,7 @@ int storm_server_create(int device, const char* genome_path, const storm_server_
     if (rc) return rc;
     s->st.arena_bytes = (int64_t)s->arena.cap();
   }
+  if (const char* f = getenv("STORM_SERVER_FAIL_UPLOAD")) s->fail_upload = atoll(f);
   s->t0 = now_s();
   storm_server* sp = s.get();
   for (auto& ln : s->lanes) ln->th = std::thread(lane_loop, sp, ln.get());

@@ -24,6 +24,7 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
 
 import numpy as np
 
@@ -154,8 +155,7 @@ def main():
     if not os.path.isdir("/root/reference"):
         raise SystemExit("the reference sources are needed to write golden fixtures")
     sh(["make", "-C", os.path.join(ROOT, "oracle"), "-j8", "ref"])
-    tmp = os.path.join(ROOT, "_work", "golden_tmp")
-    os.makedirs(tmp, exist_ok=True)
+    tmp = tempfile.mkdtemp(prefix="storm_golden_")
     only = set(sys.argv[1:])
     for c in SAM_CASES:
         if not only or c[0] in only:

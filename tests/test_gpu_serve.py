@@ -169,3 +169,29 @@ def test_served_request_matches_oracle_sam(serve_case, oracle_built):
         got = [(int(h["score"][r, k]), int(h["ref_start"][r, k]), int(h["sense"][r, k]))
                for k in range(4) if h["ref_id"][r, k] >= 0]
         assert got == want, f"read {r}"
+
+
+def test_failed_upload_fails_only_its_request(serve_case, alone, monkeypatch):
+    """One admitted region's upload is made to fail (fault injection from the host, no device
+    work): that request, and only it, ends with an error; the requests batched with it get the
+    same slots as alone."""
+    from storm_amd import lib
+    _, gpath, _, reqs = serve_case
+    monkeypatch.setenv("STORM_SERVER_FAIL_UPLOAD", "2")
+    s = lib.Server(gpath, lanes=2, max_batch_reads=4096)
+    monkeypatch.delenv("STORM_SERVER_FAIL_UPLOAD")
+    print("server: the second region upload fails", flush=True)
+    ids = [s.submit(pr, **kw) for pr, kw in reqs[:6]]
+    failed = []
+    for k, rid in enumerate(ids):
+        try:
+            h, res = s.wait(rid)
+        except lib.StormError as e:
+            assert "injected upload failure" in str(e)
+            failed.append(k)
+            continue
+        assert res["status"] == 0
+        assert _same(h, alone[k][0]), f"request {k}"
+    st = s.stats()
+    s.close()
+    assert len(failed) == 1 and st["failed"] == 1 and st["completed"] == 5
