@@ -71,6 +71,12 @@ void storm_params_default(storm_params *p, int color);
  * simd_mul = 8/4/2/1 windows per call exactly as the SSE2 row does (run.c:109).
  * `genome` must point inside a buffer registered with alignment_gfx950__set_genome (the
  * reference passes ref_seq, run.c:660), `read` is a packed read (codes.h:125-130).
+ * Threading, as the reference's rows: set_genome and init_<w> run once before the mapping loops
+ * and must not race with align calls; align_<w> calls are reentrant and may run from any number of
+ * host threads at once (run.c:900 calls them inside an OpenMP loop): each call uses a staging slot
+ * and stream of its own.  An align call that fails (no genome registered, wrong row, device error)
+ * returns 0; alignment_gfx950__status() then gives the calling thread's error code (STORM_OK after
+ * a call that succeeded) and storm_last_error() the message.
  */
 int alignment_gfx950__set_genome(const unsigned char *sequence_alloc, size_t alloc_bytes, size_t code0_offset_bytes);
 void alignment_gfx950__init_octa(unsigned match, unsigned mismatch, unsigned gapopen, unsigned gapextends,
@@ -87,6 +93,9 @@ unsigned alignment_gfx950__align_pair(unsigned char *genome, int *pos_genome, un
 unsigned alignment_gfx950__align_mono(unsigned char *genome, int *pos_genome, unsigned char *read);
 void alignment_gfx950__clean(void);
 int alignment_gfx950__compatible_proc(void);
+/* No counterpart in the reference (its rows cannot fail): the status of the calling thread's last
+ * align_<w> call. */
+int alignment_gfx950__status(void);
 
 /* ------------------------------------------------------------------------------------------
  * Group 2: batch filter (stage c) over many candidates, with inputs resident in HBM.
