@@ -163,6 +163,24 @@ int storm_align_run(storm_align_plan *plan, int reps, double *ms_per_run);
 int storm_align_fetch(storm_align_plan *plan, storm_align_result *results, uint8_t *tb);
 void storm_align_plan_destroy(storm_align_plan *plan);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the DP + traceback of
+ * n_rec records the caller holds in device memory, d_rec_rs / d_rec_diag with the meaning of
+ * storm_align_plan_create's rec_rs / rec_diag, against the plan's resident genome and reads (the
+ * plan's own records are not used), written to the caller's device buffers d_results (n_rec
+ * storm_align_result) and d_tb (n_rec x STORM_TB_BYTES).  The call allocates nothing: the direction
+ * words go to a static 16 MiB device array (held by each GPU the library runs on until the process
+ * ends), in launches of as many records as it holds (a wavefront of 64 records needs 64 x longest
+ * read x ceil((2 dp_band + 1) / 8) x 4 bytes; reads up to 256 codes at any -i fit; more is rejected
+ * with STORM_ERR_INVALID).  The array is shared by every call on a GPU: calls that overlap on one
+ * GPU race (use one stream, or wait between calls).  A record whose read strand is outside the
+ * plan's reads or whose window runs past the padded genome is not aligned: its result is
+ * score = bound = MIN_SCORE, best_j = -2, tb_len = 0.
+ * stream: a hipStream_t to enqueue on (the call returns at once), or NULL for the plan's stream
+ * (the call returns when the results are written). */
+int storm_align_run_records_noalloc(storm_align_plan *plan, const int32_t *d_rec_rs, const int32_t *d_rec_diag,
+                                    int64_t n_rec, storm_align_result *d_results, uint8_t *d_tb, void *stream);
+
 /* ------------------------------------------------------------------------------------------
  * Seed index of one reference sequence, built on the device: index__build_reference
  * (index.c:19-195, declared index.h:49-62), with the contiguous layout of index.c:157-188.

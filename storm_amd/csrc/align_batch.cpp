@@ -179,3 +179,14 @@ int storm_align_fetch(storm_align_plan* pl, storm_align_result* results, uint8_t
 void storm_align_plan_destroy(storm_align_plan* pl) { delete pl; }
 
 }  // extern "C"
+
+namespace storm {
+int align_plan_view(storm_align_plan* pl, AlignArgs& a, int& maxL, int& device, hipStream_t& st) {
+  if (!pl) return fail(STORM_ERR_INVALID, "bad plan");
+  a = pl->a;
+  maxL = pl->maxL;
+  device = pl->device;
+  st = pl->st;
+  return STORM_OK;
+}
+}  // namespace storm

@@ -23,6 +23,26 @@ __global__ __launch_bounds__(128) void k_align(AlignArgs a) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= a.n) return;
   const int64_t rec = a.base + k;
+  if (a.check_bounds) {
+    const int rs = a.rec_rs[rec];
+    bool ok = rs >= 0 && rs < 2 * a.r.n;
+    if (ok) {   // window codes win .. win + L + 2I - 1 and the look-ahead words inside the planes
+      const int64_t h = a.rec_diag[rec];
+      const int64_t win = (h - a.sc.I > -a.sc.d) ? h - a.sc.I : -a.sc.d;
+      ok = win + a.r.len[rs >> 1] + 2 * a.sc.I + 96 <= a.g.size + GPAD;
+    }
+    if (!ok) {
+      AlignRec x;
+      x.score = MIN_SCORE;
+      x.S = MIN_SCORE;
+      x.win = 0;
+      x.best_j = -2;
+      x.best_i0 = x.best_j0 = -1;
+      x.tb_len = 0;
+      a.out[rec] = x;
+      return;
+    }
+  }
   // score pass without the direction matrix; the records that can still be accepted (a few per
   // cent of the filter passes) are aligned again with directions and traceback
   AlignRec x = dp_one<IM, COLOR, false>(a.g, a.r, a.sc, a.rec_rs[rec], a.rec_diag[rec], nullptr, nullptr);

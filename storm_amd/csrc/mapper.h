@@ -234,6 +234,9 @@ int ensure_ref_resident(storm_mapper* m, size_t r);
 // DP + traceback (k_align) of records [0, n_rec) of `aa`, maxL = the longest read among them: launches
 // over chunks of whole wavefronts sized to a 256 MiB direction scratch, grown in `scratch`
 int align_records(AlignArgs aa, int maxL, int64_t n_rec, DevMem& scratch, hipStream_t st);
+// the resident inputs of a storm_align_plan (its kernel arguments with the plan's own records and
+// outputs), the longest read, its device and stream
+int align_plan_view(storm_align_plan* pl, AlignArgs& a, int& maxL, int& device, hipStream_t& st);
 // bytes of direction scratch align_records needs for n_rec records of reads up to maxL at -i I
 int64_t align_scratch_bytes(int I, int maxL, int64_t n_rec);
 int prep_device_reads(storm_mapper* m, int64_t n, int64_t code_stride, int64_t qual_stride, bool has_qual,

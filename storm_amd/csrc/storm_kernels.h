@@ -48,6 +48,9 @@ struct AlignArgs {
   int64_t scratch_stride;    // bytes per record: maxL x dp_dir_words(I) x 4
   AlignRec* out;
   uint8_t* tb;               // TBB bytes per record, traceback order (end to start)
+  int check_bounds;          // 1: records given by a caller on the device are checked first (read
+                             // strand inside r, window inside the padded genome); one outside is
+                             // written as never accepted with best_j = -2 and not aligned
 };
 
 struct PrepArgs {

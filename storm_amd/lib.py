@@ -96,6 +96,7 @@ def load() -> C.CDLL:
     lib.storm_align_run.argtypes = [P, I, C.POINTER(D)]
     lib.storm_align_fetch.argtypes = [P, P, P]
     lib.storm_align_plan_destroy.argtypes = [P]
+    lib.storm_align_run_records_noalloc.argtypes = [P, P, P, L, P, P, P]
     lib.storm_map_files.argtypes = [I, C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(Params), I, C.c_char_p, I,
                                     P, C.POINTER(MapStats)]
     lib.storm_mapper_create.argtypes = [I, C.c_char_p, C.POINTER(Params), C.POINTER(P)]
@@ -243,6 +244,14 @@ class AlignPlan:
         tb = np.zeros((self.n, TB_BYTES), np.uint8)
         check(load().storm_align_fetch(self.h, _ptr(res), _ptr(tb)))
         return res, tb
+
+    def run_records_noalloc(self, d_rec_rs: int, d_rec_diag: int, n_rec: int, d_results: int, d_tb: int,
+                            stream: int = 0) -> None:
+        """storm_align_run_records_noalloc: records and outputs are device addresses (e.g. the
+        data_ptr() of device tensors); stream 0 = the plan's stream, waited for."""
+        check(load().storm_align_run_records_noalloc(self.h, C.c_void_p(d_rec_rs), C.c_void_p(d_rec_diag), n_rec,
+                                                     C.c_void_p(d_results), C.c_void_p(d_tb),
+                                                     C.c_void_p(stream) if stream else None))
 
     def close(self):
         if self.h:
