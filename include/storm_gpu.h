@@ -279,6 +279,17 @@ int storm_mapper_map(storm_mapper *m, const storm_reads_view *reads, int mode, c
 int storm_mapper_map_hits(storm_mapper *m, const storm_reads_view *reads, int mode,
                           int32_t *score, int32_t *ref_id, int32_t *ref_start, int32_t *sense,
                           storm_map_stats *stats);
+/* The whole hit map of a mapping run in host memory, ready for SAM output: per read r and rank
+ * k < 4 the entry hits[4r + k] (HitInfoType, hit_map.h:17-40; ref_id = -1 for an empty slot) and,
+ * when tb is not NULL, the traceback of ranks k < max_hits (all four in the ordered mode) at
+ * tb + (r * max_hits + k) * STORM_TB_BYTES in alignment__traceback's packing (TRACEBACK_SEQ_SET,
+ * alignment.h:81-83; tb_len codes); bytes of empty slots are not written.  This is what
+ * hit_map__update leaves for hit_map__generate_SAM_output (hit_map.c:50-144, :197-504). */
+typedef struct storm_hit {
+  int32_t score, ref_id, ref_start, ref_end, read_start, read_end, sense, tb_len;
+} storm_hit;
+int storm_mapper_map_records(storm_mapper *m, const storm_reads_view *reads, int mode, storm_hit *hits, uint8_t *tb,
+                             storm_map_stats *stats);
 int64_t storm_mapper_free_bytes(storm_mapper *m);
 void storm_mapper_destroy(storm_mapper *m);
 

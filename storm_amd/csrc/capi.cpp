@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstddef>
 #include <chrono>
 #include <cstdarg>
 #include <cstdlib>
@@ -266,11 +267,11 @@ static int64_t pass_hit_limit() {
 
 // Reads [r0, r1) in passes of at most `chunk` reads, each halved while its raw hits exceed
 // pass_hit_limit().
-int run_passes(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int& pass_id, int64_t r0, int64_t r1,
-               TbStore& store, PassStats& stats, int64_t chunk) {
+int run_passes(storm_mapper* m, DevReads& dr, Slot* slots, const CellTarget& ct, int ref_id, int& pass_id, int64_t r0,
+               int64_t r1, PassStats& stats, int64_t chunk) {
   for (int64_t c0 = r0; c0 < r1;) {
     const int64_t c1 = std::min<int64_t>(r1, c0 + chunk);
-    const int rc = run_pass(m, dr, slots, ref_id, pass_id++, c0, c1, store, stats);
+    const int rc = run_pass(m, dr, slots, ct, ref_id, pass_id++, c0, c1, stats);
     if (rc == PASS_SPLIT) {
       chunk = std::max<int64_t>(1, (c1 - c0) / 2);
       continue;
@@ -284,8 +285,8 @@ int run_passes(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int& pass
 // One mapping pass of reads [r0, r1) (device order) against one reference sequence: the stages
 // of process_read (run.c:555-747) as separate launches, with host synchronisations to size
 // the buffers of the next stage.
-int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int pass_id, int64_t r0, int64_t r1,
-                    TbStore& store, PassStats& stats) {
+int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, const CellTarget& ct, int ref_id, int pass_id, int64_t r0,
+             int64_t r1, PassStats& stats) {
   PassScratch& P = m->ps;
   hipStream_t st = m->st;
   RefState& rs = *m->res->rstate[(size_t)ref_id];
@@ -467,41 +468,10 @@ int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int pass_id
   pa.slots = slots;
   HIPOK(launch_replay(pa, st));
   HIPOK(hipEventRecord(e[4], st));
-  // tracebacks still referenced by a slot go to the host store
-  if (!m->want_tb) {
-    HIPOK(hipEventRecord(e[5], st));
-    HIPOK(hipEventSynchronize(e[5]));
-    stats.ms_seed += elapsed(m, e[0], e[1]);
-    stats.ms_lookup += elapsed(m, e[0], e[6]);
-    stats.ms_filter += elapsed(m, e[1], e[2]);
-    stats.ms_align += elapsed(m, e[2], e[3]);
-    stats.ms_replay += elapsed(m, e[3], e[4]);
-    return STORM_OK;
-  }
-  const int64_t n_slots = 4 * dr.n;
-  HIPOK(P.mark_rec.ensure(rb * 4));
-  HIPOK(P.mark_slot.ensure(rb * 4));
-  HIPOK(P.mark_count.ensure(4));
-  HIPOK(hipMemsetAsync(P.mark_count.p, 0, 4, st));
-  HIPOK(launch_mark_referenced(slots + 4 * r0, 4 * (r1 - r0), pass_id, tb_ranks(m->p), P.mark_rec.as<int32_t>(),
-                               P.mark_slot.as<int32_t>(), P.mark_count.as<int32_t>(), st));
-  (void)n_slots;
-  int32_t n_mark = 0;
-  HIPOK(hipMemcpyAsync(&n_mark, P.mark_count.p, 4, hipMemcpyDeviceToHost, st));
-  HIPOK(hipStreamSynchronize(st));
-  if (n_mark > 0) {
-    HIPOK(P.gath.ensure((size_t)n_mark * TBB));
-    HIPOK(launch_gather_tb(P.tb.as<uint8_t>(), P.mark_rec.as<int32_t>(), n_mark, P.gath.as<uint8_t>(), st));
-    std::vector<int32_t> ids((size_t)n_mark);
-    std::vector<uint8_t> bytes((size_t)n_mark * TBB);
-    HIPOK(hipMemcpyAsync(ids.data(), P.mark_rec.p, ids.size() * 4, hipMemcpyDeviceToHost, st));
-    HIPOK(hipMemcpyAsync(bytes.data(), P.gath.p, bytes.size(), hipMemcpyDeviceToHost, st));
-    HIPOK(hipStreamSynchronize(st));
-    for (int32_t k = 0; k < n_mark; ++k) {
-      const int64_t key = ((int64_t)pass_id << 32) | (uint32_t)ids[(size_t)k];
-      store[key].assign(bytes.begin() + (size_t)k * TBB, bytes.begin() + (size_t)(k + 1) * TBB);
-    }
-  }
+  // the tracebacks this pass inserted go to the reads' cell store (device to device)
+  if (m->want_tb)
+    HIPOK(launch_store_cells(slots, 4 * r0, 4 * (r1 - r0), pass_id, tb_ranks(m->p), P.tb.as<uint8_t>(), ct.cells, ct.ids,
+                             st));
   HIPOK(hipEventRecord(e[5], st));
   HIPOK(hipEventSynchronize(e[5]));
   stats.ms_seed += elapsed(m, e[0], e[1]);
@@ -514,44 +484,66 @@ int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int pass_id
 
 // whole map call: reads of the view in the given order; results per read in view order
 
-// Host slots in view order from the device slots (device order).  Tracebacks of ranks < -M come
-// from the staged path's store (pass >= 0) or the fused path's gathered cells (PASS_FUSED).
 // seed-key lookups of a read set: per strand in the sense mask, per seed, its read positions
 static int64_t count_lookups(const storm_mapper* m, const storm_reads_view* v);
 
-static int assemble_slots(storm_mapper* m, const std::vector<Slot>& hs, const std::vector<int64_t>& order,
-                          const TbStore& store, const std::unordered_map<int64_t, const uint8_t*>* cells,
-                          std::vector<HostSlot>& out_slots, int64_t& mapped) {
-  const int64_t n = (int64_t)order.size();
-  out_slots.assign((size_t)(4 * n), HostSlot{});
+static_assert(sizeof(storm_hit) == sizeof(HitRec) && offsetof(storm_hit, tb_len) == offsetof(HitRec, tb_len),
+              "storm_hit mirrors HitRec");
+
+// The end of a map call: the device slots (device order) and the reads' traceback cells go to the
+// caller's arrays (ho: four int32 arrays; ro: whole records and tracebacks) or to host slots in
+// view order (SAM output).  Returns the mapped reads.
+static int finish_view(storm_mapper* m, const Slot* slots, const uint8_t* cells, const std::vector<int64_t>& order,
+                       int64_t n, const HitsOut* ho, const RecordsOut* ro, std::vector<HostSlot>& out_slots,
+                       int64_t& mapped) {
+  if (ho) return hits_to_host(m, slots, order, n, *ho, mapped);
+  const int M = tb_ranks(m->p);
+  const bool want_tb = m->want_tb && (!ro || ro->tb);
+  bool identity = true;
+  for (int64_t i = 0; i < n && identity; ++i) identity = order[(size_t)i] == i;
+  if (!identity) {
+    std::vector<int32_t> o32((size_t)n);
+    for (int64_t i = 0; i < n; ++i) o32[(size_t)i] = (int32_t)order[(size_t)i];
+    HIPOK(m->order_dev.ensure((size_t)n * 4));
+    HIPOK(hipMemcpyAsync(m->order_dev.p, o32.data(), (size_t)n * 4, hipMemcpyHostToDevice, m->st));
+    HIPOK(hipStreamSynchronize(m->st));
+  }
+  HIPOK(m->rec_dev.ensure((size_t)(4 * n) * sizeof(HitRec)));
+  if (want_tb) HIPOK(m->tb_dev.ensure((size_t)(n * M) * TBB));
+  HIPOK(launch_hit_records(slots, identity ? nullptr : m->order_dev.as<int32_t>(), n, M, cells, m->rec_dev.as<HitRec>(),
+                           want_tb ? m->tb_dev.as<uint8_t>() : nullptr, m->st));
+  std::vector<storm_hit> hv;
+  std::vector<uint8_t> tv;
+  storm_hit* hits = ro ? ro->hits : nullptr;
+  uint8_t* tb = ro ? ro->tb : nullptr;
+  if (!ro) {
+    hv.resize((size_t)(4 * n));
+    hits = hv.data();
+    if (want_tb) { tv.resize((size_t)(n * M) * TBB); tb = tv.data(); }
+  }
+  HIPOK(hipMemcpyAsync(hits, m->rec_dev.p, (size_t)(4 * n) * sizeof(HitRec), hipMemcpyDeviceToHost, m->st));
+  if (want_tb) HIPOK(hipMemcpyAsync(tb, m->tb_dev.p, (size_t)(n * M) * TBB, hipMemcpyDeviceToHost, m->st));
+  HIPOK(hipStreamSynchronize(m->st));
   mapped = 0;
-  for (int64_t i = 0; i < n; ++i) {
-    const int64_t j = order[(size_t)i];
-    if (hs[(size_t)(4 * i)].ref_id != -1) mapped++;
-    for (int k = 0; k < 4; ++k) {
-      const Slot& s = hs[(size_t)(4 * i + k)];
-      HostSlot& h = out_slots[(size_t)(4 * j + k)];
-      h.score = s.score; h.ref_id = s.ref_id; h.ref_start = s.ref_start; h.ref_end = s.ref_end;
-      h.read_start = s.read_start; h.read_end = s.read_end; h.sense = s.sense;
-      h.tb_len = s.tb_len;
-      if (s.ref_id == -1 || k >= tb_ranks(m->p) || !m->want_tb) continue;
-      if (s.pass == PASS_FUSED) {
-        if (!cells) return fail(STORM_ERR_DEVICE, "traceback of a fused hit was not gathered");
-        auto it = cells->find(4 * i + k);
-        if (it == cells->end()) return fail(STORM_ERR_DEVICE, "traceback of a fused hit was not gathered");
-        h.tb.assign(it->second, it->second + TBB);
-      } else {
-        auto it = store.find(((int64_t)s.pass << 32) | (uint32_t)s.rec);
-        if (it == store.end()) return fail(STORM_ERR_DEVICE, "traceback of a hit was not gathered");
-        h.tb = it->second;
-      }
+  for (int64_t i = 0; i < n; ++i) mapped += hits[4 * i].ref_id != -1;
+  if (ro) return STORM_OK;
+  out_slots.assign((size_t)(4 * n), HostSlot{});
+  for (int64_t q = 0; q < 4 * n; ++q) {
+    const storm_hit& h = hits[q];
+    HostSlot& o = out_slots[(size_t)q];
+    o.score = h.score; o.ref_id = h.ref_id; o.ref_start = h.ref_start; o.ref_end = h.ref_end;
+    o.read_start = h.read_start; o.read_end = h.read_end; o.sense = h.sense; o.tb_len = h.tb_len;
+    const int k = (int)(q & 3);
+    if (want_tb && h.ref_id != -1 && k < M) {
+      const uint8_t* src = tb + ((q >> 2) * M + k) * (int64_t)TBB;
+      o.tb.assign(src, src + TBB);
     }
   }
   return STORM_OK;
 }
 
 static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
-                           storm_map_stats* st_out, const HitsOut* ho) {
+                           storm_map_stats* st_out, const HitsOut* ho, const RecordsOut* ro) {
   const int64_t n = v->n_reads;
   if (n <= 0) return fail(STORM_ERR_INVALID, "no reads");
   // plain mode: equal-length batches (the reference maps one read length per run); the device
@@ -573,8 +565,9 @@ static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode,
   if (rc) return rc;
   DevMem& slots = m->plain_slots;
   HIPOK(slots.ensure((size_t)(4 * n) * sizeof(Slot)));
+  if (m->want_tb) HIPOK(m->plain_cells.ensure((size_t)(4 * n) * TBB));
   HIPOK(launch_slots_init(slots.as<Slot>(), 4 * n, m->st));
-  TbStore store;
+  const CellTarget ct{m->plain_cells.as<uint8_t>(), nullptr};
   PassStats ps;
   int pass_id = 0;
   const double ti = m->res->index_seconds;
@@ -583,22 +576,14 @@ static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode,
     if (rc) return rc;
     for (auto& b : batches) {
       // bound the per-pass buffers: at most 1M reads per launch group
-      rc = run_passes(m, dr, slots.as<Slot>(), (int)r, pass_id, b.first, b.second, store, ps, 1 << 20);
+      rc = run_passes(m, dr, slots.as<Slot>(), ct, (int)r, pass_id, b.first, b.second, ps, 1 << 20);
       if (rc) return rc;
     }
     if (m->res->refs.size() > 1 && !m->res->pinned) release_ref(m, r);
   }
   int64_t mapped = 0;
-  if (ho) {
-    rc = hits_to_host(m, slots.as<Slot>(), order, n, *ho, mapped);
-    if (rc) return rc;
-  } else {
-    std::vector<Slot> hs((size_t)(4 * n));
-    HIPOK(hipMemcpyAsync(hs.data(), slots.p, hs.size() * sizeof(Slot), hipMemcpyDeviceToHost, m->st));
-    HIPOK(hipStreamSynchronize(m->st));
-    rc = assemble_slots(m, hs, order, store, nullptr, out_slots, mapped);
-    if (rc) return rc;
-  }
+  rc = finish_view(m, slots.as<Slot>(), m->plain_cells.as<uint8_t>(), order, n, ho, ro, out_slots, mapped);
+  if (rc) return rc;
   if (st_out) {
     st_out->n_reads = n;
     st_out->n_mapped = mapped;
@@ -738,33 +723,34 @@ static int tune_fused(storm_mapper* m, int64_t n, int maxL, int max_grid, FusedT
 // Reads flagged by the fused kernel (LDS capacity exceeded) go through the staged kernels for
 // this reference: their planes and slots are gathered on the device into a sub-batch, mapped,
 // and the slots scattered back.
-int staged_subset(storm_mapper* m, const std::vector<int64_t>& dev_ids, int ref, int& pass_id, TbStore& store,
-                  PassStats& ps) {
+int staged_subset(storm_mapper* m, const std::vector<int64_t>& dev_ids, int ref, int& pass_id, PassStats& ps) {
   FusedState& F = m->fs;
   const int64_t k = (int64_t)dev_ids.size();
   std::vector<int32_t> ids((size_t)k);
   for (int64_t i = 0; i < k; ++i) ids[(size_t)i] = (int32_t)dev_ids[(size_t)i];
-  DevMem d_ids, sl;
-  DevReads dr;
-  HIPOK(d_ids.ensure((size_t)k * 4));
-  HIPOK(hipMemcpyAsync(d_ids.p, ids.data(), (size_t)k * 4, hipMemcpyHostToDevice, m->st));
+  DevReads& dr = m->sub_reads;   // grown when a fallback outgrows it, kept across calls
+  HIPOK(m->sub_ids.ensure((size_t)k * 4));
+  HIPOK(hipMemcpyAsync(m->sub_ids.p, ids.data(), (size_t)k * 4, hipMemcpyHostToDevice, m->st));
   HIPOK(dr.planes.ensure((size_t)(2 * k) * 2 * RW * 4));
   HIPOK(dr.qplanes.ensure((size_t)(2 * k) * QW * 4));
   HIPOK(dr.len.ensure((size_t)k * 4));
-  HIPOK(sl.ensure((size_t)(4 * k) * sizeof(Slot)));
+  HIPOK(m->sub_slots.ensure((size_t)(4 * k) * sizeof(Slot)));
   dr.n = k;
   dr.has_qual = F.reads.has_qual;
+  const int32_t* d_ids = m->sub_ids.as<int32_t>();
+  Slot* sl = m->sub_slots.as<Slot>();
   HIPOK(launch_subset_gather(F.reads.planes.as<uint32_t>(), F.reads.qplanes.as<uint32_t>(), F.reads.len.as<int32_t>(),
-                             F.slots.as<Slot>(), d_ids.as<int32_t>(), k, dr.planes.as<uint32_t>(),
-                             dr.qplanes.as<uint32_t>(), dr.len.as<int32_t>(), sl.as<Slot>(), m->st));
-  int rc = run_passes(m, dr, sl.as<Slot>(), ref, pass_id, 0, k, store, ps, k);
+                             F.slots.as<Slot>(), d_ids, k, dr.planes.as<uint32_t>(), dr.qplanes.as<uint32_t>(),
+                             dr.len.as<int32_t>(), sl, m->st));
+  const CellTarget ct{F.cells.as<uint8_t>(), d_ids};   // the subset's cells are the batch's
+  int rc = run_passes(m, dr, sl, ct, ref, pass_id, 0, k, ps, k);
   if (rc) return rc;
-  HIPOK(launch_subset_scatter(sl.as<Slot>(), d_ids.as<int32_t>(), k, F.slots.as<Slot>(), m->st));
-  HIPOK(hipStreamSynchronize(m->st));
+  HIPOK(launch_subset_scatter(sl, d_ids, k, F.slots.as<Slot>(), m->st));
+  HIPOK(hipStreamSynchronize(m->st));   // ids is released on return
   return STORM_OK;
 }
 
-int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback, TbStore& store, PassStats& ps,
+int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback, PassStats& ps,
                unsigned long long totals[4], int64_t& n_fallback) {
   FusedState& F = m->fs;
   ScoreDev sc{};
@@ -835,7 +821,7 @@ int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback,
     for (int64_t i = 0; i < n; ++i) if (ovf[(size_t)i]) over.push_back(i);
     n_fallback += (int64_t)over.size();
     if (!over.empty()) {
-      rc = fallback(over, (int)r, pass_id, store, ps);
+      rc = fallback(over, (int)r, pass_id, ps);
       if (rc) return rc;
     }
     if (n_refs > 1 && !m->res->pinned) {
@@ -850,7 +836,7 @@ static int batched_stats(storm_mapper* m, int64_t n, int64_t mapped, const unsig
                          const PassStats& ps, int64_t n_fallback, double t0, double ti, storm_map_stats* st_out);
 
 static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vector<HostSlot>& out_slots,
-                            storm_map_stats* st_out, const HitsOut* ho) {
+                            storm_map_stats* st_out, const HitsOut* ho, const RecordsOut* ro) {
   FusedState& F = m->fs;
   const int64_t n = v->n_reads;
   if (n <= 0) return fail(STORM_ERR_INVALID, "no reads");
@@ -865,47 +851,19 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
   const double ti = m->res->index_seconds;
   int rc = upload_reads(m, v, order, F.reads);
   if (rc) return rc;
-  TbStore store;
   PassStats ps;
   unsigned long long totals[4] = {0, 0, 0, 0};
   int64_t n_fallback = 0;
-  FallbackFn fb = [&](const std::vector<int64_t>& over, int ref, int& pass_id, TbStore& st, PassStats& pst) {
-    return staged_subset(m, over, ref, pass_id, st, pst);
+  FallbackFn fb = [&](const std::vector<int64_t>& over, int ref, int& pass_id, PassStats& pst) {
+    return staged_subset(m, over, ref, pass_id, pst);
   };
-  rc = fused_core(m, n, maxL, fb, store, ps, totals, n_fallback);
+  rc = fused_core(m, n, maxL, fb, ps, totals, n_fallback);
   if (rc) return rc;
   int64_t mapped = 0;
-  if (ho) {
-    rc = hits_to_host(m, F.slots.as<Slot>(), order, n, *ho, mapped);
-    if (rc) return rc;
-    return batched_stats(m, n, mapped, totals, ps, n_fallback, t0, ti, st_out);
-  }
-  // tracebacks of the fused slots that will be written (ranks < -M)
-  int32_t ng = 0;
-  std::vector<Slot> hs((size_t)(4 * n));
-  if (m->want_tb) {
-    HIPOK(launch_fused_reset(F.gcount.as<int32_t>(), nullptr, m->st));
-    HIPOK(launch_gather_cells(F.slots.as<Slot>(), 4 * n, tb_ranks(m->p), F.cells.as<uint8_t>(), F.gids.as<int32_t>(),
-                              F.gtb.as<uint8_t>(), F.gcount.as<int32_t>(), m->st));
-    HIPOK(hipMemcpyAsync(&ng, F.gcount.p, 4, hipMemcpyDeviceToHost, m->st));
-  }
-  HIPOK(hipMemcpyAsync(hs.data(), F.slots.p, hs.size() * sizeof(Slot), hipMemcpyDeviceToHost, m->st));
-  HIPOK(hipStreamSynchronize(m->st));
-  std::vector<int32_t> gids((size_t)ng);
-  std::vector<uint8_t> gtb((size_t)ng * TBB);
-  if (ng > 0) {
-    HIPOK(hipMemcpyAsync(gids.data(), F.gids.p, gids.size() * 4, hipMemcpyDeviceToHost, m->st));
-    HIPOK(hipMemcpyAsync(gtb.data(), F.gtb.p, gtb.size(), hipMemcpyDeviceToHost, m->st));
-    HIPOK(hipStreamSynchronize(m->st));
-  }
-  std::unordered_map<int64_t, const uint8_t*> cells;
-  cells.reserve((size_t)ng * 2 + 1);
-  for (int32_t q = 0; q < ng; ++q) cells[gids[(size_t)q]] = &gtb[(size_t)q * TBB];
-  rc = assemble_slots(m, hs, order, store, &cells, out_slots, mapped);
+  rc = finish_view(m, F.slots.as<Slot>(), F.cells.as<uint8_t>(), order, n, ho, ro, out_slots, mapped);
   if (rc) return rc;
   return batched_stats(m, n, mapped, totals, ps, n_fallback, t0, ti, st_out);
 }
-
 static int batched_stats(storm_mapper* m, int64_t n, int64_t mapped, const unsigned long long totals[4],
                          const PassStats& ps, int64_t n_fallback, double t0, double ti, storm_map_stats* st_out) {
   if (st_out) {
@@ -950,9 +908,9 @@ static int64_t count_lookups(const storm_mapper* m, const storm_reads_view* v) {
 }
 
 int map_view(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
-             storm_map_stats* st_out, const HitsOut* ho) {
-  int rc = (mode == STORM_MODE_BATCHED) ? map_view_batched(m, v, out_slots, st_out, ho)
-                                         : map_view_staged(m, v, mode, out_slots, st_out, ho);
+             storm_map_stats* st_out, const HitsOut* ho, const RecordsOut* ro) {
+  int rc = (mode == STORM_MODE_BATCHED) ? map_view_batched(m, v, out_slots, st_out, ho, ro)
+                                         : map_view_staged(m, v, mode, out_slots, st_out, ho, ro);
   if (rc == STORM_OK && st_out) st_out->n_lookups = count_lookups(m, v);
   return rc;
 }
@@ -1094,6 +1052,21 @@ int storm_mapper_map_hits(storm_mapper* m, const storm_reads_view* reads, int mo
   HitsOut ho{score, ref_id, ref_start, sense};
   m->want_tb = false;
   int rc = map_view(m, reads, mode, slots, &local, &ho);
+  m->want_tb = true;
+  if (rc) return rc;
+  if (stats) *stats = local;
+  return STORM_OK;
+}
+
+int storm_mapper_map_records(storm_mapper* m, const storm_reads_view* reads, int mode, storm_hit* hits, uint8_t* tb,
+                             storm_map_stats* stats) {
+  if (!m || !reads || !hits) return fail(STORM_ERR_INVALID, "null argument");
+  HIPOK(hipSetDevice(m->device));
+  std::vector<HostSlot> slots;
+  storm_map_stats local{};
+  RecordsOut ro{hits, tb};
+  m->want_tb = tb != nullptr;
+  int rc = map_view(m, reads, mode, slots, &local, nullptr, &ro);
   m->want_tb = true;
   if (rc) return rc;
   if (stats) *stats = local;

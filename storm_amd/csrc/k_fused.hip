@@ -284,7 +284,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 3 :
             uint4* dst = (uint4*)(a.cells + (read * 4 + s[k].cell) * (int64_t)TBB);
             const int n16 = ((s[k].tb_len + 1) / 2 + 15) / 16;
             for (int w = 0; w < n16; ++w) dst[w] = src[w];
-            s[k].pass = PASS_FUSED;
+            s[k].pass = PASS_CELL;
             s[k].rec = s[k].cell;
           }
           a.slots[read * 4 + k] = s[k];

@@ -76,26 +76,37 @@ __global__ __launch_bounds__(256) void k_slots_init(Slot* slots, int64_t n) {
   slots[t] = x;
 }
 
-// records of pass `pass` that a slot still references: their ids, compacted with an atomic
-// (only ranks < max_rank: a slot never moves back up, so later ranks are never written out)
-__global__ __launch_bounds__(256) void k_mark_referenced(const Slot* slots, int64_t n_slots, int pass, int max_rank,
-                                                         int32_t* out_rec, int32_t* out_slot, int32_t* counter) {
+// Tracebacks the pass inserted (ranks < max_rank: a slot never moves back up, so later ranks are
+// never written out) go from the pass's record buffer to the read's cell store; the slot then
+// names its cell (the four slots of a read hold a permutation of the four cells, so a cell is
+// overwritten only when its previous entry has been evicted).
+__global__ __launch_bounds__(256) void k_store_cells(Slot* slots, int64_t slot_begin, int64_t n_slots, int pass,
+                                                     int max_rank, const uint8_t* tb, uint8_t* cells, const int32_t* ids) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_slots) return;
-  if ((int)(t & 3) < max_rank && slots[t].pass == pass && slots[t].rec >= 0) {
-    const int o = atomicAdd(counter, 1);
-    out_rec[o] = slots[t].rec;
-    out_slot[o] = (int32_t)t;
-  }
+  const int64_t g = slot_begin + t;
+  if ((int)(g & 3) >= max_rank) return;
+  Slot s = slots[g];
+  if (s.ref_id == -1 || s.pass != pass) return;
+  const int64_t id = ids ? (int64_t)ids[g >> 2] : (g >> 2);
+  const uint4* src = (const uint4*)(tb + (int64_t)s.rec * TBB);
+  uint4* dst = (uint4*)(cells + (id * 4 + s.cell) * (int64_t)TBB);
+#pragma unroll
+  for (int w = 0; w < TBB / 16; ++w) dst[w] = src[w];
+  s.pass = PASS_CELL;
+  s.rec = s.cell;
+  slots[g] = s;
 }
 
-__global__ __launch_bounds__(256) void k_gather_tb(const uint8_t* tb, int tbb, const int32_t* recs, int n, uint8_t* out) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)n * tbb) return;
-  const int64_t e = t / tbb, b = t % tbb;
-  out[t] = tb[(int64_t)recs[e] * tbb + b];
+hipError_t launch_store_cells(Slot* slots, int64_t slot_begin, int64_t n_slots, int pass, int max_rank, const uint8_t* tb,
+                              uint8_t* cells, const int32_t* ids, hipStream_t st) {
+  if (n_slots == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_store_cells, dim3((unsigned)((n_slots + 255) / 256)), dim3(256), 0, st, slots, slot_begin, n_slots,
+                     pass, max_rank, tb, cells, ids);
+  return hipGetLastError();
 }
 
+static_assert(TBB % 16 == 0, "cells are copied in 16-byte units");
 
 // reads ids[0..k) of a batch (strand planes, quality planes, length, the four slots) into a
 // contiguous subset, for the staged pass over the reads the fused kernel handed back
@@ -195,40 +206,35 @@ hipError_t launch_slots_init(Slot* slots, int64_t n, hipStream_t st) {
   hipLaunchKernelGGL(k_slots_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, slots, n);
   return hipGetLastError();
 }
-hipError_t launch_mark_referenced(const Slot* slots, int64_t n_slots, int pass, int max_rank, int32_t* out_rec,
-                                  int32_t* out_slot, int32_t* counter, hipStream_t st) {
-  if (n_slots == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_mark_referenced, dim3((unsigned)((n_slots + 255) / 256)), dim3(256), 0, st, slots, n_slots, pass,
-                     max_rank, out_rec, out_slot, counter);
-  return hipGetLastError();
-}
 
-// tracebacks of the fused path's slots (ranks < max_rank) into a compact list
-__global__ __launch_bounds__(256) void k_gather_cells(const Slot* slots, int64_t n_slots, int max_rank, const uint8_t* cells,
-                                                      int32_t* out_slot, uint8_t* out_tb, int32_t* counter) {
+
+// The whole hit map in the caller's order: one thread per slot writes its record and, for a hit
+// at rank < max_rank, copies its traceback cell (16 bytes at a time) to the dense output.
+__global__ __launch_bounds__(256) void k_hit_records(const Slot* slots, const int32_t* order, int64_t n, int max_rank,
+                                                     const uint8_t* cells, HitRec* out, uint8_t* tb_out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_slots) return;
+  if (t >= 4 * n) return;
+  const int k = (int)(t & 3);
+  const int64_t row = order ? (int64_t)order[t >> 2] : (t >> 2);
   const Slot s = slots[t];
-  if ((int)(t & 3) < max_rank && s.ref_id != -1 && s.pass == PASS_FUSED) {
-    const int o = atomicAdd(counter, 1);
-    out_slot[o] = (int32_t)t;
-    const uint32_t* src = (const uint32_t*)(cells + ((t & ~3LL) + s.cell) * (int64_t)TBB);
-    uint32_t* dst = (uint32_t*)(out_tb + (int64_t)o * TBB);
-    for (int w = 0; w < TBB / 4; ++w) dst[w] = src[w];
+  HitRec h;
+  h.score = s.score; h.ref_id = s.ref_id; h.ref_start = s.ref_start; h.ref_end = s.ref_end;
+  h.read_start = s.read_start; h.read_end = s.read_end; h.sense = s.sense;
+  h.tb_len = s.ref_id == -1 ? 0 : s.tb_len;
+  out[row * 4 + k] = h;
+  if (tb_out && k < max_rank && s.ref_id != -1 && s.pass == PASS_CELL) {
+    const uint4* src = (const uint4*)(cells + ((t & ~3LL) + s.cell) * (int64_t)TBB);
+    uint4* dst = (uint4*)(tb_out + (row * max_rank + k) * (int64_t)TBB);
+#pragma unroll
+    for (int w = 0; w < TBB / 16; ++w) dst[w] = src[w];
   }
 }
 
-hipError_t launch_gather_cells(const Slot* slots, int64_t n_slots, int max_rank, const uint8_t* cells, int32_t* out_slot,
-                               uint8_t* out_tb, int32_t* counter, hipStream_t st) {
-  if (n_slots == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_gather_cells, dim3((unsigned)((n_slots + 255) / 256)), dim3(256), 0, st, slots, n_slots, max_rank,
-                     cells, out_slot, out_tb, counter);
-  return hipGetLastError();
-}
-hipError_t launch_gather_tb(const uint8_t* tb, const int32_t* recs, int n, uint8_t* out, hipStream_t st) {
+hipError_t launch_hit_records(const Slot* slots, const int32_t* order, int64_t n, int max_rank, const uint8_t* cells,
+                              HitRec* out, uint8_t* tb_out, hipStream_t st) {
   if (n == 0) return hipSuccess;
-  const int64_t total = (int64_t)n * TBB;
-  hipLaunchKernelGGL(k_gather_tb, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, tb, (int)TBB, recs, n, out);
+  hipLaunchKernelGGL(k_hit_records, dim3((unsigned)((4 * n + 255) / 256)), dim3(256), 0, st, slots, order, n, max_rank,
+                     cells, out, tb_out);
   return hipGetLastError();
 }
 

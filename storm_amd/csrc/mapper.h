@@ -105,7 +105,6 @@ struct PassScratch {   // reused across passes
   DevMem seg_count, seg_off, hits, kept, overflow, n_overflow, seg_begin, seg_end, sorted;
   DevMem cand_off, cand_rs, cand_diag, cand_pos, pass, pass_scan;
   DevMem rec_rs, rec_diag, rec_off, recs, tb, align_scratch, cub_tmp;
-  DevMem mark_rec, mark_slot, mark_count, gath;
 };
 
 // reads resident on the device for one map call (global read ids = device order)
@@ -174,8 +173,12 @@ struct PassStats {
   float ms_seed = 0, ms_filter = 0, ms_align = 0, ms_replay = 0, ms_lookup = 0;
 };
 
-// traceback bytes of records still referenced by a slot, kept on the host per (pass, rec)
-using TbStore = std::unordered_map<int64_t, std::vector<uint8_t>>;
+// where a staged pass leaves the tracebacks it inserts: the cell store of the reads (4 x TBB bytes
+// per read), read r's cells at (ids ? ids[r] : r) * 4
+struct CellTarget {
+  uint8_t* cells;
+  const int32_t* ids;
+};
 
 }  // namespace storm
 
@@ -193,6 +196,10 @@ struct storm_mapper {
   storm::DevReads stage;       // the caller's reads as uploaded, before the device reordering
   storm::DevReads plain_reads; // the plain path's reads in device order, reused across calls
   storm::DevMem plain_slots;   // the plain path's hit map slots, reused across calls
+  storm::DevMem plain_cells;   // the plain path's traceback cells (4 per read), reused across calls
+  storm::DevMem rec_dev, tb_dev;   // the hit map and tracebacks in the caller's order, on their way out
+  storm::DevReads sub_reads;   // the batched path's staged fallback: the handed-back reads' planes,
+  storm::DevMem sub_ids, sub_slots;   // their ids in the batch and their slots
   storm::DevMem order_dev;     // device row -> caller's read index
   storm::DevMem hits_dev;      // per-slot results on their way to the caller
   storm::SharedTune* shared_tune = nullptr;   // tuner results of sibling contexts, or null
@@ -246,32 +253,37 @@ int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<i
 // exceed PASS_HIT_LIMIT
 constexpr int PASS_SPLIT = 1;
 constexpr int64_t PASS_HIT_LIMIT = 1LL << 30;
-int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int pass_id, int64_t r0, int64_t r1,
-             TbStore& store, PassStats& stats);
+int run_pass(storm_mapper* m, DevReads& dr, Slot* slots, const CellTarget& ct, int ref_id, int pass_id, int64_t r0,
+             int64_t r1, PassStats& stats);
 // reads [r0, r1) in passes of at most `chunk` reads, split further as PASS_SPLIT asks
-int run_passes(storm_mapper* m, DevReads& dr, Slot* slots, int ref_id, int& pass_id, int64_t r0, int64_t r1,
-               TbStore& store, PassStats& stats, int64_t chunk);
+int run_passes(storm_mapper* m, DevReads& dr, Slot* slots, const CellTarget& ct, int ref_id, int& pass_id, int64_t r0,
+               int64_t r1, PassStats& stats, int64_t chunk);
 // the staged kernels over reads dev_ids of m->fs.reads, slots read from and written back to m->fs.slots
-int staged_subset(storm_mapper* m, const std::vector<int64_t>& dev_ids, int ref, int& pass_id, TbStore& store,
-                  PassStats& ps);
+int staged_subset(storm_mapper* m, const std::vector<int64_t>& dev_ids, int ref, int& pass_id, PassStats& ps);
 
 // Reads that the fused kernel handed back (device ids of m->fs.reads) for reference `ref`; the
 // callback maps them with the staged kernels and leaves their slots in m->fs.slots.
-using FallbackFn = std::function<int(const std::vector<int64_t>& dev_ids, int ref, int& pass_id, TbStore& store,
-                                     PassStats& ps)>;
+using FallbackFn = std::function<int(const std::vector<int64_t>& dev_ids, int ref, int& pass_id, PassStats& ps)>;
 // The batched path over the n reads already in m->fs.reads (device order, longest maxL): slots
 // initialised, then per reference the captured fused kernel and the fallback of its overflow
 // reads.  Slots end in m->fs.slots.  totals: candidates, filter passes, accepted, raw hits.
-int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback, TbStore& store, PassStats& ps,
+int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback, PassStats& ps,
                unsigned long long totals[4], int64_t& n_fallback);
 
 // caller arrays for the per-read slots (n_reads x 4 each, any may be null)
 struct HitsOut {
   int32_t *score, *ref_id, *ref_start, *sense;
 };
+// caller arrays for whole hit records: hits[4 * read + rank], tracebacks of ranks < tb_ranks(p)
+// at tb + (read * tb_ranks(p) + rank) * TBB (tb may be null)
+struct RecordsOut {
+  storm_hit* hits;
+  uint8_t* tb;
+};
 void length_order(const storm_reads_view* v, std::vector<int64_t>& order, bool& identity);
-// ho != null: the slots go straight to the caller's arrays (no host slot assembly, no tracebacks)
+// ho != null: the slots go straight to the caller's arrays (no host slot assembly, no tracebacks);
+// ro != null: the records and tracebacks go to the caller's arrays instead of out_slots
 int map_view(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
-             storm_map_stats* st_out, const HitsOut* ho = nullptr);
+             storm_map_stats* st_out, const HitsOut* ho = nullptr, const RecordsOut* ro = nullptr);
 
 }  // namespace storm

@@ -276,14 +276,13 @@ static int run_batch(storm_server* s, storm_server::Lane& ln, const std::vector<
   HIPOK(launch_serve_gather(ga, m->st));
   int rc = prep_device_reads(m, n, SERVE_CS, SERVE_CS, true, true, d);
   if (rc) return rc;
-  TbStore store;
   PassStats ps;
   unsigned long long totals[4];
   int64_t n_fallback = 0;
-  FallbackFn fb = [&](const std::vector<int64_t>& over, int ref, int& pass_id, TbStore& sto, PassStats& pst) {
-    return staged_subset(m, over, ref, pass_id, sto, pst);
+  FallbackFn fb = [&](const std::vector<int64_t>& over, int ref, int& pass_id, PassStats& pst) {
+    return staged_subset(m, over, ref, pass_id, pst);
   };
-  rc = fused_core(m, n, maxL, fb, store, ps, totals, n_fallback);
+  rc = fused_core(m, n, maxL, fb, ps, totals, n_fallback);
   if (rc) return rc;
   HIPOK(launch_serve_scatter(m->fs.slots.as<Slot>(), ln.rows.as<ServeRow>(), n, ln.regions.as<ServeRegion>(), m->st));
   HIPOK(hipStreamSynchronize(m->st));

@@ -18,10 +18,15 @@ constexpr int TBB = (MAXL + MAX_I + 2) / 2;   // traceback bytes per record, 2 c
 struct Slot {
   int32_t score, ref_id, ref_start, ref_end;
   int32_t read_start, read_end, sense, pass, rec, tb_len;
-  int32_t cell;   // traceback cell of this read (0..3) in the fused path's store; the four slots of a
-                  // read always hold a permutation of the four cells
+  int32_t cell;   // traceback cell of this read (0..3) in the read's cell store (4 x TBB bytes per
+                  // read); the four slots of a read always hold a permutation of the four cells
 };
-constexpr int PASS_FUSED = -2;   // Slot.pass of entries whose traceback lives in the cell store
+constexpr int PASS_CELL = -2;    // Slot.pass of entries whose traceback lives in the read's cell store
+
+// one entry of the whole hit map in the caller's order (storm_hit of storm_gpu.h, HitInfoType)
+struct HitRec {
+  int32_t score, ref_id, ref_start, ref_end, read_start, read_end, sense, tb_len;
+};
 
 struct FilterArgs {
   GenomeDev g;
@@ -236,11 +241,15 @@ hipError_t launch_compact_candidates(const CompactArgs& a, hipStream_t st);
 hipError_t launch_make_records(const RecArgs& a, hipStream_t st);
 hipError_t launch_replay(const ReplayArgs& a, hipStream_t st);
 hipError_t launch_slots_init(Slot* slots, int64_t n, hipStream_t st);
-hipError_t launch_mark_referenced(const Slot* slots, int64_t n_slots, int pass, int max_rank, int32_t* out_rec,
-                                  int32_t* out_slot, int32_t* counter, hipStream_t st);
-hipError_t launch_gather_cells(const Slot* slots, int64_t n_slots, int max_rank, const uint8_t* cells, int32_t* out_slot,
-                               uint8_t* out_tb, int32_t* counter, hipStream_t st);
-hipError_t launch_gather_tb(const uint8_t* tb, const int32_t* recs, int n, uint8_t* out, hipStream_t st);
+// tracebacks of the entries pass `pass` left in ranks < max_rank of slots [slot_begin, slot_begin +
+// n_slots), from the pass's records (tb, TBB bytes per record) into the read's cell store at
+// cells + (id * 4 + cell) * TBB, id = ids ? ids[read] : read; those entries become PASS_CELL
+hipError_t launch_store_cells(Slot* slots, int64_t slot_begin, int64_t n_slots, int pass, int max_rank, const uint8_t* tb,
+                              uint8_t* cells, const int32_t* ids, hipStream_t st);
+// the n reads' slots (device order) as HitRec rows order[i] (or i) of out, and the tracebacks of
+// ranks < max_rank into tb_out + (row * max_rank + rank) * TBB (tb_out may be null)
+hipError_t launch_hit_records(const Slot* slots, const int32_t* order, int64_t n, int max_rank, const uint8_t* cells,
+                              HitRec* out, uint8_t* tb_out, hipStream_t st);
 hipError_t launch_subset_gather(const uint32_t* planes, const uint32_t* qplanes, const int32_t* len, const Slot* slots,
                                 const int32_t* ids, int64_t k, uint32_t* sp, uint32_t* sq, int32_t* sl, Slot* ss,
                                 hipStream_t st);

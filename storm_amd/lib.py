@@ -102,6 +102,7 @@ def load() -> C.CDLL:
     lib.storm_mapper_create.argtypes = [I, C.c_char_p, C.POINTER(Params), C.POINTER(P)]
     lib.storm_mapper_map.argtypes = [P, C.POINTER(ReadsView), I, C.c_char_p, I, P, C.POINTER(MapStats)]
     lib.storm_mapper_map_hits.argtypes = [P, C.POINTER(ReadsView), I, P, P, P, P, C.POINTER(MapStats)]
+    lib.storm_mapper_map_records.argtypes = [P, C.POINTER(ReadsView), I, P, P, C.POINTER(MapStats)]
     lib.storm_mapper_free_bytes.argtypes = [P]
     lib.storm_mapper_free_bytes.restype = L
     lib.storm_mapper_destroy.argtypes = [P]
@@ -197,6 +198,9 @@ ALIGN_RESULT = np.dtype([("score", "<i4"), ("bound", "<i4"), ("win", "<i4"), ("b
                          ("best_i0", "<i2"), ("best_j0", "<i2"), ("tb_len", "<i2")])
 TB_BYTES = 144
 MIN_SCORE = -16383
+# storm_hit (HitInfoType, hit_map.h:17-40)
+HIT_RECORD = np.dtype([(f, "<i4") for f in ("score", "ref_id", "ref_start", "ref_end", "read_start", "read_end",
+                                             "sense", "tb_len")])
 
 
 class AlignPlan:
@@ -404,6 +408,27 @@ class Mapper:
                                            _ptr(out["ref_start"]), _ptr(out["sense"]), C.byref(st)))
         del keep
         return out, st.as_dict()
+
+    def map_records(self, reads, mode: int = STORM_MODE_BATCHED, lo: int = 0, hi: int | None = None,
+                    with_tb: bool = True, out=None):
+        """The whole hit map in host memory (storm_mapper_map_records): (hits, tb, stats) with hits an
+        (n, 4) HIT_RECORD array and tb an (n, M, TB_BYTES) u8 array (M = -M, 4 in the ordered mode)
+        or None.  `out` = (hits, tb) arrays to reuse (at least n rows)."""
+        v, keep = self.view(reads, lo, hi)
+        n = v.n_reads
+        M = self.params.max_hits or 4
+        if out is None:
+            hits = np.zeros((n, 4), HIT_RECORD)
+            tb = np.zeros((n, M, TB_BYTES), np.uint8) if with_tb else None
+        else:
+            hits, tb = out
+            hits = hits[:n]
+            tb = tb[:n] if (with_tb and tb is not None) else None
+        st = MapStats()
+        check(load().storm_mapper_map_records(self.h, C.byref(v), mode, _ptr(hits),
+                                              None if tb is None else _ptr(tb), C.byref(st)))
+        del keep
+        return hits, tb, st.as_dict()
 
     def free_bytes(self) -> int:
         return int(load().storm_mapper_free_bytes(self.h))

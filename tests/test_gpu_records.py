@@ -1,0 +1,54 @@
+"""The whole hit map in host memory (storm_mapper_map_records): the plain (staged) and batched
+(fused) paths return identical records and tracebacks, the records agree with
+storm_mapper_map_hits, and every traceback is a complete glocal path: its read-consuming codes
+(match, mismatch, color fix, insertion; alignment.h:81-83) cover the read from read_start to its
+last code, and its genome-consuming codes the span ref_start..ref_end (hit_map.h:17-40)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEEDS_N = "####--#-#--##--####;#####--###-#-###;###-##-#######"
+SEEDS_C = "####----##--##-####;####-###--#----####;####-####-####"
+
+
+def _codes(tb_row, n):
+    return [(tb_row[i >> 1] >> (0 if i & 1 else 4)) & 0xF for i in range(n)]
+
+
+@pytest.mark.parametrize("color,I,M", [(0, 7, 4), (0, 3, 1), (1, 3, 2), (1, 7, 4)], ids=lambda v: str(v))
+def test_records_plain_equal_batched(color, I, M, tmp_path):
+    from storm_amd import lib, synth
+    g = synth.gc_window_genome(1_000_000, 41 + I)
+    gpath = str(tmp_path / "g.fa")
+    synth.write_fasta_fast(gpath, "chr1", g)
+    if color:
+        pr = synth.fast_color_reads(g, 3000, (35, 149), 7 + I, snp=0.01, color_err=0.03, qlo=10, qhi=35)
+    else:
+        pr = synth.fast_nucleotide_reads(g, 3000, (36, 150), 7 + I, sub=0.015, indel=0.002, qlo=15, qhi=40)
+    m = lib.Mapper(gpath, color=color, allowed_indels=I, max_hits=M, z=85 if color else 100, t=100,
+                   seeds=SEEDS_C if color else SEEDS_N)
+    print(f"records color={color} -i {I} -M {M}", flush=True)
+    hp, tp, _ = m.map_records(pr, mode=lib.STORM_MODE_PLAIN)
+    hb, tb, _ = m.map_records(pr, mode=lib.STORM_MODE_BATCHED)
+    hh, _ = m.map_hits(pr, mode=lib.STORM_MODE_BATCHED)
+    m.close()
+    assert np.array_equal(hp, hb)
+    for f in ("score", "ref_id", "ref_start", "sense"):
+        assert np.array_equal(hb[f], hh[f]), f
+    n_tb = 0
+    for r in range(len(pr.len)):
+        for k in range(M):
+            h = hb[r, k]
+            if h["ref_id"] < 0:
+                continue
+            n = int(h["tb_len"])
+            nb = (n + 1) // 2
+            assert np.array_equal(tp[r, k, :nb], tb[r, k, :nb]), (r, k)
+            c = _codes(tb[r, k], n)
+            read_used = sum(1 for x in c if x == 0x3 or (x & 0xC) in (0x4, 0x8, 0xC))
+            ref_used = sum(1 for x in c if x == 0x1 or x == 0x3 or (x & 0xC) in (0x4, 0x8))
+            assert read_used == int(h["read_end"]) - int(h["read_start"]) + 1, (r, k)
+            assert ref_used == int(h["ref_end"]) - int(h["ref_start"]) + 1, (r, k)
+            n_tb += 1
+    assert n_tb > 0.5 * len(pr.len)
