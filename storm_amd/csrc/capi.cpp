@@ -72,8 +72,14 @@ int ensure_ref_resident(storm_mapper* m, size_t r) {
   const HostRef& ref = m->res->refs[r];
   const double t0 = now_s();
   DevMem dcode;
-  int rc = upload_genome(ref.code.data(), ref.mask.empty() ? nullptr : ref.mask.data(), ref.size, rs.g, m->st, &dcode);
-  if (rc) return rc;
+  int rc = STORM_OK;
+  if (!rs.g.lh.p) {   // planes and the codes the index build reads
+    rc = upload_genome(ref.code.data(), ref.mask.empty() ? nullptr : ref.mask.data(), ref.size, rs.g, m->st, &dcode);
+    if (rc) return rc;
+  } else {            // planes kept from an earlier pass: the codes only
+    HIPOK(dcode.ensure((size_t)ref.size + 64));
+    if (ref.size > 0) HIPOK(hipMemcpyAsync(dcode.p, ref.code.data(), (size_t)ref.size, hipMemcpyHostToDevice, m->st));
+  }
   rc = build_indexes(ref, dcode, m->res->seeds, m->p.stdev_distance, rs.idx, m->st);
   if (rc) return rc;
   m->res->index_seconds += now_s() - t0;
@@ -81,11 +87,23 @@ int ensure_ref_resident(storm_mapper* m, size_t r) {
   return STORM_OK;
 }
 
+// the seed indexes of a sequence are released after its pass; its planes (0.375 byte per code) stay
+// for the SAM output and the next call
 static void release_ref(storm_mapper* m, size_t r) {
   RefState& rs = *m->res->rstate[r];
-  rs.g.lh.release(); rs.g.mask.release();
   rs.idx.clear();
   rs.resident = false;
+}
+
+int ensure_planes(storm_mapper* m) {
+  for (size_t r = 0; r < m->res->refs.size(); ++r) {
+    RefState& rs = *m->res->rstate[r];
+    if (rs.g.lh.p) continue;
+    const HostRef& ref = m->res->refs[r];
+    const int rc = upload_genome(ref.code.data(), ref.mask.empty() ? nullptr : ref.mask.data(), ref.size, rs.g, m->st);
+    if (rc) return rc;
+  }
+  return STORM_OK;
 }
 
 static SeedSetDev seed_set(const RefState& rs) {
@@ -543,7 +561,7 @@ static int finish_view(storm_mapper* m, const Slot* slots, const uint8_t* cells,
 }
 
 static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
-                           storm_map_stats* st_out, const HitsOut* ho, const RecordsOut* ro) {
+                           storm_map_stats* st_out, const HitsOut* ho, const RecordsOut* ro, const FinishFn* fin) {
   const int64_t n = v->n_reads;
   if (n <= 0) return fail(STORM_ERR_INVALID, "no reads");
   // plain mode: equal-length batches (the reference maps one read length per run); the device
@@ -582,8 +600,11 @@ static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode,
     if (m->res->refs.size() > 1 && !m->res->pinned) release_ref(m, r);
   }
   int64_t mapped = 0;
-  rc = finish_view(m, slots.as<Slot>(), m->plain_cells.as<uint8_t>(), order, n, ho, ro, out_slots, mapped);
+  const double tf = now_s();
+  if (fin) rc = (*fin)(FinishCtx{slots.as<Slot>(), m->plain_cells.as<uint8_t>(), &dr, &order, n}, mapped);
+  else rc = finish_view(m, slots.as<Slot>(), m->plain_cells.as<uint8_t>(), order, n, ho, ro, out_slots, mapped);
   if (rc) return rc;
+  m->finish_seconds = fin ? now_s() - tf : 0;
   if (st_out) {
     st_out->n_reads = n;
     st_out->n_mapped = mapped;
@@ -592,7 +613,7 @@ static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode,
     st_out->n_accepted = 0;
     st_out->n_hits = ps.hits;
     st_out->seconds_index = m->res->index_seconds - ti;
-    st_out->seconds_map = now_s() - t0 - st_out->seconds_index;
+    st_out->seconds_map = now_s() - t0 - st_out->seconds_index - m->finish_seconds;
     st_out->gpu_ms_seed = ps.ms_seed;
     st_out->gpu_ms_filter = ps.ms_filter;
     st_out->gpu_ms_align = ps.ms_align;
@@ -836,7 +857,7 @@ static int batched_stats(storm_mapper* m, int64_t n, int64_t mapped, const unsig
                          const PassStats& ps, int64_t n_fallback, double t0, double ti, storm_map_stats* st_out);
 
 static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vector<HostSlot>& out_slots,
-                            storm_map_stats* st_out, const HitsOut* ho, const RecordsOut* ro) {
+                            storm_map_stats* st_out, const HitsOut* ho, const RecordsOut* ro, const FinishFn* fin) {
   FusedState& F = m->fs;
   const int64_t n = v->n_reads;
   if (n <= 0) return fail(STORM_ERR_INVALID, "no reads");
@@ -860,8 +881,11 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
   rc = fused_core(m, n, maxL, fb, ps, totals, n_fallback);
   if (rc) return rc;
   int64_t mapped = 0;
-  rc = finish_view(m, F.slots.as<Slot>(), F.cells.as<uint8_t>(), order, n, ho, ro, out_slots, mapped);
+  const double tf = now_s();
+  if (fin) rc = (*fin)(FinishCtx{F.slots.as<Slot>(), F.cells.as<uint8_t>(), &F.reads, &order, n}, mapped);
+  else rc = finish_view(m, F.slots.as<Slot>(), F.cells.as<uint8_t>(), order, n, ho, ro, out_slots, mapped);
   if (rc) return rc;
+  m->finish_seconds = fin ? now_s() - tf : 0;
   return batched_stats(m, n, mapped, totals, ps, n_fallback, t0, ti, st_out);
 }
 static int batched_stats(storm_mapper* m, int64_t n, int64_t mapped, const unsigned long long totals[4],
@@ -875,7 +899,7 @@ static int batched_stats(storm_mapper* m, int64_t n, int64_t mapped, const unsig
     st_out->n_hits = (int64_t)totals[3] + ps.hits;
     st_out->n_fallback = n_fallback;
     st_out->seconds_index = m->res->index_seconds - ti;
-    st_out->seconds_map = now_s() - t0 - st_out->seconds_index;
+    st_out->seconds_map = now_s() - t0 - st_out->seconds_index - m->finish_seconds;
     st_out->gpu_ms_seed = ps.ms_seed;
     st_out->gpu_ms_filter = ps.ms_filter;
     st_out->gpu_ms_align = ps.ms_align;
@@ -908,9 +932,10 @@ static int64_t count_lookups(const storm_mapper* m, const storm_reads_view* v) {
 }
 
 int map_view(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
-             storm_map_stats* st_out, const HitsOut* ho, const RecordsOut* ro) {
-  int rc = (mode == STORM_MODE_BATCHED) ? map_view_batched(m, v, out_slots, st_out, ho, ro)
-                                         : map_view_staged(m, v, mode, out_slots, st_out, ho, ro);
+             storm_map_stats* st_out, const HitsOut* ho, const RecordsOut* ro, const FinishFn* fin) {
+  m->finish_seconds = 0;
+  int rc = (mode == STORM_MODE_BATCHED) ? map_view_batched(m, v, out_slots, st_out, ho, ro, fin)
+                                         : map_view_staged(m, v, mode, out_slots, st_out, ho, ro, fin);
   if (rc == STORM_OK && st_out) st_out->n_lookups = count_lookups(m, v);
   return rc;
 }
@@ -1022,23 +1047,65 @@ static int build_host_reads(const storm_reads_view* v, int color, HostReads& hr)
 
 int storm_mapper_map(storm_mapper* m, const storm_reads_view* reads, int mode, const char* sam_path, int argc,
                      char** argv, storm_map_stats* stats) {
-  if (!m || !reads) return fail(STORM_ERR_INVALID, "null argument");
+  if (!m || !reads || !sam_path) return fail(STORM_ERR_INVALID, "null argument");
   HIPOK(hipSetDevice(m->device));
   std::vector<HostSlot> slots;
   storm_map_stats local{};
-  int rc = map_view(m, reads, mode, slots, &local);
-  if (rc) return rc;
-  const double t0 = now_s();
-  HostReads hr;
-  build_host_reads(reads, m->p.color, hr);
-  std::string err;
-  rc = write_sam(sam_path, m->res->refs, hr, slots, m->p, argc, argv, err);
-  if (rc) return fail(rc, "%s", err.c_str());
-  if (m->p.unmapped_path) {   // run.c:1078-1086
-    rc = write_unmapped_fastq(m->p.unmapped_path, hr, slots, m->p, err);
+  int rc = STORM_OK;
+  if (m->p.max_hits == 0) {   // ordered mode: the genome map is built on the host from the hit map
+    rc = map_view(m, reads, mode, slots, &local);
+    if (rc) return rc;
+    const double t0 = now_s();
+    HostReads hr;
+    build_host_reads(reads, m->p.color, hr);
+    std::string err;
+    rc = write_sam_ordered(sam_path, m->res->refs, hr, slots, m->p, argc, argv, err);
     if (rc) return fail(rc, "%s", err.c_str());
+    if (m->p.unmapped_path) {   // run.c:1078-1086
+      rc = write_unmapped_fastq(m->p.unmapped_path, hr, slots, m->p, err);
+      if (rc) return fail(rc, "%s", err.c_str());
+    }
+    local.seconds_sam = now_s() - t0;
+    if (stats) *stats = local;
+    return STORM_OK;
   }
-  local.seconds_sam = now_s() - t0;
+  // -M: the header on the host, the records formatted on the device straight from the hit map
+  FILE* out = fopen(sam_path, "wb");
+  if (!out) return fail(STORM_ERR_IO, "cannot write %s", sam_path);
+  sam_header(out, m->res->refs, m->p.color != 0, reads->db_name ? reads->db_name : "", "unsorted", argc, argv);
+  std::vector<uint8_t> unmapped;
+  FinishFn fin = [&](const FinishCtx& fc, int64_t& mapped) {
+    int r = sam_device(m, reads, fc, out, mapped);
+    if (r || !m->p.unmapped_path) return r;
+    // -O: the reads with an empty hit map row (run.c:1078-1086)
+    std::vector<storm_hit> hits((size_t)(4 * fc.n));
+    RecordsOut ro{hits.data(), nullptr};
+    std::vector<HostSlot> none;
+    const bool keep = m->want_tb;
+    m->want_tb = false;
+    int64_t mp = 0;
+    r = finish_view(m, fc.slots, fc.cells, *fc.order, fc.n, nullptr, &ro, none, mp);
+    m->want_tb = keep;
+    unmapped.resize((size_t)fc.n);
+    for (int64_t i = 0; i < fc.n; ++i) unmapped[(size_t)i] = hits[(size_t)(4 * i)].ref_id == -1;
+    return r;
+  };
+  rc = map_view(m, reads, mode, slots, &local, nullptr, nullptr, &fin);
+  const bool write_ok = !ferror(out);
+  if (fclose(out) != 0 || !write_ok) return rc ? rc : fail(STORM_ERR_IO, "error writing %s", sam_path);
+  if (rc) return rc;
+  local.seconds_sam = m->finish_seconds;
+  if (m->p.unmapped_path) {
+    const double t0 = now_s();
+    HostReads hr;
+    build_host_reads(reads, m->p.color, hr);
+    std::vector<HostSlot> rows((size_t)(4 * hr.n), HostSlot{});
+    for (int64_t i = 0; i < hr.n; ++i) rows[(size_t)(4 * i)].ref_id = unmapped[(size_t)i] ? -1 : 0;
+    std::string err;
+    rc = write_unmapped_fastq(m->p.unmapped_path, hr, rows, m->p, err);
+    if (rc) return fail(rc, "%s", err.c_str());
+    local.seconds_sam += now_s() - t0;
+  }
   if (stats) *stats = local;
   return STORM_OK;
 }

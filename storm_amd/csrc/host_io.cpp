@@ -621,26 +621,34 @@ SamFormat::SamFormat(const std::vector<HostRef>& refs_, const HostReads& reads_,
   }
 }
 
-void SamFormat::header(FILE* out, const char* sort_order, int argc, char** argv) const {
+void sam_header(FILE* out, const std::vector<HostRef>& refs, bool color, const std::string& db_name,
+                const char* sort_order, int argc, char** argv) {   // hit_map.c:297-315
   fprintf(out, "@HD\tVN:1.4\tSO:%s\n", sort_order);
   for (auto& r : refs)
     fprintf(out, "@SQ\tSN:%s\tLN:%ld\n", r.name.empty() ? "*" : r.name.c_str(), (long)(r.size + (color ? 1 : 0)));
-  const char* rg = reads.db_name.empty() ? "*" : reads.db_name.c_str();
+  const char* rg = db_name.empty() ? "*" : db_name.c_str();
   fprintf(out, "@RG\tID:%s\tSM:%s\n", rg, rg);
   fprintf(out, "@PG\tID:SToRM\tVN:0.0101\tCL:\"");
   for (int a = 0; a < argc; ++a) fprintf(out, "%s ", argv[a]);
   fprintf(out, "\"\n");
 }
 
-int SamFormat::mapq(const HostSlot* row, int k) const {   // hit_map.c:374-399
+void SamFormat::header(FILE* out, const char* sort_order, int argc, char** argv) const {
+  sam_header(out, refs, color, reads.db_name, sort_order, argc, argv);
+}
+
+int sam_mapq(int diff, int delta_m) {   // hit_map.c:374-399, n = diff / (m - x)
   const double lg = log(.3);
-  const int delta_m = p.match - p.mismatch;
-  const int s1 = row[0].score, s2 = (k == 0) ? std::max(row[1].score, p.t) : row[k].score;
-  const double nn = (k == 0) ? (double)(s1 - s2) / (double)delta_m : (double)(s2 - s1) / (double)delta_m;
+  const double nn = (double)diff / (double)delta_m;
   int mq = -4.343 * (nn * lg);
   mq = (int)(mq - 4.343 * lg);
   mq = mq >= 0 ? mq : 0;
   return mq < 254 ? mq : 254;
+}
+
+int SamFormat::mapq(const HostSlot* row, int k) const {   // rank 0 against max(rank 1, t), later ranks against rank 0
+  const int s1 = row[0].score, s2 = (k == 0) ? std::max(row[1].score, p.t) : row[k].score;
+  return sam_mapq(k == 0 ? s1 - s2 : s2 - s1, p.match - p.mismatch);
 }
 
 void SamFormat::record(std::string& line, int64_t id, const HostSlot* row, int k, int flag, std::string& cigar,
@@ -739,60 +747,6 @@ int write_unmapped_fastq(const char* path, const HostReads& reads, const std::ve
     }
     line += '\n';
   });
-  if (fclose(out) != 0 || !ok) { err = std::string("error writing ") + path; return STORM_ERR_IO; }
-  return STORM_OK;
-}
-
-int write_sam(const char* path, const std::vector<HostRef>& refs, const HostReads& reads,
-              const std::vector<HostSlot>& slots, const storm_params& p, int argc, char** argv, std::string& err) {
-  if (p.max_hits == 0) return write_sam_ordered(path, refs, reads, slots, p, argc, argv, err);
-  FILE* out = fopen(path, "wb");
-  if (!out) { err = std::string("cannot write ") + path; return STORM_ERR_IO; }
-  static const char BASES[4] = {'A', 'C', 'G', 'T'};
-  static const short UB[4] = {2, 7, 10, 93};
-  const bool color = p.color != 0;
-  const SamFormat sf(refs, reads, p);
-  sf.header(out, "unsorted", argc, argv);
-  const int M = p.max_hits;
-  // the records of read `id` appended to `line` (hit_map.c:325-504); reads are formatted in
-  // parallel chunks and written in read order
-  auto format_read = [&](int64_t id, std::string& line, std::string& cigar, std::string& seq) {
-    const HostSlot* row = &slots[(size_t)(4 * id)];
-    const int L = reads.len[id];
-    const bool rq = reads.has_qual && !(!reads.no_qual.empty() && reads.no_qual[(size_t)id]);
-    const uint8_t* qs = rq ? &reads.qual[(size_t)(id * reads.qual_stride)] : nullptr;
-    auto reportable = [&](int k) {   // hit_map.c:359-370
-      return k < M && row[k].ref_start != -1 &&
-             (p.uniqueness == 0 || ((k + 1 < M) && (row[k].score - row[k + 1].score) >= p.uniqueness) || (k + 1 == M));
-    };
-    char buf[32];
-    if (row[0].ref_id != -1) {
-      int nrep = 0;
-      for (int k = 0; reportable(k); ++k) nrep++;
-      for (int k = 0; reportable(k); ++k) {
-        sf.record(line, id, row, k, (row[k].sense == -1 ? 0x10 : 0) | (k ? 0x100 : 0), cigar, seq);
-        snprintf(buf, sizeof buf, "\tNH:i:%d\n", nrep);
-        line += buf;
-      }
-    } else if (p.print_all) {
-      const std::string& nm = reads.names[(size_t)id];
-      if (color) {
-        line += nm;
-        line += "\t4\t*\t0\t255\t*\t*\t0\t0\t*\t*\n";
-      } else {
-        line += nm;
-        line += "\t4\t*\t0\t255\t*\t*\t0\t0\t";
-        const uint8_t* cs = &reads.codes[(size_t)(id * reads.code_stride)];
-        for (int q = 0; q < L; ++q) line += BASES[get2(cs, q)];
-        line += '\t';
-        if (qs) {
-          for (int q = 0; q < L; ++q) { int v = p.qual_base + UB[get2(qs, q)]; line += (char)(v < 126 ? v : 126); }
-        } else line += '*';
-        line += '\n';
-      }
-    }
-  };
-  const bool ok = write_blocks(out, reads.n, format_read);
   if (fclose(out) != 0 || !ok) { err = std::string("error writing ") + path; return STORM_ERR_IO; }
   return STORM_OK;
 }

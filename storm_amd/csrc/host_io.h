@@ -57,11 +57,6 @@ struct HostSlot {
   int tb_len;
 };
 
-// hit_map__generate_SAM_output (hit_map.c:197-504) for reads in order; slots[4*r + k].  With
-// p.max_hits == 0 (no -M) the ordered output, write_sam_ordered.
-int write_sam(const char* path, const std::vector<HostRef>& refs, const HostReads& reads,
-              const std::vector<HostSlot>& slots, const storm_params& p, int argc, char** argv, std::string& err);
-
 // the ordered mode (no -M, the reference's default): the genome map built from the hit map,
 // greedy (p.greedy, -G) or contextual, and its coordinate-ordered SAM (genome_map.c:186-800)
 int write_sam_ordered(const char* path, const std::vector<HostRef>& refs, const HostReads& reads,
@@ -73,7 +68,8 @@ int write_sam_ordered(const char* path, const std::vector<HostRef>& refs, const 
 int write_unmapped_fastq(const char* path, const HostReads& reads, const std::vector<HostSlot>& slots,
                          const storm_params& p, std::string& err);
 
-// SAM text shared by the two writers: header lines, MAPQ, and the fields of a mapped record
+// SAM text of the ordered writer: header lines, MAPQ, and the fields of a mapped record (the -M
+// records are formatted on the device, k_sam.hip)
 struct SamFormat {
   SamFormat(const std::vector<HostRef>& refs, const HostReads& reads, const storm_params& p);
   void header(FILE* out, const char* sort_order, int argc, char** argv) const;
@@ -87,6 +83,12 @@ struct SamFormat {
   bool color;
   std::vector<std::vector<uint8_t>> sym;   // color: base before each position of each reference
 };
+
+// the SAM header lines (hit_map.c:297-315): @HD with sort_order, @SQ per sequence, @RG, @PG with argv
+void sam_header(FILE* out, const std::vector<HostRef>& refs, bool color, const std::string& db_name,
+                const char* sort_order, int argc, char** argv);
+// MAPQ of a record whose score is `diff` above its comparison score (hit_map.c:374-399)
+int sam_mapq(int diff, int delta_m);
 
 // items [0, n) formatted by fmt(item, text, cigar scratch, seq scratch) on up to 16 threads in
 // blocks, written to out in item order; false when a write failed

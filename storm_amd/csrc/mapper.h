@@ -92,6 +92,14 @@ struct RefState {
 // Genome (one encoding: nucleotides or colors) and its seed indexes, shared by every mapper that
 // maps against it.  With `pinned` every sequence stays resident; otherwise a multi-sequence genome
 // is uploaded one sequence at a time (each pass releases the previous one).
+// Tables the SAM formatter reads, built once per resident genome: the sequences' planes, their
+// names, and in color space the base symbol before every 32-code word of each sequence.
+struct SamTables {
+  std::mutex mu;
+  bool ready = false;
+  DevMem genomes, sym_ck, sym_off, names, name_off;
+};
+
 struct Resident {
   std::vector<HostRef> refs;
   std::vector<SeedSpec> seeds;
@@ -99,6 +107,30 @@ struct Resident {
   std::vector<std::unique_ptr<RefState>> rstate;
   double index_seconds = 0;
   bool pinned = false;
+  SamTables sam;
+};
+
+// pinned host memory owned by a C++ object
+struct PinnedMem {
+  void* p = nullptr;
+  size_t n = 0;
+  PinnedMem() = default;
+  PinnedMem(const PinnedMem&) = delete;
+  PinnedMem& operator=(const PinnedMem&) = delete;
+  ~PinnedMem() { release(); }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= n && p) return hipSuccess;
+    release();
+    const size_t b = bytes ? bytes : 16;
+    hipError_t e = hipHostMalloc(&p, b, hipHostMallocDefault);
+    if (e == hipSuccess) n = b;
+    return e;
+  }
 };
 
 struct PassScratch {   // reused across passes
@@ -198,6 +230,12 @@ struct storm_mapper {
   storm::DevMem plain_slots;   // the plain path's hit map slots, reused across calls
   storm::DevMem plain_cells;   // the plain path's traceback cells (4 per read), reused across calls
   storm::DevMem rec_dev, tb_dev;   // the hit map and tracebacks in the caller's order, on their way out
+  // SAM records formatted on the device: the caller's read names, the MAPQ table of the scores,
+  // caller row -> device row, per-row byte counts and offsets, the text and its host copy
+  storm::DevMem sam_names, sam_name_off, sam_no_qual, sam_lut, sam_inv, sam_lens, sam_offs, sam_text, sam_mapped;
+  int sam_lut_dm = 0;
+  storm::PinnedMem sam_host;
+  double finish_seconds = 0;   // time spent in a map call's finish step (SAM formatting)
   storm::DevReads sub_reads;   // the batched path's staged fallback: the handed-back reads' planes,
   storm::DevMem sub_ids, sub_slots;   // their ids in the batch and their slots
   storm::DevMem order_dev;     // device row -> caller's read index
@@ -274,6 +312,23 @@ int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback,
 struct HitsOut {
   int32_t *score, *ref_id, *ref_start, *sense;
 };
+// the state a map call leaves on the device for its output: slots (device order, 4 per read), the
+// reads' traceback cells, the reads (device order) and device row -> caller row
+struct FinishCtx {
+  const Slot* slots;
+  const uint8_t* cells;
+  const DevReads* reads;
+  const std::vector<int64_t>* order;
+  int64_t n;
+};
+// replaces the default finish (records to host) when given; sets *mapped
+using FinishFn = std::function<int(const FinishCtx& fc, int64_t& mapped)>;
+// the planes of every sequence resident (the SAM formatter reads them all)
+int ensure_planes(storm_mapper* m);
+// the -M SAM records of a finished map call of view v, formatted on the device and written to out
+// (after the header) in v's order
+int sam_device(storm_mapper* m, const storm_reads_view* v, const FinishCtx& fc, FILE* out, int64_t& mapped);
+
 // caller arrays for whole hit records: hits[4 * read + rank], tracebacks of ranks < tb_ranks(p)
 // at tb + (read * tb_ranks(p) + rank) * TBB (tb may be null)
 struct RecordsOut {
@@ -284,6 +339,7 @@ void length_order(const storm_reads_view* v, std::vector<int64_t>& order, bool& 
 // ho != null: the slots go straight to the caller's arrays (no host slot assembly, no tracebacks);
 // ro != null: the records and tracebacks go to the caller's arrays instead of out_slots
 int map_view(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
-             storm_map_stats* st_out, const HitsOut* ho = nullptr, const RecordsOut* ro = nullptr);
+             storm_map_stats* st_out, const HitsOut* ho = nullptr, const RecordsOut* ro = nullptr,
+             const FinishFn* fin = nullptr);
 
 }  // namespace storm

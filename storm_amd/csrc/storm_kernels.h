@@ -241,6 +241,36 @@ hipError_t launch_compact_candidates(const CompactArgs& a, hipStream_t st);
 hipError_t launch_make_records(const RecArgs& a, hipStream_t st);
 hipError_t launch_replay(const ReplayArgs& a, hipStream_t st);
 hipError_t launch_slots_init(Slot* slots, int64_t n, hipStream_t st);
+// SAM records of the -M output formatted on the device (k_sam.hip), caller rows [row0, row0 + n)
+struct SamArgs {
+  const Slot* slots;          // device order, 4 per read
+  const uint8_t* cells;       // traceback cells, 4 x TBB bytes per read (device order)
+  const int32_t* inv;         // caller row -> device row, or null (the same order)
+  int64_t row0, n;
+  const uint8_t* codes;       // device order: packed forward codes (codes.h:125-130)
+  int64_t code_stride;
+  const uint8_t* qual;        // device order: packed 2-bit levels, or null
+  int64_t qual_stride;
+  const uint8_t* first_qual;  // color: level of the primer-side color (device order), or null
+  const int32_t* len;         // device order
+  const uint8_t* no_qual;     // caller order: 1 = the read has no quality, or null
+  const char* names;          // caller order: row r's name at names[name_off[r] .. name_off[r + 1]),
+  const int64_t* name_off;    // or null: "r<name_base + r>"
+  int64_t name_base;
+  const GenomeDev* genomes;   // per reference sequence
+  const uint8_t* sym_ck;      // color: per sequence, the base symbol before each 32-code word
+  const int64_t* sym_off;     // (hit_map.c:236-249), sequence r's at sym_ck + sym_off[r]
+  const char* ref_names;      // RNAME of sequence r at ref_names[ref_name_off[r] .. ref_name_off[r + 1])
+  const int32_t* ref_name_off;
+  const uint8_t* mapq_lut;    // MAPQ by score difference + 32768 (65536 entries)
+  int M, uniqueness, print_all, qual_base, color, t;
+  int64_t* lens;              // count pass: bytes of row row0 + v
+  int64_t* mapped;            // count pass: reads with a hit (added to), or null
+  const int64_t* offs;        // write pass: where row row0 + v starts in text
+  char* text;
+};
+hipError_t launch_sam(const SamArgs& a, bool write, hipStream_t st);
+
 // tracebacks of the entries pass `pass` left in ranks < max_rank of slots [slot_begin, slot_begin +
 // n_slots), from the pass's records (tb, TBB bytes per record) into the read's cell store at
 // cells + (id * 4 + cell) * TBB, id = ids ? ids[read] : read; those entries become PASS_CELL
