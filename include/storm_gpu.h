@@ -326,6 +326,8 @@ typedef struct storm_server_config {
   const char *seeds_color;        /* NULL = reference defaults (seed.h:31) */
   int stdev_distance;         /* -l (default 100) */
   int ignore_lowercase;       /* -j */
+  int sam_records;            /* 1 (default): each request also gets its -M SAM records (hit_map.c:325-504,
+                                 formatted on the device with its batches); 0: hits only */
 } storm_server_config;
 
 typedef struct storm_server_stats {
@@ -351,13 +353,23 @@ typedef struct storm_request_result {
 
 void storm_server_config_default(storm_server_config *cfg);
 int storm_server_create(int device, const char *genome_path, const storm_server_config *cfg, storm_server **out);
-/* Copies the reads (storm_reads_view: codes, qual, len, first_qual; names are not used) and queues
+/* Copies the reads (storm_reads_view: codes, qual, len, first_qual, and with sam_records names and
+ * no_qual) and queues
  * them; *id identifies the request.  Fails at once when the request alone exceeds the budget. */
 int storm_server_submit(storm_server *s, const storm_reads_view *reads, const storm_params *params, int64_t *id);
 /* Blocks until request `id` is done, then copies its slots (n_reads x 4, as storm_mapper_map_hits)
  * into the non-NULL arrays and releases it.  Returns the request's status. */
 int storm_server_wait(storm_server *s, int64_t id, int32_t *score, int32_t *ref_id, int32_t *ref_start, int32_t *sense,
                       storm_request_result *out);
+/* As storm_server_wait, and the request's SAM records: the lines hit_map__generate_SAM_output writes
+ * for its reads (hit_map.c:325-504, no header lines), in the request's read order, as
+ * storm_mapper_map writes them for the same reads and parameters; names from the submitted view
+ * (NULL names: "r<row>").  *sam is allocated by the library (release it with storm_free), NUL
+ * terminated, *sam_bytes long.  The request's latency runs to the last record being ready.  Needs
+ * sam_records = 1. */
+int storm_server_wait_sam(storm_server *s, int64_t id, char **sam, int64_t *sam_bytes, int32_t *score, int32_t *ref_id,
+                          int32_t *ref_start, int32_t *sense, storm_request_result *out);
+void storm_free(void *p);
 int storm_server_get_stats(storm_server *s, storm_server_stats *out);
 /* Finishes every submitted request, then stops the lanes and frees the device memory. */
 void storm_server_destroy(storm_server *s);

@@ -511,9 +511,8 @@ static_assert(sizeof(storm_hit) == sizeof(HitRec) && offsetof(storm_hit, tb_len)
 // The end of a map call: the device slots (device order) and the reads' traceback cells go to the
 // caller's arrays (ho: four int32 arrays; ro: whole records and tracebacks) or to host slots in
 // view order (SAM output).  Returns the mapped reads.
-static int finish_view(storm_mapper* m, const Slot* slots, const uint8_t* cells, const std::vector<int64_t>& order,
-                       int64_t n, const HitsOut* ho, const RecordsOut* ro, std::vector<HostSlot>& out_slots,
-                       int64_t& mapped) {
+int finish_view(storm_mapper* m, const Slot* slots, const uint8_t* cells, const std::vector<int64_t>& order, int64_t n,
+                const HitsOut* ho, const RecordsOut* ro, std::vector<HostSlot>& out_slots, int64_t& mapped) {
   if (ho) return hits_to_host(m, slots, order, n, *ho, mapped);
   const int M = tb_ranks(m->p);
   const bool want_tb = m->want_tb && (!ro || ro->tb);
@@ -1074,8 +1073,15 @@ int storm_mapper_map(storm_mapper* m, const storm_reads_view* reads, int mode, c
   if (!out) return fail(STORM_ERR_IO, "cannot write %s", sam_path);
   sam_header(out, m->res->refs, m->p.color != 0, reads->db_name ? reads->db_name : "", "unsorted", argc, argv);
   std::vector<uint8_t> unmapped;
+  std::string names;
+  std::vector<int64_t> name_off;
+  const SamRows rows = sam_rows_of_view(reads, names, name_off);
+  SamSink to_file = [&](const char* text, int64_t bytes, int64_t, int64_t, const int64_t*) {
+    if (bytes && fwrite(text, 1, (size_t)bytes, out) != (size_t)bytes) return fail(STORM_ERR_IO, "SAM write failed");
+    return STORM_OK;
+  };
   FinishFn fin = [&](const FinishCtx& fc, int64_t& mapped) {
-    int r = sam_device(m, reads, fc, out, mapped);
+    int r = sam_device(m, rows, fc, to_file, false, mapped);
     if (r || !m->p.unmapped_path) return r;
     // -O: the reads with an empty hit map row (run.c:1078-1086)
     std::vector<storm_hit> hits((size_t)(4 * fc.n));

@@ -325,9 +325,31 @@ struct FinishCtx {
 using FinishFn = std::function<int(const FinishCtx& fc, int64_t& mapped)>;
 // the planes of every sequence resident (the SAM formatter reads them all)
 int ensure_planes(storm_mapper* m);
-// the -M SAM records of a finished map call of view v, formatted on the device and written to out
-// (after the header) in v's order
-int sam_device(storm_mapper* m, const storm_reads_view* v, const FinishCtx& fc, FILE* out, int64_t& mapped);
+// The rows a SAM formatting pass covers, in output order (row r = caller row r of the map call):
+// names packed (row r's at names[name_off[r] .. name_off[r + 1]); names = null: "r<r>"), which
+// rows have no quality (QUAL '*'), and the strides of the device copies of the reads.
+struct SamRows {
+  int64_t n;
+  const char* names;
+  const int64_t* name_off;
+  const uint8_t* no_qual;     // per row, or null
+  bool has_qual, has_first_qual;
+  int64_t code_stride, qual_stride;
+};
+// receives the text of rows [row0, row0 + n) and, when the pass asks for them, the rows' start
+// offsets in it (n + 1 entries, relative to text)
+using SamSink = std::function<int(const char* text, int64_t bytes, int64_t row0, int64_t n, const int64_t* row_off)>;
+// the -M SAM records of a finished map call formatted on the device, handed to `sink` in chunks of
+// rows in row order; with_offsets: the sink also gets each row's offset
+int sam_device(storm_mapper* m, const SamRows& rows, const FinishCtx& fc, const SamSink& sink, bool with_offsets,
+               int64_t& mapped);
+// SamRows of a storm_reads_view (names packed into `names` / `name_off`)
+SamRows sam_rows_of_view(const storm_reads_view* v, std::string& names, std::vector<int64_t>& name_off);
+
+// the default finish: slots to the caller's arrays (ho), whole records (ro), or host slots
+struct RecordsOut;
+int finish_view(storm_mapper* m, const Slot* slots, const uint8_t* cells, const std::vector<int64_t>& order, int64_t n,
+                const HitsOut* ho, const RecordsOut* ro, std::vector<HostSlot>& out_slots, int64_t& mapped);
 
 // caller arrays for whole hit records: hits[4 * read + rank], tracebacks of ranks < tb_ranks(p)
 // at tb + (read * tb_ranks(p) + rank) * TBB (tb may be null)

@@ -60,10 +60,40 @@ int sam_tables(storm_mapper* m) {
 
 }  // namespace
 
-int sam_device(storm_mapper* m, const storm_reads_view* v, const FinishCtx& fc, FILE* out, int64_t& mapped) {
+SamRows sam_rows_of_view(const storm_reads_view* v, std::string& names, std::vector<int64_t>& name_off) {
+  SamRows r{};
+  r.n = v->n_reads;
+  if (v->names) {
+    name_off.resize((size_t)r.n + 1);
+    size_t total = 0;
+    for (int64_t i = 0; i < r.n; ++i) total += strlen(v->names[i]);
+    names.resize(total);
+    size_t o = 0;
+    for (int64_t i = 0; i < r.n; ++i) {
+      name_off[(size_t)i] = (int64_t)o;
+      const size_t l = strlen(v->names[i]);
+      memcpy(&names[o], v->names[i], l);
+      o += l;
+    }
+    name_off[(size_t)r.n] = (int64_t)o;
+    r.names = names.data();
+    r.name_off = name_off.data();
+  }
+  r.no_qual = v->qual ? v->no_qual : nullptr;
+  r.has_qual = v->qual != nullptr;
+  r.has_first_qual = v->first_qual != nullptr;
+  r.code_stride = v->code_stride;
+  r.qual_stride = v->qual ? v->qual_stride : 0;
+  return r;
+}
+
+int sam_device(storm_mapper* m, const SamRows& rows, const FinishCtx& fc, const SamSink& sink, bool with_offsets,
+               int64_t& mapped) {
   const int64_t n = fc.n;
   mapped = 0;
   if (n == 0) return STORM_OK;
+  if (rows.n != n) return fail(STORM_ERR_INVALID, "SAM rows (%lld) differ from the reads mapped (%lld)", (long long)rows.n,
+                               (long long)n);
   int rc = sam_tables(m);
   if (rc) return rc;
   hipStream_t st = m->st;
@@ -88,31 +118,16 @@ int sam_device(storm_mapper* m, const storm_reads_view* v, const FinishCtx& fc, 
     HIPOK(m->sam_inv.ensure((size_t)n * 4));
     HIPOK(hipMemcpyAsync(m->sam_inv.p, inv.data(), (size_t)n * 4, hipMemcpyHostToDevice, st));
   }
-  // names in the caller's order
-  std::string names;
-  std::vector<int64_t> noff;
-  if (v->names) {
-    noff.resize((size_t)n + 1);
-    size_t total = 0;
-    for (int64_t i = 0; i < n; ++i) total += strlen(v->names[i]);
-    names.resize(total);
-    size_t o = 0;
-    for (int64_t i = 0; i < n; ++i) {
-      noff[(size_t)i] = (int64_t)o;
-      const size_t l = strlen(v->names[i]);
-      memcpy(&names[o], v->names[i], l);
-      o += l;
-    }
-    noff[(size_t)n] = (int64_t)o;
-    HIPOK(m->sam_names.ensure(names.size() + 1));
-    HIPOK(m->sam_name_off.ensure(noff.size() * 8));
-    HIPOK(hipMemcpyAsync(m->sam_names.p, names.data(), names.size(), hipMemcpyHostToDevice, st));
-    HIPOK(hipMemcpyAsync(m->sam_name_off.p, noff.data(), noff.size() * 8, hipMemcpyHostToDevice, st));
+  if (rows.names) {
+    const int64_t nb = rows.name_off[n];
+    HIPOK(m->sam_names.ensure((size_t)nb + 1));
+    HIPOK(m->sam_name_off.ensure((size_t)(n + 1) * 8));
+    if (nb) HIPOK(hipMemcpyAsync(m->sam_names.p, rows.names, (size_t)nb, hipMemcpyHostToDevice, st));
+    HIPOK(hipMemcpyAsync(m->sam_name_off.p, rows.name_off, (size_t)(n + 1) * 8, hipMemcpyHostToDevice, st));
   }
-  const bool no_qual = v->qual && v->no_qual;
-  if (no_qual) {
+  if (rows.no_qual) {
     HIPOK(m->sam_no_qual.ensure((size_t)n));
-    HIPOK(hipMemcpyAsync(m->sam_no_qual.p, v->no_qual, (size_t)n, hipMemcpyHostToDevice, st));
+    HIPOK(hipMemcpyAsync(m->sam_no_qual.p, rows.no_qual, (size_t)n, hipMemcpyHostToDevice, st));
   }
   HIPOK(m->sam_mapped.ensure(8));
   HIPOK(hipMemsetAsync(m->sam_mapped.p, 0, 8, st));
@@ -123,14 +138,14 @@ int sam_device(storm_mapper* m, const storm_reads_view* v, const FinishCtx& fc, 
   a.cells = fc.cells;
   a.inv = identity ? nullptr : m->sam_inv.as<int32_t>();
   a.codes = d.codes.as<uint8_t>();
-  a.code_stride = v->code_stride;
-  a.qual = v->qual ? d.qual.as<uint8_t>() : nullptr;
-  a.qual_stride = v->qual ? v->qual_stride : 0;
-  a.first_qual = (m->p.color && v->first_qual) ? d.first_qual.as<uint8_t>() : nullptr;
+  a.code_stride = rows.code_stride;
+  a.qual = rows.has_qual ? d.qual.as<uint8_t>() : nullptr;
+  a.qual_stride = rows.has_qual ? rows.qual_stride : 0;
+  a.first_qual = (m->p.color && rows.has_first_qual) ? d.first_qual.as<uint8_t>() : nullptr;
   a.len = d.len.as<int32_t>();
-  a.no_qual = no_qual ? m->sam_no_qual.as<uint8_t>() : nullptr;
-  a.names = v->names ? m->sam_names.as<char>() : nullptr;
-  a.name_off = v->names ? m->sam_name_off.as<int64_t>() : nullptr;
+  a.no_qual = rows.no_qual ? m->sam_no_qual.as<uint8_t>() : nullptr;
+  a.names = rows.names ? m->sam_names.as<char>() : nullptr;
+  a.name_off = rows.names ? m->sam_name_off.as<int64_t>() : nullptr;
   a.name_base = 0;
   a.genomes = T.genomes.as<GenomeDev>();
   a.sym_ck = m->p.color ? T.sym_ck.as<uint8_t>() : nullptr;
@@ -151,6 +166,7 @@ int sam_device(storm_mapper* m, const storm_reads_view* v, const FinishCtx& fc, 
   size_t tmpb = 0;
   HIPOK(scan_exclusive_i64(nullptr, nullptr, ch + 1, nullptr, tmpb, st));
   HIPOK(m->ps.cub_tmp.ensure(tmpb + 256));
+  std::vector<int64_t> offs;
   for (int64_t r0 = 0; r0 < n; r0 += ch) {
     const int64_t c = std::min(ch, n - r0);
     a.row0 = r0;
@@ -160,18 +176,24 @@ int sam_device(storm_mapper* m, const storm_reads_view* v, const FinishCtx& fc, 
     HIPOK(launch_sam(a, false, st));
     size_t tb = m->ps.cub_tmp.n;
     HIPOK(scan_exclusive_i64(a.lens, m->sam_offs.as<int64_t>(), c + 1, m->ps.cub_tmp.p, tb, st));
+    if (with_offsets) {
+      offs.resize((size_t)c + 1);
+      HIPOK(hipMemcpyAsync(offs.data(), m->sam_offs.p, (size_t)(c + 1) * 8, hipMemcpyDeviceToHost, st));
+    }
     int64_t total = 0;
     HIPOK(hipMemcpyAsync(&total, m->sam_offs.as<int64_t>() + c, 8, hipMemcpyDeviceToHost, st));
     HIPOK(hipStreamSynchronize(st));
-    if (total == 0) continue;
-    HIPOK(m->sam_text.ensure((size_t)total));
-    HIPOK(m->sam_host.ensure((size_t)total));
-    a.offs = m->sam_offs.as<int64_t>();
-    a.text = m->sam_text.as<char>();
-    HIPOK(launch_sam(a, true, st));
-    HIPOK(hipMemcpyAsync(m->sam_host.p, m->sam_text.p, (size_t)total, hipMemcpyDeviceToHost, st));
-    HIPOK(hipStreamSynchronize(st));
-    if (fwrite(m->sam_host.p, 1, (size_t)total, out) != (size_t)total) return fail(STORM_ERR_IO, "SAM write failed");
+    if (total > 0) {
+      HIPOK(m->sam_text.ensure((size_t)total));
+      HIPOK(m->sam_host.ensure((size_t)total));
+      a.offs = m->sam_offs.as<int64_t>();
+      a.text = m->sam_text.as<char>();
+      HIPOK(launch_sam(a, true, st));
+      HIPOK(hipMemcpyAsync(m->sam_host.p, m->sam_text.p, (size_t)total, hipMemcpyDeviceToHost, st));
+      HIPOK(hipStreamSynchronize(st));
+    }
+    rc = sink((const char*)m->sam_host.p, total, r0, c, with_offsets ? offs.data() : nullptr);
+    if (rc) return rc;
   }
   HIPOK(hipMemcpyAsync(&mapped, m->sam_mapped.p, 8, hipMemcpyDeviceToHost, st));
   HIPOK(hipStreamSynchronize(st));

@@ -22,6 +22,7 @@
 // staged kernels (one launch per stage and read length), as storm_mapper_map_hits does.
 #include <algorithm>
 #include <condition_variable>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
@@ -100,6 +101,7 @@ class Arena {
 // parameters that change the kernels' work; requests batch together only within one class
 struct ClassKey {
   int color, I, z, t, match, mismatch, gap_open, gap_extend, sense_mask;
+  int max_hits, uniqueness, print_all, qual_base;   // SAM output options (one formatting pass per batch)
   bool operator==(const ClassKey& o) const { return memcmp(this, &o, sizeof o) == 0; }
 };
 
@@ -108,6 +110,7 @@ static ClassKey class_of(const storm_params& p) {
   memset(&k, 0, sizeof k);
   k.color = p.color; k.I = p.allowed_indels; k.z = p.z; k.t = p.t; k.match = p.match; k.mismatch = p.mismatch;
   k.gap_open = p.gap_open; k.gap_extend = p.gap_extend; k.sense_mask = p.sense_mask;
+  k.max_hits = p.max_hits; k.uniqueness = p.uniqueness; k.print_all = p.print_all; k.qual_base = p.qual_base;
   return k;
 }
 
@@ -132,6 +135,24 @@ struct Request {
   int status = STORM_OK;
   std::string err;
   std::vector<int32_t> hits;              // n * 4 * {score, ref_id, ref_start, sense}
+  // SAM records (sam_records): the reads' names packed (empty: "r<row>") and no-quality flags;
+  // each row's text as batches deliver it, then the request's text in row order
+  std::string names;
+  std::vector<int64_t> name_off;
+  std::vector<uint8_t> no_qual;
+  std::mutex sam_mu;
+  std::string sam_parts;
+  std::vector<int64_t> part_off;
+  std::vector<int32_t> part_len;
+  std::string sam;
+  const char* name_of(int64_t row, int& len, char* buf) const {
+    if (!name_off.empty()) {
+      len = (int)(name_off[(size_t)row + 1] - name_off[(size_t)row]);
+      return names.data() + name_off[(size_t)row];
+    }
+    len = snprintf(buf, 24, "r%lld", (long long)row);
+    return buf;
+  }
 };
 
 // a range of one request's reads inside a batch
@@ -285,6 +306,47 @@ static int run_batch(storm_server* s, storm_server::Lane& ln, const std::vector<
   rc = fused_core(m, n, maxL, fb, ps, totals, n_fallback);
   if (rc) return rc;
   HIPOK(launch_serve_scatter(m->fs.slots.as<Slot>(), ln.rows.as<ServeRow>(), n, ln.regions.as<ServeRegion>(), m->st));
+  if (s->cfg.sam_records) {
+    // the batch's SAM records (rows in batch order), each row's text handed to its request
+    std::string names;
+    std::vector<int64_t> name_off((size_t)n + 1);
+    std::vector<uint8_t> no_qual((size_t)n);
+    char buf[24];
+    for (int64_t k = 0; k < n; ++k) {
+      const ServeRow rw = ln.h_rows[(size_t)k];
+      const Request* r = regs[(size_t)rw.region];
+      int len = 0;
+      const char* nm = r->name_of(rw.row, len, buf);
+      name_off[(size_t)k] = (int64_t)names.size();
+      names.append(nm, (size_t)len);
+      no_qual[(size_t)k] = !r->has_qual || (!r->no_qual.empty() && r->no_qual[(size_t)rw.row]);
+    }
+    name_off[(size_t)n] = (int64_t)names.size();
+    SamRows rows{};
+    rows.n = n;
+    rows.names = names.data();
+    rows.name_off = name_off.data();
+    rows.no_qual = no_qual.data();
+    rows.has_qual = rows.has_first_qual = true;   // the gather writes levels (3 without quality) for every row
+    rows.code_stride = rows.qual_stride = SERVE_CS;
+    std::vector<int64_t> ident((size_t)n);
+    for (int64_t k = 0; k < n; ++k) ident[(size_t)k] = k;
+    const FinishCtx fc{m->fs.slots.as<Slot>(), m->fs.cells.as<uint8_t>(), &m->fs.reads, &ident, n};
+    SamSink to_requests = [&](const char* text, int64_t, int64_t row0, int64_t c, const int64_t* off) {
+      for (int64_t k = 0; k < c; ++k) {
+        const ServeRow rw = ln.h_rows[(size_t)(row0 + k)];
+        Request* r = regs[(size_t)rw.region];
+        std::lock_guard<std::mutex> lk(r->sam_mu);
+        r->part_off[(size_t)rw.row] = (int64_t)r->sam_parts.size();
+        r->part_len[(size_t)rw.row] = (int32_t)(off[k + 1] - off[k]);
+        r->sam_parts.append(text + (off[k] - off[0]), (size_t)(off[k + 1] - off[k]));
+      }
+      return STORM_OK;
+    };
+    int64_t mapped = 0;
+    rc = sam_device(m, rows, fc, to_requests, true, mapped);
+    if (rc) return rc;
+  }
   HIPOK(hipStreamSynchronize(m->st));
   std::lock_guard<std::mutex> lk(s->mu);
   s->st.n_fallback += n_fallback;
@@ -309,7 +371,46 @@ static int run_plain(storm_server* s, storm_server::Lane& ln, Request& r) {
   const size_t ns = (size_t)r.n * 4;
   std::vector<int32_t> f((size_t)4 * ns);
   HitsOut ho{f.data(), f.data() + ns, f.data() + 2 * ns, f.data() + 3 * ns};
-  int rc = map_view(m, &v, STORM_MODE_PLAIN, slots, &ms, &ho);
+  int rc = STORM_OK;
+  if (s->cfg.sam_records) {   // the request's SAM records in row order, then its hits
+    std::vector<int64_t> name_off;
+    std::string names;
+    if (r.name_off.empty()) {
+      name_off.resize((size_t)r.n + 1);
+      char buf[24];
+      for (int64_t k = 0; k < r.n; ++k) {
+        int len = 0;
+        const char* nm = r.name_of(k, len, buf);
+        name_off[(size_t)k] = (int64_t)names.size();
+        names.append(nm, (size_t)len);
+      }
+      name_off[(size_t)r.n] = (int64_t)names.size();
+    }
+    SamRows rows{};
+    rows.n = r.n;
+    rows.names = r.name_off.empty() ? names.data() : r.names.data();
+    rows.name_off = r.name_off.empty() ? name_off.data() : r.name_off.data();
+    rows.no_qual = r.no_qual.empty() ? nullptr : r.no_qual.data();
+    rows.has_qual = r.has_qual;
+    rows.has_first_qual = r.has_fq;
+    rows.code_stride = r.cs;
+    rows.qual_stride = r.qs;
+    SamSink to_request = [&](const char* text, int64_t bytes, int64_t, int64_t, const int64_t*) {
+      r.sam.append(text, (size_t)bytes);
+      return STORM_OK;
+    };
+    FinishFn fin = [&](const FinishCtx& fc, int64_t& mapped) {
+      int e = sam_device(m, rows, fc, to_request, false, mapped);
+      if (e) return e;
+      int64_t mp = 0;
+      return finish_view(m, fc.slots, fc.cells, *fc.order, fc.n, &ho, nullptr, slots, mp);
+    };
+    m->want_tb = true;
+    rc = map_view(m, &v, STORM_MODE_PLAIN, slots, &ms, nullptr, nullptr, &fin);
+  } else {
+    m->want_tb = false;
+    rc = map_view(m, &v, STORM_MODE_PLAIN, slots, &ms, &ho);
+  }
   if (rc) return rc;
   r.hits.resize(4 * ns);
   for (size_t k = 0; k < ns; ++k)
@@ -394,6 +495,20 @@ static bool has_work(storm_server* s) {
 // retire its region, wake its waiter
 static void complete(storm_server* s, storm_server::Lane& ln, Request* r) {
   int rc = STORM_OK;
+  if (s->cfg.sam_records && s->cfg.mode != STORM_MODE_PLAIN && r->status == STORM_OK) {
+    // the rows' records in row order (batches delivered them in their own order)
+    std::lock_guard<std::mutex> lk(r->sam_mu);
+    size_t total = 0;
+    for (int64_t k = 0; k < r->n; ++k) total += (size_t)r->part_len[(size_t)k];
+    r->sam.resize(total);
+    size_t o = 0;
+    for (int64_t k = 0; k < r->n; ++k) {
+      const size_t l = (size_t)r->part_len[(size_t)k];
+      memcpy(&r->sam[o], r->sam_parts.data() + r->part_off[(size_t)k], l);
+      o += l;
+    }
+    std::string().swap(r->sam_parts);
+  }
   if (s->cfg.mode != STORM_MODE_PLAIN && r->status == STORM_OK) {
     r->hits.resize((size_t)r->n * 16);
     const ServeRegion g = region_view(s, *r);
@@ -482,6 +597,9 @@ static void lane_loop(storm_server* s, storm_server::Lane* ln) {
 
 }  // namespace storm
 
+static int wait_request(storm_server* s, int64_t id, int32_t* score, int32_t* ref_id, int32_t* ref_start, int32_t* sense,
+                        storm_request_result* out, std::string* sam);
+
 extern "C" {
 
 void storm_server_config_default(storm_server_config* c) {
@@ -494,6 +612,7 @@ void storm_server_config_default(storm_server_config* c) {
   c->hbm_reserve_bytes = 8LL << 30;
   c->encodings = 3;
   c->stdev_distance = 100;
+  c->sam_records = 1;
 }
 
 int storm_server_create(int device, const char* genome_path, const storm_server_config* cfg, storm_server** out) {
@@ -525,7 +644,7 @@ int storm_server_create(int device, const char* genome_path, const storm_server_
     std::unique_ptr<storm_server::Lane> ln(new storm_server::Lane());
     ln->m.reset(new storm_mapper());
     ln->m->device = device;
-    ln->m->want_tb = false;
+    ln->m->want_tb = s->cfg.sam_records != 0;   // tracebacks only for SAM records
     ln->m->shared_tune = &s->tune;
     HIPOK(hipStreamCreateWithFlags(&ln->m->st, hipStreamNonBlocking));
     s->lanes.push_back(std::move(ln));
@@ -595,6 +714,21 @@ int storm_server_submit(storm_server* s, const storm_reads_view* v, const storm_
     if (L < wl) return fail(STORM_ERR_INVALID, "read %lld is shorter (%d) than the filter band lanes (%d)", (long long)i, L, wl);
     if (r->cs * 4 < L) return fail(STORM_ERR_INVALID, "code_stride %lld too small for length %d", (long long)r->cs, L);
   }
+  if (s->cfg.sam_records) {
+    if (v->names) {
+      r->name_off.resize((size_t)r->n + 1);
+      for (int64_t i = 0; i < r->n; ++i) {
+        r->name_off[(size_t)i] = (int64_t)r->names.size();
+        r->names += v->names[i];
+      }
+      r->name_off[(size_t)r->n] = (int64_t)r->names.size();
+    }
+    if (v->qual && v->no_qual) r->no_qual.assign(v->no_qual, v->no_qual + r->n);
+    if (s->cfg.mode != STORM_MODE_PLAIN) {
+      r->part_off.assign((size_t)r->n, 0);
+      r->part_len.assign((size_t)r->n, 0);
+    }
+  }
   r->codes.assign(v->codes, v->codes + r->n * r->cs);
   if (r->has_qual) r->qual.assign(v->qual, v->qual + r->n * r->qs);
   r->len.assign(v->len, v->len + r->n);
@@ -621,6 +755,13 @@ int storm_server_submit(storm_server* s, const storm_reads_view* v, const storm_
 
 int storm_server_wait(storm_server* s, int64_t id, int32_t* score, int32_t* ref_id, int32_t* ref_start, int32_t* sense,
                       storm_request_result* out) {
+  return wait_request(s, id, score, ref_id, ref_start, sense, out, nullptr);
+}
+
+}  // extern "C"
+
+static int wait_request(storm_server* s, int64_t id, int32_t* score, int32_t* ref_id, int32_t* ref_start, int32_t* sense,
+                        storm_request_result* out, std::string* sam) {
   if (!s) return fail(STORM_ERR_INVALID, "null server");
   std::unique_ptr<Request> r;
   {
@@ -656,8 +797,31 @@ int storm_server_wait(storm_server* s, int64_t id, int32_t* score, int32_t* ref_
     if (r->hits[(size_t)(i * 16 + 1)] != -1) mapped++;
   }
   if (out) out->n_mapped = mapped;
+  if (sam) sam->swap(r->sam);
   return STORM_OK;
 }
+
+extern "C" {
+
+int storm_server_wait_sam(storm_server* s, int64_t id, char** sam, int64_t* sam_bytes, int32_t* score, int32_t* ref_id,
+                          int32_t* ref_start, int32_t* sense, storm_request_result* out) {
+  if (!s || !sam || !sam_bytes) return fail(STORM_ERR_INVALID, "null argument");
+  *sam = nullptr;
+  *sam_bytes = 0;
+  if (!s->cfg.sam_records) return fail(STORM_ERR_INVALID, "the server was created without SAM records (sam_records = 0)");
+  std::string text;
+  int rc = wait_request(s, id, score, ref_id, ref_start, sense, out, &text);
+  if (rc) return rc;
+  char* buf = (char*)malloc(text.size() + 1);
+  if (!buf) return fail(STORM_ERR_MEMORY, "SAM text of request %lld", (long long)id);
+  memcpy(buf, text.data(), text.size());
+  buf[text.size()] = 0;
+  *sam = buf;
+  *sam_bytes = (int64_t)text.size();
+  return STORM_OK;
+}
+
+void storm_free(void* p) { free(p); }
 
 int storm_server_get_stats(storm_server* s, storm_server_stats* out) {
   if (!s || !out) return fail(STORM_ERR_INVALID, "null argument");

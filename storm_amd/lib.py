@@ -52,7 +52,7 @@ class ServerConfig(C.Structure):
     _fields_ = [("mode", C.c_int), ("lanes", C.c_int), ("cross_request", C.c_int), ("max_batch_reads", C.c_int64),
                 ("hbm_budget_bytes", C.c_int64), ("hbm_reserve_bytes", C.c_int64), ("encodings", C.c_int),
                 ("seeds_nucleotide", C.c_char_p), ("seeds_color", C.c_char_p), ("stdev_distance", C.c_int),
-                ("ignore_lowercase", C.c_int)]
+                ("ignore_lowercase", C.c_int), ("sam_records", C.c_int)]
 
 
 class ServerStats(C.Structure):
@@ -110,6 +110,9 @@ def load() -> C.CDLL:
     lib.storm_server_create.argtypes = [I, C.c_char_p, C.POINTER(ServerConfig), C.POINTER(P)]
     lib.storm_server_submit.argtypes = [P, C.POINTER(ReadsView), C.POINTER(Params), C.POINTER(L)]
     lib.storm_server_wait.argtypes = [P, L, P, P, P, P, C.POINTER(RequestResult)]
+    lib.storm_server_wait_sam.argtypes = [P, L, C.POINTER(C.c_void_p), C.POINTER(L), P, P, P, P,
+                                          C.POINTER(RequestResult)]
+    lib.storm_free.argtypes = [P]
     lib.storm_server_get_stats.argtypes = [P, C.POINTER(ServerStats)]
     lib.storm_server_destroy.argtypes = [P]
     lib.storm_index_build.argtypes = [I, P, L, P, P, I, C.c_char_p, I, C.POINTER(P)]
@@ -483,6 +486,21 @@ class Server:
                                       _ptr(out["sense"]), C.byref(res))
         check(rc)
         return out, res.as_dict()
+
+    def wait_sam(self, rid: int):
+        """As wait(), plus the request's SAM records as bytes: (hits, result, sam)."""
+        n = self._keep.pop(rid)
+        out = {k: np.zeros((n, 4), np.int32) for k in ("score", "ref_id", "ref_start", "sense")}
+        res = RequestResult()
+        text = C.c_void_p()
+        nb = C.c_int64()
+        rc = load().storm_server_wait_sam(self.h, rid, C.byref(text), C.byref(nb), _ptr(out["score"]),
+                                          _ptr(out["ref_id"]), _ptr(out["ref_start"]), _ptr(out["sense"]),
+                                          C.byref(res))
+        check(rc)
+        sam = C.string_at(text, nb.value)
+        load().storm_free(text)
+        return out, res.as_dict(), sam
 
     def stats(self) -> dict:
         st = ServerStats()

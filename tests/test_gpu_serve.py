@@ -144,12 +144,13 @@ def test_plain_server_equals_batched(serve_case, alone):
 
 def test_served_request_matches_oracle_sam(serve_case, oracle_built):
     """One single-length nucleotide request, -i 3 -M 4: the served slots are the records of the
-    CPU restatement's SAM (rank order; POS, strand flag, AS:i)."""
+    CPU restatement's SAM (rank order; POS, strand flag, AS:i), and the served SAM records are
+    its SAM lines."""
     from storm_amd import lib, synth
     d, gpath, g, _ = serve_case
     pr = synth.fast_nucleotide_reads(g, 400, 100, 77, sub=0.015, indel=0.002, qlo=15, qhi=40)
     s = lib.Server(gpath, lanes=2, max_batch_reads=256)
-    h, _ = s.wait(s.submit(pr, **class_params(0, 3)))
+    h, _, sam = s.wait_sam(s.submit(pr, **class_params(0, 3)))
     s.close()
     rpath = str(d / "oracle_reads.fq")
     synth.write_fastq(rpath, synth.packed_to_simreads(pr, np.arange(len(pr.len))))
@@ -157,9 +158,11 @@ def test_served_request_matches_oracle_sam(serve_case, oracle_built):
     subprocess.run([os.path.join(oracle_built, "oracle-nucleotide"), "-g", gpath, "-r", rpath, "-s", SEEDS_N,
                     "-i", "3", "-M", "4", "-z", "100", "-t", "100", "-o", out], check=True, stderr=subprocess.DEVNULL)
     recs = {}
+    body = []
     for line in open(out):
         if line.startswith("@"):
             continue
+        body.append(line)
         f = line.rstrip("\n").split("\t")
         score = int(next(x for x in f[11:] if x.startswith("AS:i:"))[5:])
         recs.setdefault(int(f[0][1:]), []).append((score, int(f[3]) - 1, -1 if int(f[1]) & 16 else 1))
@@ -169,6 +172,57 @@ def test_served_request_matches_oracle_sam(serve_case, oracle_built):
         got = [(int(h["score"][r, k]), int(h["ref_start"][r, k]), int(h["sense"][r, k]))
                for k in range(4) if h["ref_id"][r, k] >= 0]
         assert got == want, f"read {r}"
+    # the reference writes reads in its tag order (reads_data.c:59-67), a request in the order given:
+    # the same lines read by read
+    def by_read(lines):
+        out = {}
+        for line in lines:
+            out.setdefault(line.split("\t", 1)[0], []).append(line)
+        return out
+    assert by_read(sam.decode().splitlines(keepends=True)) == by_read(body)
+
+
+def _sam_body(path):
+    return [l for l in open(path) if not l.startswith("@")]
+
+
+def test_served_sam_equals_mapper_sam(serve_case, tmp_path):
+    """Every request's SAM records, served in the mix (batches across requests, lengths and
+    encodings), are the lines storm_mapper_map writes for the same reads alone."""
+    from storm_amd import lib
+    _, gpath, _, reqs = serve_case
+    s = lib.Server(gpath, lanes=3, max_batch_reads=4096)
+    print("server: SAM records of every request in the mix", flush=True)
+    ids = [s.submit(pr, **kw) for pr, kw in reqs]
+    served = [s.wait_sam(rid) for rid in ids]
+    st = s.stats()
+    s.close()
+    assert st["max_requests_per_batch"] >= 2
+    mappers = {}
+    for k, (pr, kw) in enumerate(reqs):
+        key = (kw["color"], kw["allowed_indels"])
+        if key not in mappers:
+            mappers[key] = lib.Mapper(gpath, seeds=SEEDS_C if kw["color"] else SEEDS_N, **kw)
+        out = str(tmp_path / f"req{k}.sam")
+        mappers[key].map_sam(pr, out, mode=lib.STORM_MODE_PLAIN)
+        want = _sam_body(out)
+        got = served[k][2].decode().splitlines(keepends=True)
+        assert len(got) == len(want) and got == want, f"request {k}"
+        assert served[k][1]["n_mapped"] > 0
+    for m in mappers.values():
+        m.close()
+
+
+def test_plain_server_sam_equals_batched(serve_case):
+    from storm_amd import lib
+    _, gpath, _, reqs = serve_case
+    out = {}
+    for mode in (lib.STORM_MODE_PLAIN, lib.STORM_MODE_BATCHED):
+        s = lib.Server(gpath, mode=mode)
+        ids = [s.submit(pr, **kw) for pr, kw in reqs[:4]]
+        out[mode] = [s.wait_sam(rid)[2] for rid in ids]
+        s.close()
+    assert out[lib.STORM_MODE_PLAIN] == out[lib.STORM_MODE_BATCHED]
 
 
 def test_failed_upload_fails_only_its_request(serve_case, alone, monkeypatch):
