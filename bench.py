@@ -1,36 +1,50 @@
-"""Benchmark of the SToRM mapping hot path on one MI355X.
+"""Benchmark of the SToRM mapping hot path on one MI355X: every workload of the metric in one run.
 
-Default workload (one "step" = one launch of the banded filter over the whole batch):
-  10M candidate pairs = 5M reads of 75 nt x 2 windows (true origin - 7, uniform random decoy) on a
-  1 Mbp i.i.d. genome, -i 7 (W = 8 lanes, 16 diagonals per the reference's geometry), saturating
-  u8 scores, z = 100.  Inputs are resident in HBM before the timed region.
-Reported: filter GCUPS (cells = candidates x L x 2W), the filter through the per-call drop-in
-entry point (one launch per read, as run.c calls the SIMD table) on a sample, whole -M mapping
-runs in reads/s (plain per-batch and batched), and the reference's OpenMP SIMD filter timed on
-this host's cores as the CPU baseline.
+Headline (`value`, one "step" = one launch of the banded filter over the whole batch): the filter
+alone (10M candidate pairs = 5M reads of 75 nt x 2 windows, true origin - 7 and a uniform decoy,
+on a 1 Mbp i.i.d. genome, -i 7: W = 8 lanes, 16 diagonals, u8 scores, z = 100), inputs resident in
+HBM, filter GCUPS against the VALU issue roofline.  The same JSON line carries:
+  * the DP stage alone (DP + traceback of 2M filter passes) against the reference's
+    alignment__align / alignment__traceback on the host cores;
+  * the mapping workloads 1 (100k x 50 nt, 1 Mbp), 3 (5M color reads of 35-75 bp, 50 Mbp) and 4
+    (20M x 100 nt, 100 Mbp with repeats, -M 4): reads/s from packed reads in host memory to the
+    hit map and tracebacks in host memory (storm_mapper_map_records, the records the SAM text is
+    made of), batched (persistent fused kernel) and plain (staged kernels, one batch per read
+    length), first call and warm, per-stage times of the timed calls, identical records on both
+    paths; the SAM text (formatted on the device) timed separately; and the reference program's
+    own mapping phase on the host cores over a sample (per read length for workload 3), with the
+    GPU SAM compared to the reference's;
+  * workload 5: Poisson arrivals of requests on the serving loop (resident indexes, admission by
+    HBM, continuous batching, SAM records per request) and on the plain per-request server; p50 /
+    p99 latency to the last SAM record, reads/s, identical results on both.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-map] [--no-cpu]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--config all|2|1|3|4|5] [--no-cpu]
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
+import hashlib
 import json
 import os
 import subprocess
 import sys
 import tempfile
+import threading
 import time
+from types import SimpleNamespace
 
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+REF = os.path.join(ROOT, "oracle", "_ref")
 
 METRIC = "mapped reads/s + filter GCUPS, batched vs plain GPU; p50/p99 request latency"
 OPS_PER_CELL = 13            # compute chain of the reference's SIMD cell (alignment_simd.c:3810-3880)
 VALU_PEAK_INT32 = 39.3       # T VALU lane-instructions/s: 256 CU x 64 lanes x 2.4 GHz
 VALU_PEAK_PK16 = 78.6        # T 16-bit ops/s with packed VALU instructions (2 per lane-instruction)
+HBM_PEAK_GBPS = 8000.0
 # VALU lane-instructions k_filter<8> issues per cell at L=75 (SQ_INSTS_VALU x 64 / cells, rocprofv3 --pmc,
 # profiles/k_filter_W8_L75_10Mcand.md); the kernel's own work measure for the VALU-issue roofline
 VALU_INSTR_PER_CELL_W8_L75 = 5.438
@@ -46,11 +60,53 @@ HBM_BYTES_PER_CAND_W8_L75 = 45.6
 # codes, 23 B) + the u8 score and the pass byte
 ALG_BYTES_PER_CAND_W8_L75 = 44.0
 
+SEEDS_N = "####--#-#--##--####;#####--###-#-###;###-##-#######"   # seed.h:29 (nucleotide defaults)
+SEEDS_C = "####----##--##-####;####-###--#----####;####-####-####"   # seed.h:31 (color defaults)
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# ------------------------------------------------------------------------------------------ host
+def host_info() -> dict:
+    """The host cores: nproc, lscpu's sockets / cores / threads per core, and the threads the CPU
+    baselines use (the process's CPU set, capped by OMP_NUM_THREADS when the environment sets it:
+    on a shared GPU box that is the CPU share of one GPU)."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    info = {"nproc": avail, "os_cpu_count": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, check=True).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k] = v
+        if "Socket(s)" in info and "Core(s) per socket" in info:
+            info["physical_cores"] = int(info["Socket(s)"]) * int(info["Core(s) per socket"])
+    except (OSError, subprocess.CalledProcessError, ValueError):
+        pass
+    omp = os.environ.get("OMP_NUM_THREADS")
+    used = avail
+    if omp and omp.isdigit() and int(omp) > 0:
+        used = min(avail, int(omp))
+    info["threads_used"] = used
+    info["threads_note"] = ("the process's CPU set" if used == avail else
+                            f"OMP_NUM_THREADS={omp} (the CPU share of this process) of {avail} available")
+    return info
+
+
+HOST = None
+
+
+def cores() -> int:
+    return HOST["threads_used"]
+
+
+# ------------------------------------------------------------------------------------------ filter
 def make_filter_workload(n_reads: int, L: int, W: int, d: int, seed: int = 2, genome_len: int = 1_000_000):
     """Packed reads (n_reads x stride bytes) and 2 window starts per read, built in chunks."""
     from storm_amd import synth
@@ -87,10 +143,10 @@ def gpu_filter(lib, g, reads, pos, stride, L, W, steps, warmup):
     cr = np.repeat(np.arange(n_reads, dtype=np.int32), 2)
     cp = np.ascontiguousarray(pos.reshape(-1))
     h = P()
-    lib.check(lib.load().storm_filter_plan_create(0, P(gp.ctypes.data), len(g), P(reads.ctypes.data), stride,
-                                                  P(rl.ctypes.data), n_reads, P(cr.ctypes.data), P(cp.ctypes.data),
-                                                  len(cr), W, 5, -4, -9, -4, 100, C.byref(h)))
     so = lib.load()
+    lib.check(so.storm_filter_plan_create(0, P(gp.ctypes.data), len(g), P(reads.ctypes.data), stride, P(rl.ctypes.data),
+                                          n_reads, P(cr.ctypes.data), P(cp.ctypes.data), len(cr), W, 5, -4, -9, -4, 100,
+                                          C.byref(h)))
     ms = C.c_double()
     lib.check(so.storm_filter_run(h, warmup, C.byref(ms)))
     torch.cuda.synchronize()
@@ -105,7 +161,7 @@ def gpu_filter(lib, g, reads, pos, stride, L, W, steps, warmup):
     return ms.value, wall, scores, passf, cr, cp
 
 
-def percall_filter(lib, g, reads, pos, stride, L, W, n_calls):
+def percall_filter(lib, g, reads, pos, L, n_calls):
     """The drop-in row: alignment_gfx950__init_pair + __align_pair, one call per read (run.c:660)."""
     from storm_amd import synth
     P = C.c_void_p
@@ -115,28 +171,20 @@ def percall_filter(lib, g, reads, pos, stride, L, W, n_calls):
     gp[pad:pad + (len(g) + 3) // 4] = synth.pack_codes(g)
     lib.check(so.alignment_gfx950__set_genome(P(gp.ctypes.data), gp.nbytes, pad))
     so.alignment_gfx950__init_pair(5, 4, 9, 4, 100, L)
-    bits = np.zeros(n_calls, np.uint32)
     t0 = time.perf_counter()
     for k in range(n_calls):
         win = np.ascontiguousarray(pos[k])
-        bits[k] = so.alignment_gfx950__align_pair(P(gp.ctypes.data + pad), P(win.ctypes.data), P(reads[k].ctypes.data))
+        so.alignment_gfx950__align_pair(P(gp.ctypes.data + pad), P(win.ctypes.data), P(reads[k].ctypes.data))
     dt = time.perf_counter() - t0
     so.alignment_gfx950__clean()
-    return dt, bits
+    return dt
 
 
-def cpu_baseline(g, reads, pos, stride, L, d, n_sample, target_s=10.0):
-    """The reference's SSE2 filter (its default Makefile build) with OpenMP on this host."""
+def cpu_filter_baseline(g, reads, pos, L, d, n_sample, target_s=8.0):
+    """The reference's SIMD filter (alignment_<isa>__align_pair of alignment_simd.c) built with the
+    default Makefile flags (SSE2, Makefile:7) and with -mavx2, OpenMP over the reads on the host
+    cores, on the first n_sample reads of the workload (2 windows each), repeated to ~target_s."""
     from storm_amd import synth
-    path = os.path.join(ROOT, "oracle", "_ref", "libref_filter.so")
-    if not os.path.exists(path):
-        return None
-    lib = C.CDLL(path)
-    P = C.c_void_p
-    lib.ref_filter_time.restype = C.c_double
-    lib.ref_filter_time.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, P, C.c_long, P, P, P, C.c_long, P, C.c_int,
-                                    C.c_int]
-    cores = min(16, os.cpu_count() or 1)
     pad = 1024
     gp = np.zeros(pad + (len(g) + 3) // 4 + pad, np.uint8)
     gp[pad:pad + (len(g) + 3) // 4] = synth.pack_codes(g)
@@ -145,21 +193,47 @@ def cpu_baseline(g, reads, pos, stride, L, d, n_sample, target_s=10.0):
     seg = np.arange(0, 2 * n + 1, 2, dtype=np.int64)
     ps = np.ascontiguousarray(pos[:n].reshape(-1))
     bits = np.zeros(2 * n, np.uint8)
+    P = C.c_void_p
     out = {}
-    for isa, name in ((0, "sse2"), (1, "avx2")):
+    for isa, name, so_name in ((0, "sse2", "libref_filter_sse2.so"), (1, "avx2", "libref_filter_avx2.so")):
+        path = os.path.join(REF, so_name)
+        if not os.path.exists(path):
+            continue
+        lib = C.CDLL(path)
+        lib.ref_filter_time.restype = C.c_double
+        lib.ref_filter_time.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, P, C.c_long, P, P, P, C.c_long, P, C.c_int,
+                                        C.c_int]
         t = lib.ref_filter_time(isa, d, L, 100, P(gp.ctypes.data + pad), n, P(seg.ctypes.data), P(ps.ctypes.data),
-                                P(rd.ctypes.data), stride, P(bits.ctypes.data), cores, 1)
+                                P(rd.ctypes.data), rd.shape[1], P(bits.ctypes.data), cores(), 1)
         if t <= 0:
             continue
         reps = max(1, int(target_s / max(t, 1e-3)))
-        if name == "avx2":
-            reps = max(1, reps // 2)
         t = lib.ref_filter_time(isa, d, L, 100, P(gp.ctypes.data + pad), n, P(seg.ctypes.data), P(ps.ctypes.data),
-                                P(rd.ctypes.data), stride, P(bits.ctypes.data), cores, reps)
-        out[name] = (2 * n * reps, t, cores)
+                                P(rd.ctypes.data), rd.shape[1], P(bits.ctypes.data), cores(), reps)
+        out[name] = (2 * n * reps, t)
     return out
 
 
+def oracle_spot_check(g, reads, scores, cr, cp, L, W, n_check=2000):
+    """The CPU restatement's u8 score of a sample of the filter workload's candidates."""
+    from storm_amd import synth
+    orc = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+    if not os.path.exists(orc):
+        return None
+    ol = C.CDLL(orc)
+    ol.orc_filter_score.restype = C.c_int
+    ol.orc_filter_score.argtypes = [C.c_void_p, C.c_long, C.c_void_p, C.c_int, C.c_int] + [C.c_int] * 4
+    pad = 64
+    gp = np.zeros(pad + (len(g) + 3) // 4 + pad, np.uint8)
+    gp[pad:pad + (len(g) + 3) // 4] = synth.pack_codes(g)
+    sel = np.linspace(0, len(cr) - 1, n_check).astype(np.int64)
+    want = np.array([min(255, ol.orc_filter_score(C.c_void_p(gp.ctypes.data + pad), int(cp[c]),
+                                                  C.c_void_p(reads[int(cr[c])].ctypes.data), L, W, 5, -4, -9, -4))
+                     for c in sel])
+    return bool(np.array_equal(scores[sel].astype(np.int64), want))
+
+
+# ------------------------------------------------------------------------------------------ DP stage
 def gpu_align_stage(lib, g, reads, pos, stride, L, I, d, n_rec, reps):
     """Stage (d) alone through storm_align_plan_*: the DP + traceback of the filter's true-origin
     passes (read strand 0 at diagonal origin), qualities Q20-40 (all level 3, so no quality
@@ -184,96 +258,66 @@ def gpu_align_stage(lib, g, reads, pos, stride, L, I, d, n_rec, reps):
     tb = np.zeros((n, lib.TB_BYTES), np.uint8)
     lib.check(so.storm_align_fetch(h, P(res.ctypes.data), P(tb.ctypes.data)))
     so.storm_align_plan_destroy(h)
-    return ms.value, res, tb, rs, diag
+    return ms.value, res, tb, diag
 
 
-def cpu_align_baseline(g, reads, L, I, rs, diag, res, tb, n_sample=20000):
-    """The oracle's restatement of alignment__align + alignment__traceback (orc_align, one
-    thread, kind "port") on the first n_sample records of the DP workload, timed on this host;
-    the GPU records of the sample are compared with it."""
-    path = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+def cpu_align_baseline(g, reads, L, I, t, diag, res, tb, n_sample=200_000, target_s=4.0):
+    """The reference's alignment__align + alignment__traceback (alignment.c:488-734, its SSE2 build)
+    as process_read calls them for a filter pass, one AlignmentType per OpenMP thread on the host
+    cores, over the first n_sample records of the DP workload; every record compared with the GPU's
+    (score; for accepted records the last-row column, the traceback start and its codes)."""
+    from storm_amd import synth
+    path = os.path.join(REF, "libref_align.so")
     if not os.path.exists(path):
         return None
-
-    class OrcParams(C.Structure):
-        _fields_ = [(f, C.c_int) for f in (
-            "color", "match", "mismatch", "gap_open", "gap_extend", "allowed_indels", "simd_diags", "t", "z",
-            "qual_base", "filter_repeats", "stdev_distance", "n_bytes", "sense_mask", "ignore_lowercase",
-            "unordered_M", "uniqueness", "uniqueness_threshold", "print_all", "greedy")]
-
-    class OrcAlignOut(C.Structure):
-        _fields_ = [("score", C.c_int), ("best_i", C.c_int), ("best_j", C.c_int), ("best_i0", C.c_int),
-                    ("best_j0", C.c_int), ("tb", C.c_uint8 * 1024), ("tb_len", C.c_int)]
-
-    o = C.CDLL(path)
+    lib = C.CDLL(path)
     P = C.c_void_p
-    o.orc_params_default.argtypes = [C.POINTER(OrcParams), C.c_int]
-    o.orc_align.argtypes = [P, P, C.c_int, P, P, C.c_int, C.POINTER(OrcParams), C.c_int, C.POINTER(OrcAlignOut)]
-    o.orc_align.restype = C.c_int
-    op = OrcParams()
-    o.orc_params_default(C.byref(op), 0)
-    op.allowed_indels, op.simd_diags = I, min(I, 15)
-    n = min(n_sample, len(rs))
-    pad = 64
-    gpad = np.zeros(pad + len(g) + pad, np.uint8)
-    gpad[pad:pad + len(g)] = g
-    mpad = np.ones_like(gpad)
-    mpad[pad:pad + len(g)] = 0
-    qlev = np.full(L, 3, np.uint8)
-    codes = np.stack([(reads[:n, :(L + 3) // 4] >> s) & 3 for s in (0, 2, 4, 6)], axis=2).reshape(n, -1)[:, :L]
-    codes = np.ascontiguousarray(codes, dtype=np.uint8)
-    d = min(I, 15)
-    wins = [int(h - I) if h - I > -d else -d for h in diag[:n]]
-    wref = [np.ascontiguousarray(gpad[pad + w:pad + w + L + 2 * I + 1]) for w in wins]
-    wmask = [np.ascontiguousarray(mpad[pad + w:pad + w + L + 2 * I + 1]) for w in wins]
-    outs = [OrcAlignOut() for _ in range(n)]
-    t0 = time.perf_counter()
-    for k in range(n):
-        o.orc_align(P(codes[k].ctypes.data), P(qlev.ctypes.data), L, P(wref[k].ctypes.data), P(wmask[k].ctypes.data),
-                    I, C.byref(op), op.t, C.byref(outs[k]))
-    dt = time.perf_counter() - t0
-    same = True
-    for k in range(n):
-        a = outs[k]
-        if (int(res["score"][k]), int(res["best_j"][k]), int(res["tb_len"][k])) != (a.score, a.best_j, a.tb_len if a.score > op.t else 0):
-            same = False
-            break
-        if a.score > op.t:
-            got = bytes(((tb[k][i >> 1] >> (0 if i & 1 else 4)) & 0xF) for i in range(a.tb_len))
-            if got != bytes(a.tb[:a.tb_len]):
+    lib.ref_align_time.restype = C.c_double
+    lib.ref_align_time.argtypes = [P, P, P, C.c_long, C.c_int, P, C.c_long] + [C.c_int] * 8 + [P] * 6 + [C.c_int]
+    n = min(n_sample, len(diag))
+    pad = 1024
+    gbytes = (len(g) + 3) // 4
+    gp = np.zeros(pad + gbytes + pad, np.uint8)
+    gp[pad:pad + gbytes] = synth.pack_codes(g)
+    mp = np.full(pad + gbytes + pad, 0x55, np.uint8)   # the reference's masked buffer: 0x55 padding
+    mp[pad:pad + gbytes] = 0
+    rd = np.ascontiguousarray(reads[:n])
+    dg = np.ascontiguousarray(diag[:n], np.int32)
+    sc, bj, bi0, bj0, tl = (np.zeros(n, np.int32) for _ in range(5))
+    tbr = np.zeros((n, 144), np.uint8)
+    args = lambda reps: (P(gp.ctypes.data + pad), P(mp.ctypes.data + pad), P(rd.ctypes.data), rd.shape[1], L,
+                         P(dg.ctypes.data), n, I, t, 5, -4, -9, -4, cores(), reps, P(sc.ctypes.data), P(bj.ctypes.data),
+                         P(bi0.ctypes.data), P(bj0.ctypes.data), P(tl.ctypes.data), P(tbr.ctypes.data), 144)
+    t1 = lib.ref_align_time(*args(1))
+    reps = max(1, int(target_s / max(t1, 1e-3)))
+    dt = lib.ref_align_time(*args(reps))
+    r = res[:n]
+    acc = sc > t
+    same = bool(np.array_equal(r["score"], sc))
+    same = same and bool(np.array_equal(r["best_j"][acc], bj[acc]) and np.array_equal(r["best_i0"][acc], bi0[acc]) and
+                         np.array_equal(r["best_j0"][acc], bj0[acc]) and np.array_equal(r["tb_len"][acc], tl[acc]))
+    if same:
+        for k in np.nonzero(acc)[0]:
+            nb = (int(tl[k]) + 1) // 2
+            if not np.array_equal(tb[k, :nb], tbr[k, :nb]):
                 same = False
                 break
-    return {"value": round(n * L * (2 * I + 1) / dt / 1e9, 4), "unit": "GCUPS", "cores": 1, "kind": "port",
-            "sample": f"orc_align (the oracle's restatement of alignment__align + alignment__traceback) on the first "
-                      f"{n} records, one thread, {dt:.1f}s", "records_per_s": round(n / dt, 1),
+    return {"value": round(n * L * (2 * I + 1) / dt / 1e9, 3), "unit": "GCUPS", "cores": cores(), "kind": "reference",
+            "sample": f"the reference's alignment__align + alignment__traceback (SSE2 build, libref_align.so) on the "
+                      f"first {n} records, OpenMP, {dt:.2f}s per pass", "records_per_s": round(n / dt, 1),
             "gpu_records_identical": same}
 
 
-SEEDS_N = "####--#-#--##--####;#####--###-#-###;###-##-#######"   # seed.h:29 (nucleotide defaults)
-SEEDS_C = "####----##--##-####;####-###--#----####;####-####-####"   # seed.h:31 (color defaults)
-
-
-def host_cores() -> int:
-    try:
-        return max(1, min(16, len(os.sched_getaffinity(0))))
-    except AttributeError:
-        return max(1, min(16, os.cpu_count() or 1))
-
-
-def sam_body(path):
-    with open(path) as f:
-        return [l for l in f if not l.startswith("@PG")]
-
-
-def ref_binary(color: int):
-    p = os.path.join(ROOT, "oracle", "_ref", "storm-color" if color else "storm-nucleotide")
+# ------------------------------------------------------------------------------------------ mapping
+def ref_harness(color: int):
+    p = os.path.join(REF, "ref-map-harness-color" if color else "ref-map-harness-nucleotide")
     return p if os.path.exists(p) else None
 
 
 def cli_args(kw) -> list:
     out = ["-i", str(kw["allowed_indels"]), "-z", str(kw["z"]), "-t", str(kw["t"])]
-    if kw["max_hits"]:
-        out += ["-M", str(kw["max_hits"])]   # no -M: the ordered (genome map) output
+    if kw.get("max_hits"):
+        out += ["-M", str(kw["max_hits"])]
     return out
 
 
@@ -289,141 +333,153 @@ def write_read_files(d, pr, idx, color, tag):
     return ["-r", path], (path, None)
 
 
-def run_reference(color, gpath, rargs, kw, sam, cores):
-    exe = ref_binary(color)
-    cmd = [exe, "-g", gpath] + rargs + cli_args(kw) + ["-s", SEEDS_C if color else SEEDS_N, "-N", str(cores),
-                                                     "-o", sam, "-v", "0"]
+def sam_records_body(path):
+    with open(path) as f:
+        return [l for l in f if not l.startswith("@")]
+
+
+def cpu_mapping_baseline(lib, m, d, gpath, pr, kw, color, per_length: int, max_reads: int):
+    """The reference program (its own sources, SSE2 flags, OpenMP on the host cores) on a sample of
+    the workload through oracle/_ref/ref-map-harness-*: one run per read length (the reference
+    maps one length per run, reads_data.c:156-163), the sequences and seed indexes built once,
+    timed from the end of the index build to the start of the SAM output (its forward and reverse
+    process_read passes, run.c:865-994).  reads/s = sample reads / summed mapping time.  The GPU
+    maps each run's file (loaded and ordered by tag like the reference) and its SAM records are
+    compared with the reference's."""
+    exe = ref_harness(color)
+    if exe is None:
+        return None
+    lens = np.asarray(pr.len)
+    runs = []
+    for Lv in sorted(np.unique(lens), reverse=True):   # longest first: the first run pads the sequence
+        idx = np.nonzero(lens == Lv)[0][:per_length]
+        if len(idx):
+            runs.append((int(Lv), idx))
+    total = sum(len(i) for _, i in runs)
+    if total > max_reads:   # an even share of max_reads per length
+        share = max(1, max_reads // len(runs))
+        runs = [(Lv, idx[:share]) for Lv, idx in runs]
+        total = sum(len(i) for _, i in runs)
+    args = [exe, os.path.join(d, "ref_times.txt"), "-g", gpath] + cli_args(kw) + \
+        ["-s", SEEDS_C if color else SEEDS_N, "-N", str(cores()), "-v", "0"]
+    files = []
+    for k, (Lv, idx) in enumerate(runs):
+        rargs, f = write_read_files(d, pr, idx, color, f"ref_{k}")
+        files.append(f)
+        args += ["--"] + rargs + ["-o", os.path.join(d, f"ref_{k}.sam")]
     t0 = time.perf_counter()
     # the reference ORs color reads into an uninitialised buffer; a fixed perturbation byte makes it deterministic
-    subprocess.run(cmd, check=True, env=dict(os.environ, MALLOC_PERTURB_="255"), stderr=subprocess.DEVNULL,
-                   stdout=subprocess.DEVNULL)
-    return time.perf_counter() - t0
+    subprocess.run(args, check=True, env=dict(os.environ, MALLOC_PERTURB_="255"), stdout=subprocess.DEVNULL,
+                   stderr=subprocess.DEVNULL)
+    wall = time.perf_counter() - t0
+    secs = [float(l.split()[3]) for l in open(os.path.join(d, "ref_times.txt"))]
+    same = True
+    for k, (rpath, qpath) in enumerate(files):
+        rd = lib.load_reads(rpath, qpath, color=color)
+        v = SimpleNamespace(codes=rd["codes"], qual=rd["qual"], len=rd["len"], first_base=rd["first_base"],
+                            first_qual=rd["first_qual"])
+        out = os.path.join(d, f"gpu_{k}.sam")
+        m.map_sam(v, out, mode=lib.STORM_MODE_BATCHED, names=rd["names"])
+        same = same and sam_records_body(out) == sam_records_body(os.path.join(d, f"ref_{k}.sam"))
+    t_map = sum(secs)
+    return {"value": round(total / t_map, 1), "unit": "reads/s", "cores": cores(), "kind": "reference",
+            "sample": f"{total} reads in {len(runs)} run(s) of one read length each ({per_length if len(runs) > 1 else total}"
+                      f" per length); the reference's mapping phase (process_read, forward and reverse) summed over the "
+                      f"runs: {t_map:.2f}s; harness wall time with its one genome load and index build {wall:.1f}s",
+            "runs": len(runs), "map_seconds": round(t_map, 3),
+            "gpu_sam_identical_to_reference": bool(same)}
 
 
-def cpu_mapping_baseline(lib, d, gpath, pr, kw, color, n_sample, ordered=False):
-    """The reference binary (oracle/_ref, default Makefile flags, OpenMP) on a sample of the
-    workload's reads, timed on this host: one run on 16 reads gives the fixed cost (genome load,
-    index build), one on the sample; reads/s = sample / (difference).  The GPU maps the same files
-    and its SAM is compared with the reference's."""
-    if ref_binary(color) is None:
-        return None
-    cores = host_cores()
-    # the reference maps one read length per run (reads_data.c:156-163): a mixed-length workload
-    # is sampled at its median length
-    lens = np.asarray(pr.len)
-    if lens.min() != lens.max():
-        L_med = int(np.median(lens))
-        pick = np.nonzero(lens == L_med)[0]
-        length_note = f" (all of length {L_med}, the median: the reference takes one read length per run)"
-    else:
-        pick = np.arange(len(lens))
-        length_note = ""
-    n_sample = min(n_sample, len(pick))
-    r_tiny, _ = write_read_files(d, pr, pick[:16], color, "tiny")
-    r_samp, files = write_read_files(d, pr, pick[:n_sample], color, "sample")
-    t_fixed = run_reference(color, gpath, r_tiny, kw, os.path.join(d, "ref_tiny.sam"), cores)
-    t_all = run_reference(color, gpath, r_samp, kw, os.path.join(d, "ref_sample.sam"), cores)
-    gsam = os.path.join(d, "gpu_sample.sam")
-    lib.map_files(gpath, files[0], gsam, qual=files[1], mode=1, color=color, **kw)
-    same = sam_body(gsam) == sam_body(os.path.join(d, "ref_sample.sam"))
-    dt = max(t_all - t_fixed, 1e-6)
-    out = {"value": round(n_sample / dt, 1), "unit": "reads/s", "cores": cores, "kind": "reference",
-           "sample": f"{n_sample} reads of the workload{length_note}; reference {os.path.basename(ref_binary(color))} -N {cores}: "
-                     f"{t_all:.1f}s total minus {t_fixed:.1f}s fixed cost (load + index, measured on 16 reads)",
-           "gpu_sam_identical_to_reference": bool(same)}
-    if ordered:
-        # the same sample without -M: the ordered output (genome map placement, coordinate-sorted SAM)
-        ko = dict(kw, max_hits=0)
-        t_ref = run_reference(color, gpath, r_samp, ko, os.path.join(d, "ref_ordered.sam"), cores)
-        osam = os.path.join(d, "gpu_ordered.sam")
-        t0 = time.perf_counter()
-        so = lib.map_files(gpath, files[0], osam, qual=files[1], mode=1, color=color, **ko)
-        t_gpu = time.perf_counter() - t0
-        same_o = sam_body(osam) == sam_body(os.path.join(d, "ref_ordered.sam"))
-        out["ordered"] = {"reads": n_sample, "gpu_sam_identical_to_reference": bool(same_o),
-                          "gpu_call_s": round(t_gpu, 3), "gpu_map_s": round(so["seconds_map"], 3),
-                          "genome_map_and_sam_s": round(so["seconds_sam"], 3),
-                          "reference_s": round(t_ref, 2), "reference_minus_fixed_s": round(t_ref - t_fixed, 2)}
-        log(f"ordered mode on {n_sample} reads: GPU call {t_gpu:.2f}s (genome map + SAM {so['seconds_sam']:.2f}s), "
-            f"reference {t_ref:.2f}s; SAM identical: {same_o}")
-    return out
-
-
-def map_config(lib, name, gpath, pr, kw, color, n_sample_cpu, sam_reads, skip_plain=False, ordered=False,
-               ordered_reads=0):
-    """Batched (persistent fused kernel in a hipGraph) and plain (staged kernels, one batch per
-    read length) on the same resident mapper; hits returned to host memory; whole-call wall time
-    (the mean of repeated calls when one call takes under 0.1 s)."""
-    t0 = time.perf_counter()
-    m = lib.Mapper(gpath, color=color, **kw)
-    t_create = time.perf_counter() - t0
+def map_config(lib, name, gpath, pr, kw, color, chunk, sam_reads, cpu=True, ref_per_length=0, ref_max=0):
+    """Batched (persistent fused kernel in a hipGraph) and plain (staged kernels, one batch per read
+    length) on one resident mapper: reads/s to the whole hit map and tracebacks in host memory
+    (storm_mapper_map_records), in calls of up to `chunk` reads; first call (code objects, autotune,
+    graph capture, buffer growth) and warm pass; per-stage times summed over the warm calls; the
+    records of both paths compared; the SAM text formatted on the device timed apart."""
+    from storm_amd import lib as L
     n = len(pr.len)
-    w = min(n, 50_000)
-    m.map_hits(pr, mode=1, lo=0, hi=w)   # warm-up: code objects, autotune, graph capture
-    if not skip_plain:
-        m.map_hits(pr, mode=0, lo=0, hi=w)
-    res = {"reads": n, "genome_load_index_s": round(t_create, 2)}
-
-    def timed(mode):
-        """One whole call; a call shorter than 0.1 s is repeated for about 0.3 s and averaged."""
-        t0 = time.perf_counter()
-        h, st = m.map_hits(pr, mode=mode)
-        t = time.perf_counter() - t0
-        if t < 0.1:
-            reps = max(1, min(50, int(0.3 / max(t, 1e-4))))
-            t0 = time.perf_counter()
-            for _ in range(reps):
-                m.map_hits(pr, mode=mode)
-            t = (time.perf_counter() - t0) / reps
-        return h, st, t
-
-    hb, sb, tb = timed(1)
-    res["batched"] = {"reads_per_s": round(n / tb, 1), "seconds": round(tb, 3), "mapped": sb["n_mapped"],
-                      "gpu_ms_fused": round(sb["gpu_ms_filter"], 2),
-                      "staged_fallback_gpu_ms": round(sb["gpu_ms_seed"] + sb["gpu_ms_align"] + sb["gpu_ms_replay"], 2),
-                      "fallback_reads": sb["n_fallback"],
-                      "candidates": sb["n_candidates"], "filter_passes": sb["n_filter_pass"],
-                      "lookups": sb["n_lookups"], "raw_hits": sb["n_hits"]}
-    log(f"{name}: batched {n} reads in {tb:.2f}s = {n / tb:.0f} reads/s, mapped {sb['n_mapped']}, "
-        f"candidates {sb['n_candidates']}, fused kernel {sb['gpu_ms_filter']:.1f} ms")
-    if not skip_plain:
-        hp, sp, tp = timed(0)
-        same = all(np.array_equal(hb[k], hp[k]) for k in hb)
-        lk_bytes = 8 * sp["n_lookups"]
-        res["plain"] = {"reads_per_s": round(n / tp, 1), "seconds": round(tp, 3), "mapped": sp["n_mapped"],
-                        "gpu_ms": {k: round(sp["gpu_ms_" + k], 2) for k in ("seed", "lookup", "filter", "align",
-                                                                            "replay")},
-                        "index_lookup_GBps": round(lk_bytes / (sp["gpu_ms_lookup"] * 1e-3) / 1e9, 1)
-                        if sp["gpu_ms_lookup"] > 0 else None,
-                        # the same against the 8 TB/s HBM peak (8 B of first[] per (seed, strand, position)
-                        # lookup; random gathers, each touching its own cache line)
-                        "index_lookup_hbm_frac": round(lk_bytes / (sp["gpu_ms_lookup"] * 1e-3) / 8e12, 4)
-                        if sp["gpu_ms_lookup"] > 0 else None}
-        res["batched_over_plain"] = round(tp / tb, 2)
-        res["plain_equals_batched"] = bool(same)
-        log(f"{name}: plain {n / tp:.0f} reads/s (batched/plain = {tp / tb:.2f}x), identical hits: {same}")
+    M = kw["max_hits"]
+    t0 = time.perf_counter()
+    m = L.Mapper(gpath, color=color, seeds=SEEDS_C if color else SEEDS_N, **kw)
+    res = {"reads": n, "genome_load_index_s": round(time.perf_counter() - t0, 2), "max_hits": M,
+           "timed_region": "packed reads in host memory -> hit map records + tracebacks in host memory "
+                           "(storm_mapper_map_records); SAM text timed apart"}
+    chunks = [(lo, min(n, lo + chunk)) for lo in range(0, n, chunk)]
+    views = [m.view(pr, lo, hi) for lo, hi in chunks]
+    c0 = chunks[0][1] - chunks[0][0]
+    bufs = {mode: (np.zeros((c0, 4), L.HIT_RECORD), np.zeros((c0, M, L.TB_BYTES), np.uint8)) for mode in (0, 1)}
+    first = {}
+    for mode in (1, 0):   # first calls, on the first chunk
+        h, tb = bufs[mode]
+        t = time.perf_counter()
+        m.records_call(views[0][0], mode, h, tb)
+        first[mode] = time.perf_counter() - t
+    keys = ("gpu_ms_seed", "gpu_ms_lookup", "gpu_ms_filter", "gpu_ms_align", "gpu_ms_replay", "n_candidates",
+            "n_filter_pass", "n_hits", "n_lookups", "n_mapped", "n_fallback")
+    reps = 1
+    if len(chunks) == 1 and max(first.values()) < 0.15:
+        reps = 5   # a short workload: the warm figure is the mean of several calls
+    wall = {0: 0.0, 1: 0.0}
+    sums = {0: dict.fromkeys(keys, 0.0), 1: dict.fromkeys(keys, 0.0)}
+    same = True
+    for _ in range(reps):
+        for (v, keep), (lo, hi) in zip(views, chunks):
+            k = hi - lo
+            for mode in (1, 0):
+                h, tb = bufs[mode]
+                t = time.perf_counter()
+                st = m.records_call(v, mode, h[:k], tb[:k])
+                wall[mode] += time.perf_counter() - t
+                for q in keys:
+                    sums[mode][q] += st[q]
+            same = same and np.array_equal(bufs[0][0][:k], bufs[1][0][:k]) and np.array_equal(bufs[0][1][:k],
+                                                                                               bufs[1][1][:k])
+    for mode in (0, 1):
+        for q in keys:
+            sums[mode][q] /= reps
+        wall[mode] /= reps
+    tb_, tp = wall[1], wall[0]
+    sb, sp = sums[1], sums[0]
+    res["calls"] = {"chunk_reads": chunk, "calls_per_pass": len(chunks), "warm_passes": reps}
+    res["batched"] = {
+        "reads_per_s": round(n / tb_, 1), "seconds": round(tb_, 4), "mapped": int(sb["n_mapped"]),
+        "first_call_reads_per_s": round(c0 / first[1], 1), "first_call_s": round(first[1], 4),
+        "stage_ms": {"fused_kernel": round(sb["gpu_ms_filter"], 2),
+                     "staged_fallback": round(sb["gpu_ms_seed"] + sb["gpu_ms_align"] + sb["gpu_ms_replay"], 2),
+                     "host_and_copies": round(tb_ * 1e3 - sb["gpu_ms_filter"] - sb["gpu_ms_seed"] - sb["gpu_ms_align"] -
+                                              sb["gpu_ms_replay"], 2)},
+        "fallback_reads": int(sb["n_fallback"]), "candidates": int(sb["n_candidates"]),
+        "filter_passes": int(sb["n_filter_pass"]), "lookups": int(sb["n_lookups"]), "raw_hits": int(sb["n_hits"])}
+    gpu_p = sp["gpu_ms_seed"] + sp["gpu_ms_filter"] + sp["gpu_ms_align"] + sp["gpu_ms_replay"]
+    res["plain"] = {
+        "reads_per_s": round(n / tp, 1), "seconds": round(tp, 4), "mapped": int(sp["n_mapped"]),
+        "first_call_reads_per_s": round(c0 / first[0], 1), "first_call_s": round(first[0], 4),
+        "stage_ms": {"seeds_lookups_candidates": round(sp["gpu_ms_seed"], 2), "of_which_lookup": round(sp["gpu_ms_lookup"], 2),
+                     "filter": round(sp["gpu_ms_filter"], 2), "dp_traceback": round(sp["gpu_ms_align"], 2),
+                     "hit_map_replay": round(sp["gpu_ms_replay"], 2), "host_and_copies": round(tp * 1e3 - gpu_p, 2)},
+        "index_lookup_GBps": round(8 * sp["n_lookups"] / (sp["gpu_ms_lookup"] * 1e-3) / 1e9, 1)
+        if sp["gpu_ms_lookup"] > 0 else None}
+    if res["plain"]["index_lookup_GBps"]:
+        res["plain"]["index_lookup_hbm_frac"] = round(res["plain"]["index_lookup_GBps"] / HBM_PEAK_GBPS, 4)
+    res["batched_over_plain"] = round(tp / tb_, 2)
+    res["plain_records_equal_batched"] = bool(same)
+    log(f"{name}: batched {n / tb_:.0f} reads/s ({tb_:.3f}s; first call {first[1]:.3f}s), plain {n / tp:.0f} reads/s "
+        f"({tp:.3f}s) -> {tp / tb_:.2f}x; records identical: {same}")
+    del bufs
     if sam_reads:
         k = min(n, sam_reads)
-        st = m.map_sam(pr, "/dev/null", mode=1, lo=0, hi=k)
-        res["sam_writer_reads_per_s"] = round(k / max(st["seconds_sam"], 1e-9), 1)
-    m.close()
-    if ordered_reads:
-        # the reference's default output on the same reads: no -M, reads placed on the genome map
-        # (host, after the device hit map), coordinate-sorted SAM
-        k = min(n, ordered_reads)
-        mo = lib.Mapper(gpath, color=color, **dict(kw, max_hits=0))
-        st = mo.map_sam(pr, "/dev/null", mode=1, lo=0, hi=k)
-        mo.close()
-        res["ordered"] = {"reads": k, "gpu_map_s": round(st["seconds_map"], 3),
-                          "genome_map_and_sam_s": round(st["seconds_sam"], 3),
-                          "reads_per_s": round(k / max(st["seconds_map"] + st["seconds_sam"], 1e-9), 1)}
-        log(f"{name}: ordered mode on {k} reads: map {st['seconds_map']:.2f}s, genome map + SAM {st['seconds_sam']:.2f}s")
-    if n_sample_cpu:
-        cb = cpu_mapping_baseline(lib, os.path.dirname(gpath), gpath, pr, kw, color, n_sample_cpu, ordered)
+        st = m.map_sam(pr, os.devnull, mode=1, lo=0, hi=k)
+        res["sam_text"] = {"reads": k, "seconds": round(st["seconds_sam"], 4),
+                           "reads_per_s": round(k / max(st["seconds_sam"], 1e-9), 1),
+                           "where": "records formatted on the device (k_sam.hip), text copied to the host and written"}
+        log(f"{name}: SAM text {k / max(st['seconds_sam'], 1e-9):.0f} reads/s")
+    if cpu and ref_max:
+        cb = cpu_mapping_baseline(L, m, os.path.dirname(gpath), gpath, pr, kw, color, ref_per_length, ref_max)
         res["cpu_baseline"] = cb
         if cb:
-            log(f"{name}: reference CPU {cb['value']} reads/s on {cb['cores']} cores; GPU SAM identical on the sample: "
+            log(f"{name}: reference CPU {cb['value']} reads/s on {cb['cores']} threads; GPU SAM identical: "
                 f"{cb['gpu_sam_identical_to_reference']}")
+    m.close()
     return res
 
 
@@ -437,26 +493,22 @@ def config1(lib, cpu=True):
     synth.write_fasta_fast(gpath, "chr1", g)
     pr = synth.fast_nucleotide_reads(g, 100_000, 50, 101, sub=0.02, indel=0.001, qlo=20, qhi=40)
     kw = dict(allowed_indels=3, z=100, t=100, max_hits=1)
-    return map_config(lib, "config1", gpath, pr, kw, 0, 100_000 if cpu else 0, 100_000, ordered=True)
+    return map_config(lib, "config1", gpath, pr, kw, 0, 100_000, 100_000, cpu, 100_000, 100_000)
 
 
-def config3(lib, cpu=True, n_reads=5_000_000):
+def config3(lib, cpu=True, n_reads=5_000_000, g=None, gpath=None):
     """50 Mbp genome with GC drawn in 40-60 % per 10 kb window; 5M SOLiD color reads of 34-74
     colors after the primer (35-75 bp reads), 3 % color errors, 1 % SNPs, Q10-35; the three
     default weight-12 color seeds, -i 3 -z 85 -t 100 -M 1."""
     from storm_amd import synth
-    d = tempfile.mkdtemp(prefix="storm_c3_")
-    g = synth.gc_window_genome(50_000_000, 3)
-    gpath = os.path.join(d, "g.fa")
-    synth.write_fasta_fast(gpath, "chr1", g)
     t0 = time.perf_counter()
     pr = synth.fast_color_reads(g, n_reads, (34, 74), 303, snp=0.01, color_err=0.03, qlo=10, qhi=35)
     log(f"config3: {n_reads} reads generated in {time.perf_counter() - t0:.1f}s")
     kw = dict(allowed_indels=3, z=85, t=100, max_hits=1)
-    return map_config(lib, "config3", gpath, pr, kw, 1, 150_000 if cpu else 0, 500_000)
+    return map_config(lib, "config3", gpath, pr, kw, 1, n_reads, 1_000_000, cpu, 2000, 82_000)
 
 
-def config4(lib, cpu=True, n_reads=20_000_000, skip_plain=False):
+def config4(lib, cpu=True, n_reads=20_000_000):
     """100 Mbp genome: i.i.d. background, 10 % segmental duplications (1-20 kb copies, 1-5 %
     divergence), 5 % tandem repeats (units of 2-50 bp); 20M reads of 100 nt, 1.5 % substitutions,
     0.2 % indels, Q15-40; the three default weight-12 seeds, -i 7 -z 100 -t 115 -M 4."""
@@ -469,10 +521,10 @@ def config4(lib, cpu=True, n_reads=20_000_000, skip_plain=False):
     pr = synth.fast_nucleotide_reads(g, n_reads, 100, 404, sub=0.015, indel=0.002, qlo=15, qhi=40)
     log(f"config4: {n_reads} reads generated in {time.perf_counter() - t0:.1f}s")
     kw = dict(allowed_indels=7, z=100, t=115, max_hits=4)
-    return map_config(lib, "config4", gpath, pr, kw, 0, 10_000 if cpu else 0, 200_000, skip_plain=skip_plain,
-                      ordered_reads=min(n_reads, 2_000_000))
+    return map_config(lib, "config4", gpath, pr, kw, 0, min(n_reads, 4_000_000), 1_000_000, cpu, 40_000, 40_000)
 
 
+# ------------------------------------------------------------------------------------------ serving
 def serve_trace(g, n_req: int, seed: int = 5, lo: int = 1_000, hi: int = 1_000_000):
     """Requests of the serving workload: sizes log-uniform in [lo, hi] reads, read lengths uniform
     in 36-150, half nucleotide and half color reads, -i 3 or 7, drawn from `seed`."""
@@ -494,18 +546,23 @@ def serve_trace(g, n_req: int, seed: int = 5, lo: int = 1_000, hi: int = 1_000_0
 
 def serve_run(server, reqs, arrivals):
     """Open loop: request k is submitted at arrivals[k] seconds (Poisson process), a waiter thread
-    per request collects it.  Returns per-request latencies and the wall time from the first
-    submission to the last result."""
-    import threading
+    per request collects its hits and SAM records.  Returns per-request latencies (submit -> last
+    SAM record ready), the wall time from the first submission to the last result, and per request
+    a digest of its hits and SAM text."""
     lat = [0.0] * len(reqs)
     info = [None] * len(reqs)
+    digest = [None] * len(reqs)
     errs = []
 
     def waiter(k, rid):
         try:
-            _, res = server.wait(rid)
+            h, res, sam = server.wait_sam(rid)
             lat[k] = res["seconds_latency"]
             info[k] = res
+            hh = hashlib.blake2b(digest_size=16)
+            for f in ("score", "ref_id", "ref_start", "sense"):
+                hh.update(h[f].tobytes())
+            digest[k] = (hh.hexdigest(), hashlib.blake2b(sam, digest_size=16).hexdigest(), len(sam))
         except Exception as e:   # noqa: BLE001 - reported after the run
             errs.append(f"request {k}: {e}")
 
@@ -524,25 +581,22 @@ def serve_run(server, reqs, arrivals):
     wall = time.perf_counter() - t0
     if errs:
         raise RuntimeError("; ".join(errs[:3]))
-    return np.array(lat), wall, info
+    return np.array(lat), wall, info, digest
 
 
-def config5(lib, n_req=40, load=0.8, lanes=3, max_batch=1 << 18):
+def config5(lib, g, gpath, n_req=40, load=0.8, lanes=3, max_batch=1 << 18):
     """Serving: the config-3 genome (50 Mbp, GC 40-60 % per 10 kb window) with nucleotide and color
     indexes resident; Poisson arrivals of requests of 1k-1M reads (log-uniform), lengths 36-150,
     50/50 nucleotide/color, -i 3 or 7 per request, -M 1; admission by free HBM; continuous
-    batching vs the plain per-request path on the same arrival trace."""
+    batching, SAM records per request; against the plain per-request path on the same arrivals."""
     from storm_amd import synth
-    d = tempfile.mkdtemp(prefix="storm_c5_")
-    g = synth.gc_window_genome(50_000_000, 3)
-    gpath = os.path.join(d, "g.fa")
-    synth.write_fasta_fast(gpath, "chr1", g)
     t0 = time.perf_counter()
     reqs = serve_trace(g, n_req)
     sizes = np.array([len(pr.len) for pr, _ in reqs])
     total = int(sizes.sum())
     log(f"config5: {n_req} requests, {total} reads generated in {time.perf_counter() - t0:.1f}s")
-    res = {"requests": n_req, "reads": total, "request_reads_min_max": [int(sizes.min()), int(sizes.max())]}
+    res = {"requests": n_req, "reads": total, "request_reads_min_max": [int(sizes.min()), int(sizes.max())],
+           "latency": "submit -> the request's last SAM record ready (records formatted on the device per batch)"}
     t0 = time.perf_counter()
     srv = lib.Server(gpath, lanes=lanes, max_batch_reads=max_batch)
     res["server_create_s"] = round(time.perf_counter() - t0, 2)
@@ -554,16 +608,16 @@ def config5(lib, n_req=40, load=0.8, lanes=3, max_batch=1 << 18):
     for _ in range(lanes):
         ids = [srv.submit(pr, **kw) for pr, kw in warm]
         for rid in ids:
-            srv.wait(rid)
+            srv.wait_sam(rid)
     # capacity: every request submitted at once (closed loop)
-    lat_c, wall_c, _ = serve_run(srv, reqs, np.zeros(n_req))
+    _, wall_c, _, _ = serve_run(srv, reqs, np.zeros(n_req))
     cap = total / wall_c
     log(f"config5: saturated batched capacity {cap:.0f} reads/s ({wall_c:.2f}s)")
     rate = load * cap / sizes.mean()
     arrivals = np.cumsum(synth.rng(55).exponential(1.0 / rate, size=n_req))
     arrivals -= arrivals[0]
     s0 = srv.stats()
-    lat_b, wall_b, info_b = serve_run(srv, reqs, arrivals)
+    lat_b, wall_b, info_b, dig_b = serve_run(srv, reqs, arrivals)
     s1 = srv.stats()
     srv.close()
     nb = s1["batches"] - s0["batches"]
@@ -577,89 +631,33 @@ def config5(lib, n_req=40, load=0.8, lanes=3, max_batch=1 << 18):
                       "mean_requests_per_batch": round((s1["batch_requests"] - s0["batch_requests"]) / max(nb, 1), 2),
                       "max_lanes_busy": s1["max_lanes_busy"], "admission_waits": s1["admission_waits"],
                       "arena_bytes": s1["arena_bytes"], "arena_peak_bytes": s1["arena_peak_bytes"],
-                      "mapped": int(sum(i["n_mapped"] for i in info_b))}
+                      "mapped": int(sum(i["n_mapped"] for i in info_b)),
+                      "sam_bytes": int(sum(x[2] for x in dig_b))}
     log(f"config5: batched {total / wall_b:.0f} reads/s, p50 {np.percentile(lat_b, 50):.3f}s "
         f"p99 {np.percentile(lat_b, 99):.3f}s")
     t0 = time.perf_counter()
     srv = lib.Server(gpath, mode=lib.STORM_MODE_PLAIN)
     log(f"config5: plain server created in {time.perf_counter() - t0:.1f}s")
     for pr, kw in warm:
-        srv.wait(srv.submit(pr, **kw))
-    lat_p, wall_p, info_p = serve_run(srv, reqs, arrivals)
+        srv.wait_sam(srv.submit(pr, **kw))
+    lat_p, wall_p, info_p, dig_p = serve_run(srv, reqs, arrivals)
     srv.close()
     res["plain"] = {"reads_per_s": round(total / wall_p, 1), "wall_s": round(wall_p, 3),
                     "p50_latency_s": round(float(np.percentile(lat_p, 50)), 4),
                     "p99_latency_s": round(float(np.percentile(lat_p, 99)), 4),
                     "mapped": int(sum(i["n_mapped"] for i in info_p))}
     res["batched_over_plain"] = round(wall_p / wall_b, 2)
+    res["served_equals_plain_per_request"] = {
+        "hits": all(a[0] == b[0] for a, b in zip(dig_b, dig_p)),
+        "sam": all(a[1] == b[1] for a, b in zip(dig_b, dig_p))}
     log(f"config5: plain {total / wall_p:.0f} reads/s, p50 {np.percentile(lat_p, 50):.3f}s "
-        f"p99 {np.percentile(lat_p, 99):.3f}s; batched/plain {wall_p / wall_b:.2f}x")
+        f"p99 {np.percentile(lat_p, 99):.3f}s; batched/plain {wall_p / wall_b:.2f}x; identical: "
+        f"{res['served_equals_plain_per_request']}")
     return res
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--reads", type=int, default=5_000_000, help="reads of the filter workload (2 windows each)")
-    ap.add_argument("--no-map", action="store_true")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--percall", type=int, default=20000)
-    ap.add_argument("--dp-records", type=int, default=2_000_000, help="records of the DP stage measurement")
-    ap.add_argument("--config", default="default", choices=["default", "1", "3", "4", "5"],
-                    help="default: filter headline + config-1 mapping; 1/3/4: that mapping workload alone")
-    ap.add_argument("--reads-c3", type=int, default=5_000_000)
-    ap.add_argument("--reads-c4", type=int, default=20_000_000)
-    ap.add_argument("--no-plain", action="store_true", help="mapping workloads: batched path only")
-    ap.add_argument("--requests", type=int, default=40, help="config 5: requests in the arrival trace")
-    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
-                    help="first copy MIB MiB device to device 3 times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
-    a = ap.parse_args()
-    if a.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the path shards reads across GPUs by design "
-              f"but this build measures one GPU.", file=sys.stderr)
-        sys.exit(2)
-    import torch
-    from storm_amd import lib
-    assert torch.cuda.is_available()
-    torch.zeros(1, device="cuda:0")
-    lib.load()
-    if a.calib_copy:
-        src = torch.ones(a.calib_copy << 18, dtype=torch.float32, device="cuda:0")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"calibration: copied {a.calib_copy} MiB device to device 3 times")
-        del src, dst
-    if a.config == "5":
-        res = config5(lib, n_req=a.requests)
-        best = res["batched"]
-        line = {"metric": METRIC, "value": best["reads_per_s"], "unit": "reads/s", "n_gpus": 1, "steps": 1,
-                "warmup": 1, "ms_per_step": round(best["wall_s"] * 1e3, 2), "higher_is_better": True,
-                "scaling": "weak", "vs_baseline": None, "dtype": "u8/int32", "data": "synthetic",
-                "config": {"workload": "serving config 5: " + " ".join(config5.__doc__.split())},
-                "p50_latency_s": best["p50_latency_s"], "p99_latency_s": best["p99_latency_s"], "serving": res}
-        print(json.dumps(line), flush=True)
-        return
-    if a.config != "default":
-        cpu = not a.no_cpu
-        if a.config == "1":
-            res = config1(lib, cpu)
-        elif a.config == "3":
-            res = config3(lib, cpu, a.reads_c3)
-        else:
-            res = config4(lib, cpu, a.reads_c4, skip_plain=a.no_plain)
-        best = res["batched"]
-        line = {"metric": METRIC, "value": best["reads_per_s"], "unit": "reads/s", "n_gpus": 1, "steps": 1,
-                "warmup": 1, "ms_per_step": round(best["seconds"] * 1e3, 2), "higher_is_better": True,
-                "scaling": "weak", "vs_baseline": None, "dtype": "u8/int32", "data": "synthetic",
-                "config": {"workload": f"mapping config {a.config}: " + " ".join(
-                    {"1": config1, "3": config3, "4": config4}[a.config].__doc__.split())},
-                "mapping": res}
-        print(json.dumps(line), flush=True)
-        return
+# ------------------------------------------------------------------------------------------ main
+def filter_and_dp(lib, a):
     L, W, d = 75, 8, 7
     t0 = time.perf_counter()
     g, reads, pos, stride = make_filter_workload(a.reads, L, W, d)
@@ -670,54 +668,33 @@ def main():
     gcups = cells / (ms * 1e-3) / 1e9
     log(f"filter: {ms:.3f} ms/launch (events), {wall * 1e3:.3f} ms/step (wall), {gcups:.1f} GCUPS, "
         f"pass rate {passf.mean():.3f}")
-    # part of the CPU-baseline leg: a spot check against the CPU restatement (the full parity tests
-    # live in tests/)
-    spot = None
-    orc = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
-    if not a.no_cpu and os.path.exists(orc):
-        from storm_amd import synth
-        ol = C.CDLL(orc)
-        ol.orc_filter_score.restype = C.c_int
-        ol.orc_filter_score.argtypes = [C.c_void_p, C.c_long, C.c_void_p, C.c_int, C.c_int] + [C.c_int] * 4
-        pad = 64
-        gp = np.zeros(pad + (len(g) + 3) // 4 + pad, np.uint8)
-        gp[pad:pad + (len(g) + 3) // 4] = synth.pack_codes(g)
-        sel = np.linspace(0, n_cand - 1, 2000).astype(np.int64)
-        want = np.array([min(255, ol.orc_filter_score(C.c_void_p(gp.ctypes.data + pad), int(cp[c]),
-                                                      C.c_void_p(reads[int(cr[c])].ctypes.data), L, W, 5, -4, -9, -4))
-                         for c in sel])
-        spot = bool(np.array_equal(scores[sel].astype(np.int64), want))
-        log(f"spot check vs oracle on 2000 candidates: {'identical' if spot else 'MISMATCH'}")
-    # stage (d): the DP + traceback of the true-origin passes
+    spot = None if a.no_cpu else oracle_spot_check(g, reads, scores, cr, cp, L, W)
     n_dp = min(a.dp_records, a.reads)
-    dp_ms, dp_res, dp_tb, dp_rs, dp_diag = gpu_align_stage(lib, g, reads, pos, stride, L, 7, d, n_dp, max(1, a.steps // 4))
-    dp_cells = n_dp * L * 15
-    dp_gcups = dp_cells / (dp_ms * 1e-3) / 1e9
+    dp_ms, dp_res, dp_tb, dp_diag = gpu_align_stage(lib, g, reads, pos, stride, L, 7, d, n_dp, max(1, a.steps // 4))
+    dp_gcups = n_dp * L * 15 / (dp_ms * 1e-3) / 1e9
     n_tb = int((dp_res["tb_len"] > 0).sum())
-    log(f"align stage: {n_dp} records in {dp_ms:.2f} ms = {dp_gcups:.1f} GCUPS ({n_dp / dp_ms / 1e3:.1f} M records/s), "
-        f"{n_tb} tracebacks")
-    pc_dt, _ = percall_filter(lib, g, reads, pos, stride, L, W, min(a.percall, a.reads))
-    pc_gcups = 2 * min(a.percall, a.reads) * L * 2 * W / pc_dt / 1e9
-    log(f"per-call drop-in row: {pc_dt:.2f}s for {min(a.percall, a.reads)} calls, {pc_gcups:.3f} GCUPS")
-    mapping = None if a.no_map else config1(lib, cpu=not a.no_cpu)
+    log(f"align stage: {n_dp} records in {dp_ms:.2f} ms = {dp_gcups:.1f} GCUPS, {n_tb} tracebacks")
+    pc_n = min(a.percall, a.reads)
+    pc_dt = percall_filter(lib, g, reads, pos, L, pc_n)
+    pc_gcups = 2 * pc_n * L * 2 * W / pc_dt / 1e9
+    log(f"per-call drop-in row: {pc_dt:.2f}s for {pc_n} calls, {pc_gcups:.3f} GCUPS")
     cpu = None
-    if not a.no_cpu:
-        cb = cpu_baseline(g, reads, pos, stride, L, d, n_sample=500_000)
-        if cb and "sse2" in cb:
-            nc, t, cores = cb["sse2"]
-            cpu = {"value": round(nc * L * 2 * W / t / 1e9, 3), "unit": "GCUPS", "cores": cores, "kind": "reference",
-                   "sample": f"reference SSE2 filter (alignment_sse2__align_pair, default Makefile flags) over "
-                             f"{nc} candidate evaluations of the same workload (first 500k reads x 2 windows, "
-                             f"repeated), OpenMP, {t:.1f}s"}
-            if "avx2" in cb:
-                nc2, t2, _ = cb["avx2"]
-                cpu["avx2_gcups"] = round(nc2 * L * 2 * W / t2 / 1e9, 3)
-            log(f"cpu baseline: {cpu['value']} GCUPS on {cores} threads")
     dp_cpu = None
     if not a.no_cpu:
-        dp_cpu = cpu_align_baseline(g, reads, L, 7, dp_rs, dp_diag, dp_res, dp_tb)
+        cb = cpu_filter_baseline(g, reads, pos, L, d, n_sample=500_000)
+        if "sse2" in cb:
+            nc, t = cb["sse2"]
+            cpu = {"value": round(nc * L * 2 * W / t / 1e9, 3), "unit": "GCUPS", "cores": cores(), "kind": "reference",
+                   "sample": f"the reference's SSE2 filter (alignment_sse2__align_pair, default Makefile flags, "
+                             f"libref_filter_sse2.so) over {nc} candidate evaluations of the same workload (first "
+                             f"500k reads x 2 windows, repeated), OpenMP on {cores()} threads, {t:.1f}s"}
+            if "avx2" in cb:
+                nc2, t2 = cb["avx2"]
+                cpu["avx2_gcups"] = round(nc2 * L * 2 * W / t2 / 1e9, 3)
+            log(f"cpu baseline: {cpu['value']} GCUPS (SSE2), {cpu.get('avx2_gcups')} (AVX2) on {cores()} threads")
+        dp_cpu = cpu_align_baseline(g, reads, L, 7, 115, dp_diag, dp_res, dp_tb)
         if dp_cpu:
-            log(f"align stage cpu baseline: {dp_cpu['value']} GCUPS on 1 thread; GPU records identical on the sample: "
+            log(f"align stage cpu baseline: {dp_cpu['value']} GCUPS on {cores()} threads; GPU records identical: "
                 f"{dp_cpu['gpu_records_identical']}")
     dp_achieved = dp_gcups * VALU_INSTR_PER_DP_CELL_I7_L75 / 1e3
     align_stage = {
@@ -743,7 +720,7 @@ def main():
                      "traffic": int(HBM_BYTES_PER_CAND_W8_L75 * n_cand),
                      "work_per_cell": VALU_INSTR_PER_CELL_W8_L75,
                      "hbm": {"achieved": round(ALG_BYTES_PER_CAND_W8_L75 * n_cand / (ms * 1e-3) / 1e9, 1),
-                             "peak": 8000.0, "unit": "GB/s",
+                             "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                              "frac": round(ALG_BYTES_PER_CAND_W8_L75 * n_cand / (ms * 1e-3) / 8e12, 4),
                              "algorithmic_bytes_per_candidate": ALG_BYTES_PER_CAND_W8_L75,
                              "measured_bytes_per_candidate": HBM_BYTES_PER_CAND_W8_L75},
@@ -756,8 +733,80 @@ def main():
         "batched_over_percall": round(gcups / pc_gcups, 1),
         "spot_check_identical": spot,
         "align_stage": align_stage,
-        "mapping": mapping,
     }
+    return line
+
+
+def main():
+    global HOST
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--reads", type=int, default=5_000_000, help="reads of the filter workload (2 windows each)")
+    ap.add_argument("--no-cpu", action="store_true", help="leave out the reference CPU baselines")
+    ap.add_argument("--percall", type=int, default=20000)
+    ap.add_argument("--dp-records", type=int, default=2_000_000, help="records of the DP stage measurement")
+    ap.add_argument("--config", default="all", choices=["all", "2", "1", "3", "4", "5"],
+                    help="all (default): every workload in one line; 2: the filter headline alone; 1/3/4/5: one workload")
+    ap.add_argument("--reads-c3", type=int, default=5_000_000)
+    ap.add_argument("--reads-c4", type=int, default=20_000_000)
+    ap.add_argument("--requests", type=int, default=40, help="config 5: requests in the arrival trace")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device 3 times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the path shards reads across GPUs by design "
+              f"but this build measures one GPU.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from storm_amd import lib, synth
+    assert torch.cuda.is_available()
+    torch.zeros(1, device="cuda:0")
+    lib.load()
+    HOST = host_info()
+    log(f"host: {json.dumps(HOST)}")
+    if a.calib_copy:
+        src = torch.ones(a.calib_copy << 18, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration: copied {a.calib_copy} MiB device to device 3 times")
+        del src, dst
+    cpu = not a.no_cpu
+    t_all = time.perf_counter()
+    line = None
+    if a.config in ("all", "2"):
+        line = filter_and_dp(lib, a)
+    cfgs = {}
+    if a.config in ("all", "1"):
+        cfgs["1"] = config1(lib, cpu)
+    if a.config in ("all", "3", "5"):
+        d = tempfile.mkdtemp(prefix="storm_c3_")
+        g3 = synth.gc_window_genome(50_000_000, 3)
+        g3path = os.path.join(d, "g.fa")
+        synth.write_fasta_fast(g3path, "chr1", g3)
+        if a.config in ("all", "3"):
+            cfgs["3"] = config3(lib, cpu, a.reads_c3, g3, g3path)
+        if a.config in ("all", "5"):
+            cfgs["5"] = config5(lib, g3, g3path, n_req=a.requests)
+    if a.config in ("all", "4"):
+        cfgs["4"] = config4(lib, cpu, a.reads_c4)
+    if line is None:   # one mapping workload alone: its batched reads/s is the line's value
+        k = next(iter(cfgs))
+        r = cfgs[k]
+        best = r["batched"]
+        line = {"metric": METRIC, "value": best["reads_per_s"], "unit": "reads/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+                "ms_per_step": round(best.get("seconds", best.get("wall_s", 0)) * 1e3, 2), "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "u8/int32", "data": "synthetic",
+                "config": {"workload": f"mapping config {k}: " + " ".join(
+                    {"1": config1, "3": config3, "4": config4, "5": config5}[k].__doc__.split())}}
+    if cfgs:
+        line["configs"] = cfgs
+        line["batched_over_plain"] = {k: v.get("batched_over_plain") for k, v in cfgs.items()}
+    line["host"] = HOST
+    line["bench_seconds"] = round(time.perf_counter() - t_all, 1)
     print(json.dumps(line), flush=True)
 
 

@@ -283,7 +283,7 @@ int storm_mapper_map_hits(storm_mapper *m, const storm_reads_view *reads, int mo
  * k < 4 the entry hits[4r + k] (HitInfoType, hit_map.h:17-40; ref_id = -1 for an empty slot) and,
  * when tb is not NULL, the traceback of ranks k < max_hits (all four in the ordered mode) at
  * tb + (r * max_hits + k) * STORM_TB_BYTES in alignment__traceback's packing (TRACEBACK_SEQ_SET,
- * alignment.h:81-83; tb_len codes); bytes of empty slots are not written.  This is what
+ * alignment.h:81-83; tb_len codes, the rest of the STORM_TB_BYTES zero, all zero for an empty slot).  This is what
  * hit_map__update leaves for hit_map__generate_SAM_output (hit_map.c:50-144, :197-504). */
 typedef struct storm_hit {
   int32_t score, ref_id, ref_start, ref_end, read_start, read_end, sense, tb_len;

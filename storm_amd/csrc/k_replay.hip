@@ -222,11 +222,30 @@ __global__ __launch_bounds__(256) void k_hit_records(const Slot* slots, const in
   h.read_start = s.read_start; h.read_end = s.read_end; h.sense = s.sense;
   h.tb_len = s.ref_id == -1 ? 0 : s.tb_len;
   out[row * 4 + k] = h;
-  if (tb_out && k < max_rank && s.ref_id != -1 && s.pass == PASS_CELL) {
+  if (tb_out && k < max_rank) {
+    // the traceback's (tb_len + 1) / 2 bytes, the rest of the record zero (all of it for an empty
+    // slot), so the output depends on the hit map alone
+    const bool hit = s.ref_id != -1 && s.pass == PASS_CELL;
+    const int nb = hit ? (h.tb_len + 1) / 2 : 0;
     const uint4* src = (const uint4*)(cells + ((t & ~3LL) + s.cell) * (int64_t)TBB);
     uint4* dst = (uint4*)(tb_out + (row * max_rank + k) * (int64_t)TBB);
 #pragma unroll
-    for (int w = 0; w < TBB / 16; ++w) dst[w] = src[w];
+    for (int w = 0; w < TBB / 16; ++w) {
+      uint4 x = make_uint4(0, 0, 0, 0);
+      if (16 * w < nb) {
+        x = src[w];
+        const int keep = nb - 16 * w;   // bytes of this word that belong to the traceback
+        if (keep < 16) {
+          uint32_t* xs = (uint32_t*)&x;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int kb = keep - 4 * q;
+            xs[q] = kb >= 4 ? xs[q] : kb <= 0 ? 0u : (xs[q] & ((1u << (8 * kb)) - 1u));
+          }
+        }
+      }
+      dst[w] = x;
+    }
   }
 }
 

@@ -433,6 +433,13 @@ class Mapper:
         del keep
         return hits, tb, st.as_dict()
 
+    def records_call(self, v: ReadsView, mode: int, hits: np.ndarray, tb) -> dict:
+        """storm_mapper_map_records on a prepared view (Mapper.view) into the given arrays."""
+        st = MapStats()
+        check(load().storm_mapper_map_records(self.h, C.byref(v), mode, _ptr(hits), None if tb is None else _ptr(tb),
+                                              C.byref(st)))
+        return st.as_dict()
+
     def free_bytes(self) -> int:
         return int(load().storm_mapper_free_bytes(self.h))
 

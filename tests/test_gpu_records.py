@@ -34,6 +34,7 @@ def test_records_plain_equal_batched(color, I, M, tmp_path):
     hh, _ = m.map_hits(pr, mode=lib.STORM_MODE_BATCHED)
     m.close()
     assert np.array_equal(hp, hb)
+    assert np.array_equal(tp, tb)   # traceback bytes past tb_len are zero on both paths
     for f in ("score", "ref_id", "ref_start", "sense"):
         assert np.array_equal(hb[f], hh[f]), f
     n_tb = 0
