@@ -11,7 +11,9 @@
 // run, no atomics), an exclusive scan of the counts left by the repeat filter gives `first`, and
 // every sorted position is written at first[key] + its rank inside the run.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
 
 #include "storm_kernels.h"
 
@@ -161,25 +163,20 @@ hipError_t launch_index_scatter(const uint32_t* keys, const int32_t* vals, int64
   return hipGetLastError();
 }
 
-// stable LSD radix sort of (key, position) pairs over bits [0, end_bit); the result is in
-// keys.Current() / vals.Current()
-static hipError_t sort_key_pos(hipcub::DoubleBuffer<uint32_t>& keys, hipcub::DoubleBuffer<int32_t>& vals, int64_t n,
-                        int end_bit, void* tmp, size_t& tmp_bytes, hipStream_t st) {
-  return hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, keys, vals, (int)n, 0, end_bit, st);
-}
-
+// stable LSD radix sort of (key, position) pairs over bits [0, end_bit); result_sel = 1 when the
+// sorted pairs end in (k1, v1), 0 when in (k0, v0)
 hipError_t sort_key_pos_raw(uint32_t* k0, uint32_t* k1, int32_t* v0, int32_t* v1, int64_t n, int end_bit, void* tmp,
                             size_t& tmp_bytes, int& result_sel, hipStream_t st) {
-  hipcub::DoubleBuffer<uint32_t> kb(k0, k1);
-  hipcub::DoubleBuffer<int32_t> vb(v0, v1);
-  hipError_t e = sort_key_pos(kb, vb, n, end_bit, tmp, tmp_bytes, st);
-  result_sel = kb.selector;
+  rocprim::double_buffer<uint32_t> kb(k0, k1);
+  rocprim::double_buffer<int32_t> vb(v0, v1);
+  hipError_t e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kb, vb, (size_t)n, 0, (unsigned)end_bit, st);
+  result_sel = kb.current() == k0 ? 0 : 1;
   return e;
 }
 
 hipError_t scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, void* tmp, size_t& tmp_bytes,
                               hipStream_t st) {
-  return hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, in, out, (int)n, st);
+  return rocprim::exclusive_scan(tmp, tmp_bytes, in, out, (int32_t)0, (size_t)n, rocprim::plus<int32_t>(), st);
 }
 
 }  // namespace storm

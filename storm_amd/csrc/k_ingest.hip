@@ -9,7 +9,9 @@
 // the last-32-codes window, reads_data.c:494-502).  The reads are then ordered by tag with a stable
 // radix sort (the reference sorts them with qsort on the tag, reads_data.c:59-67).
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
 
 #include "storm_kernels.h"
 
@@ -184,10 +186,10 @@ int64_t nl_chunks(int64_t n) { return (n + NL_CHUNK - 1) / NL_CHUNK; }
 // stable LSD radix sort of (tag, read index) pairs; the order ends in idx.Current()
 hipError_t sort_tags(uint64_t* k0, uint64_t* k1, int32_t* v0, int32_t* v1, int64_t n, void* tmp, size_t& tmp_bytes,
                      int& result_sel, hipStream_t st) {
-  hipcub::DoubleBuffer<uint64_t> kb(k0, k1);
-  hipcub::DoubleBuffer<int32_t> vb(v0, v1);
-  hipError_t e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, kb, vb, (int)n, 0, 64, st);
-  result_sel = vb.selector;
+  rocprim::double_buffer<uint64_t> kb(k0, k1);
+  rocprim::double_buffer<int32_t> vb(v0, v1);
+  hipError_t e = rocprim::radix_sort_pairs(tmp, tmp_bytes, kb, vb, (size_t)n, 0, 64, st);
+  result_sel = vb.current() == v0 ? 0 : 1;
   return e;
 }
 

@@ -13,7 +13,9 @@
 // and finished by the overflow path (device radix sort of its segment, then k_unique_segments).
 #include "storm_kernels.h"
 
-#include <hipcub/hipcub.hpp>
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
 
 namespace storm {
 
@@ -279,16 +281,25 @@ hipError_t launch_make_records(const RecArgs& a, hipStream_t st) {
 }
 
 hipError_t scan_exclusive_i64(const int64_t* in, int64_t* out, int64_t n, void* tmp, size_t& tmp_bytes, hipStream_t st) {
-  return hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, in, out, (int)n, st);
+  return rocprim::exclusive_scan(tmp, tmp_bytes, in, out, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), st);
 }
+
+namespace {
+struct WidenU8 {
+  __host__ __device__ int64_t operator()(uint8_t x) const { return (int64_t)x; }
+};
+}  // namespace
+
 hipError_t scan_exclusive_u8(const uint8_t* in, int64_t* out, int64_t n, void* tmp, size_t& tmp_bytes, hipStream_t st) {
-  hipcub::TransformInputIterator<int64_t, hipcub::CastOp<int64_t>, const uint8_t*> it(in, hipcub::CastOp<int64_t>());
-  return hipcub::DeviceScan::ExclusiveSum(tmp, tmp_bytes, it, out, (int)n, st);
+  rocprim::transform_iterator<const uint8_t*, WidenU8, int64_t> it(in, WidenU8());
+  return rocprim::exclusive_scan(tmp, tmp_bytes, it, out, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), st);
 }
+
+// the raw diagonals of each overflowing read strand, ascending (segments [begin, end) of `in`)
 hipError_t segsort_i32(const int32_t* in, int32_t* out, int64_t n_items, int64_t n_seg, const int64_t* begin,
                        const int64_t* end, void* tmp, size_t& tmp_bytes, hipStream_t st) {
-  return hipcub::DeviceSegmentedRadixSort::SortKeys(tmp, tmp_bytes, in, out, (int)n_items, (int)n_seg, begin, end, 0,
-                                                    32, st);
+  return rocprim::segmented_radix_sort_keys(tmp, tmp_bytes, in, out, (unsigned)n_items, (unsigned)n_seg, begin, end, 0,
+                                            32, st);
 }
 
 }  // namespace storm

@@ -291,7 +291,7 @@ hipError_t launch_hits_out(const Slot* slots, const int32_t* order, int64_t n, i
                            int32_t* ref_start, int32_t* sense, hipStream_t st);
 hipError_t launch_overflow_ranges(const int64_t* seg_off, const uint8_t* overflow, int64_t n_rs, int64_t* begin,
                                   int64_t* end, hipStream_t st);
-// device-wide helpers (hipcub)
+// device-wide helpers (rocPRIM)
 hipError_t scan_exclusive_i64(const int64_t* in, int64_t* out, int64_t n, void* tmp, size_t& tmp_bytes, hipStream_t st);
 hipError_t scan_exclusive_u8(const uint8_t* in, int64_t* out, int64_t n, void* tmp, size_t& tmp_bytes, hipStream_t st);
 hipError_t segsort_i32(const int32_t* in, int32_t* out, int64_t n_items, int64_t n_seg, const int64_t* begin,
