@@ -1,4 +1,5 @@
-"""The C-ABI library loads and exports every function include/storm_gpu.h declares (no GPU call)."""
+"""The C-ABI library loads and exports every function include/storm_gpu.h and
+include/storm_ref_dropin.h declare (no GPU call)."""
 import ctypes as C
 import os
 import re
@@ -13,6 +14,26 @@ def test_library_exports_every_declared_symbol():
     names = set(re.findall(r"\b((?:storm|alignment_gfx950)_\w+)\s*\(", hdr))
     assert len(names) > 20
     for n in sorted(names):
+        assert hasattr(so, n), n
+
+
+def test_library_exports_reference_signature_dropins():
+    """storm_ref_dropin.h maps each name to its _nt / _cs build by NUCLEOTIDES; both must exist."""
+    from storm_amd import lib
+    so = lib.load()
+    hdr = open(os.path.join(ROOT, "include", "storm_ref_dropin.h")).read()
+    declared = set(re.findall(r"\b((?:storm|index_gfx950|alignment_gfx950)_\w+)\s*\(", hdr))
+    mapped = dict(re.findall(r"#define\s+(\w+)\s+(\w+_(?:nt|cs))\b", hdr))
+    assert {"index_gfx950__build_reference", "index_gfx950__destroy", "alignment_gfx950__align_batch",
+            "storm_ref_abi_layout"} <= declared
+    exported = set()
+    for n in declared:
+        if n in mapped:
+            exported |= {n + "_nt", n + "_cs"}
+        else:
+            exported.add(n)
+    assert set(mapped.values()) <= exported
+    for n in sorted(exported):
         assert hasattr(so, n), n
 
 
