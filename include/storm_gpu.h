@@ -168,7 +168,8 @@ void storm_align_plan_destroy(storm_align_plan *plan);
  * n_rec records the caller holds in device memory, d_rec_rs / d_rec_diag with the meaning of
  * storm_align_plan_create's rec_rs / rec_diag, against the plan's resident genome and reads (the
  * plan's own records are not used), written to the caller's device buffers d_results (n_rec
- * storm_align_result) and d_tb (n_rec x STORM_TB_BYTES).  The call allocates nothing: the direction
+ * storm_align_result) and d_tb (n_rec x STORM_TB_BYTES, 4-byte aligned: tracebacks are written in
+ * 32-bit words; an unaligned d_tb or d_results is rejected).  The call allocates nothing: the direction
  * words go to a static 16 MiB device array (held by each GPU the library runs on until the process
  * ends), in launches of as many records as it holds (a wavefront of 64 records needs 64 x longest
  * read x ceil((2 dp_band + 1) / 8) x 4 bytes; reads up to 256 codes at any -i fit; more is rejected

@@ -223,12 +223,14 @@ static int hits_to_host(storm_mapper* m, const Slot* slots, const std::vector<in
 // are interleaved) within a 256 MiB direction scratch
 static int64_t align_chunk(int64_t scratch_stride) {
   const int64_t budget = 256LL << 20;
-  return std::max<int64_t>(64, (budget / std::max<int64_t>(scratch_stride, 1)) & ~63LL);
+  return std::max<int64_t>(ALIGN_WAVE_RECORDS,
+                           (budget / std::max<int64_t>(scratch_stride, 1)) & ~(int64_t)(ALIGN_WAVE_RECORDS - 1));
 }
 
 int64_t align_scratch_bytes(int I, int maxL, int64_t n_rec) {
   const int64_t stride = (int64_t)maxL * dp_dir_words(I) * 4;
-  const int64_t chunk_alloc = (std::min(align_chunk(stride), std::max<int64_t>(n_rec, 1)) + 63) & ~63LL;
+  const int64_t chunk_alloc = (std::min(align_chunk(stride), std::max<int64_t>(n_rec, 1)) + ALIGN_WAVE_RECORDS - 1) &
+                              ~(int64_t)(ALIGN_WAVE_RECORDS - 1);
   return chunk_alloc * stride;
 }
 
@@ -775,7 +777,7 @@ int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback,
   FusedState& F = m->fs;
   ScoreDev sc{};
   fill_score(m->p, sc);
-  const int64_t stride = 2 * (int64_t)maxL * dp_dir_words(sc.I) * 4;   // two records per lane (I <= 7)
+  const int64_t stride = (int64_t)maxL * dp_dir_words(sc.I) * 4;   // direction words per lane
   const int bpc = fused_blocks_per_cu(sc.W, sc.I, m->p.color);
   if (bpc <= 0) return fail(STORM_ERR_DEVICE, "fused kernel: no occupancy for this band class");
   const int max_grid = device_cus(m->device) * bpc;

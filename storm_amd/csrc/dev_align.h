@@ -52,6 +52,7 @@ struct DPLane {
   const uint32_t* pl;
   const uint32_t* qp;
   uint32_t rlo, rhi, rq;
+  uint32_t rlo_n, rhi_n, rq_n;   // the next read and quality words, loaded a word ahead
   int has_qual;
   int L, i, win, S, bp;
 
@@ -74,6 +75,9 @@ struct DPLane {
     rlo = pl[0];
     rhi = pl[RW];
     rq = has_qual ? qp[0] : 0xffffffffu;
+    rlo_n = L > 32 ? pl[1] : 0u;
+    rhi_n = L > 32 ? pl[RW + 1] : 0u;
+    rq_n = has_qual && L > 16 ? qp[1] : 0xffffffffu;
     const int NC = k.NC, I = k.I;
     const int r0 = (int)((rlo & 1u) | ((rhi & 1u) << 1)), q0 = (int)(rq & 3u);
     const int s_m = k.match(q0), s_x = k.mismatch(q0);
@@ -132,8 +136,17 @@ struct DPLane {
       gh[NW - 1] = p.y;
       gm[NW - 1] = g.mask[wi + NW - 1];
     }
-    if ((i & 31) == 0) { rlo = pl[i >> 5]; rhi = pl[RW + (i >> 5)]; }
-    if ((i & 15) == 0 && has_qual) rq = qp[i >> 4];
+    // read and quality words: the one loaded a word ahead, and the load of the next (so a row
+    // never waits for memory on them)
+    if ((i & 31) == 0) {
+      rlo = rlo_n;
+      rhi = rhi_n;
+      if (i + 32 < L) { rlo_n = pl[(i >> 5) + 1]; rhi_n = pl[RW + (i >> 5) + 1]; }
+    }
+    if ((i & 15) == 0 && has_qual) {
+      rq = rq_n;
+      if (i + 16 < L) rq_n = qp[(i >> 4) + 1];
+    }
     const int NC = k.NC, I = k.I, I4 = k.I4, gap_open = k.gap_open, gap_ext = k.gap_ext;
     const int ri = (int)(((rlo >> (i & 31)) & 1u) | (((rhi >> (i & 31)) & 1u) << 1));
     const int qi = (int)((rq >> ((i & 15) << 1)) & 3u);
@@ -289,14 +302,93 @@ __device__ __forceinline__ bool dp_bounded_by_filter(const ReadsDev& r, const DP
   return true;
 }
 
+// The traceback walk of one record (alignment.c:642-734) from (L-1, bc) over its direction words
+// (nibble c & 7 of word i * DW + (c >> 3), at tbw[STRIDE * (i * DW + (c >> 3))]: the lanes of a
+// wavefront interleave their words), codes to tb in the reference's packing (bytes past the codes'
+// last 32-bit word are left as they were; the rest of that word is 0).  Each step reads a
+// word the previous step chose, so the rows are loaded RB at a time (one memory latency per RB
+// rows instead of one per step) and the step picks its word from registers; the read and genome
+// words the steps compare are kept in registers too and loaded again only when the walk leaves
+// them (every 32 rows or columns), not at every step.
+template <int IM, bool COLOR, int STRIDE>
+__device__ __forceinline__ void tb_walk(const GenomeDev& g, const uint32_t* pl, int L, int win, int NC, int bc,
+                                        const uint32_t* tbw, uint8_t* tb, AlignRec& o) {
+  constexpr int DW = (2 * IM + 1 + 7) / 8;
+  constexpr int RB = DW <= 2 ? 8 : DW <= 4 ? 4 : 2;
+  uint32_t rows[RB][DW];
+  int lo = L;   // rows lo .. lo + RB - 1 are in `rows`
+  int rw = -1;   // read word rw (codes 32 rw ..) in rl / rh
+  uint32_t rl = 0, rh = 0;
+  int64_t gw = -1;   // genome word gw (codes 32 gw - GPAD ..) in gwp
+  uint2 gwp = make_uint2(0u, 0u);
+  uint32_t* tbw32 = (uint32_t*)tb;   // tb is 4-byte aligned (records of TBB bytes)
+  uint32_t acc = 0;
+  int i = L - 1, c = bc, n = 0, color = 0;
+  while (i >= 0 && c >= 0 && c < NC) {
+    if (i < lo) {   // the RB rows up to row i, loaded together (rows below 0 are not read)
+      lo = i - RB + 1;
+#pragma unroll
+      for (int q = 0; q < RB; ++q)
+#pragma unroll
+        for (int w = 0; w < DW; ++w) rows[q][w] = lo + q >= 0 ? tbw[STRIDE * ((lo + q) * DW + w)] : 0u;
+    }
+    uint32_t word = 0;
+#pragma unroll
+    for (int q = 0; q < RB; ++q)
+#pragma unroll
+      for (int w = 0; w < DW; ++w)
+        if (q == i - lo && w == (c >> 3)) word = rows[q][w];
+    o.best_i0 = (int16_t)i;
+    o.best_j0 = (int16_t)(i + c);
+    const int cell = (int)(word >> ((c & 7) * 4)) & 0xF;
+    const int from = cell & 3;
+    if ((i >> 5) != rw) {
+      rw = i >> 5;
+      rl = pl[rw];
+      rh = pl[RW + rw];
+    }
+    const int ri = (int)(((rl >> (i & 31)) & 1u) | (((rh >> (i & 31)) & 1u) << 1));
+    int code;
+    if (from == 2) code = 0xC | ri;             // insertion in the read
+    else if (from == 1) code = 0x1;             // deletion
+    else {
+      const int64_t b = (int64_t)win + i + c + GPAD;
+      if ((b >> 5) != gw) {
+        gw = b >> 5;
+        gwp = g.lh[gw];
+      }
+      const int gj = (int)(((gwp.x >> (b & 31)) & 1u) | (((gwp.y >> (b & 31)) & 1u) << 1));
+      if (ri != gj) {
+        if (!COLOR) code = 0x4 | ri;
+        else if (color || (i != 0 && (cell & 4))) { code = 0x8 | ri; color ^= (ri ^ gj); }
+        else code = 0x4 | ri;
+      } else code = 0x3;
+    }
+    // TRACEBACK_SEQ_SET (alignment.h:83): code n in byte n / 2, even index in the high nibble;
+    // eight codes make a 32-bit word, stored whole
+    acc |= (uint32_t)code << (((n >> 1) & 3) * 8 + ((n & 1) ? 0 : 4));
+    if ((n & 7) == 7) {
+      tbw32[n >> 3] = acc;
+      acc = 0;
+    }
+    ++n;
+    if (from == 3) { --i; }
+    else if (from == 2) { --i; ++c; }
+    else { --c; }
+  }
+  if (n & 7) tbw32[n >> 3] = acc;   // the last word, its unused nibbles 0
+  o.tb_len = (int16_t)n;
+}
+
 // One candidate: read strand rs (2*read + strand), diagonal h.
 // TB = true: tbw is this lane's direction words (L rows of DPLane::DW words, interleaved with the
 // other lanes of the wavefront) and, when the record can be accepted, the traceback (alignment
 // order, 2 codes per byte) goes to tb.
-// TB = false: the score pass alone, nothing is stored.  A record can only ever be accepted when
-// score > T and S >= T with T >= t (T starts at -t and only rises, run.c:568,683-685), so this
-// pass stops at the first row where the bound drops below t (S < t then holds) and returns
-// score = MIN_SCORE; otherwise it returns the same score, S and best_j as the TB pass.
+// TB = false: the score pass alone, nothing is stored.
+// A record can only ever be accepted when score > T and S >= T with T >= t (T starts at -t and
+// only rises, run.c:568,683-685), so both stop at the first row where the bound drops below t
+// (S < t then holds) and return score = MIN_SCORE; otherwise they return the same score, S and
+// best_j.
 template <int IM, bool COLOR, bool TB = true>
 __device__ AlignRec dp_one(const GenomeDev& g, const ReadsDev& r, const ScoreDev& scd, int rs, int h, uint32_t* tbw,
                            uint8_t* tb) {
@@ -305,44 +397,13 @@ __device__ AlignRec dp_one(const GenomeDev& g, const ReadsDev& r, const ScoreDev
   st.template begin<TB>(g, r, k, rs, h, tbw);
   while (st.more()) {
     st.template row<TB>(g, k, tbw);
-    if (!TB && st.S < k.t) return st.rejected();   // the rest of the rows cannot raise S
+    if (st.S < k.t) return st.rejected();   // the rest of the rows cannot raise S
   }
   int bc;
   AlignRec o = st.result(k, bc);
   // A record can be accepted only if score > T and S >= T with T >= t (run.c:568,675-685).
-  if (TB && bc >= 0 && o.score > k.t && st.S >= k.t) {
-    const int NC = k.NC, win = st.win;
-    const uint32_t* pl = st.pl;
-    int i = st.L - 1, c = bc, n = 0, color = 0;
-    while (i >= 0 && c >= 0 && c < NC) {
-      const int j = i + c;
-      o.best_i0 = (int16_t)i;
-      o.best_j0 = (int16_t)j;
-      constexpr int DW = DPLane<IM, COLOR>::DW;
-      const int cell = (int)(tbw[64 * (i * DW + (c >> 3))] >> ((c & 7) * 4)) & 0xF;
-      const int from = cell & 3;
-      const int ri = rcode(pl, i);
-      int code;
-      if (from == 2) code = 0xC | ri;             // insertion in the read
-      else if (from == 1) code = 0x1;             // deletion
-      else {
-        const int gj = gcode(g, (int64_t)win + j);
-        if (ri != gj) {
-          if (!COLOR) code = 0x4 | ri;
-          else if (color || (i != 0 && (cell & 4))) { code = 0x8 | ri; color ^= (ri ^ gj); }
-          else code = 0x4 | ri;
-        } else code = 0x3;
-      }
-      // TRACEBACK_SEQ_SET (alignment.h:83): even index -> high nibble
-      if (n & 1) tb[n >> 1] = (uint8_t)((tb[n >> 1] & 0xF0) | code);
-      else tb[n >> 1] = (uint8_t)(code << 4);
-      ++n;
-      if (from == 3) { --i; }
-      else if (from == 2) { --i; ++c; }
-      else { --c; }
-    }
-    o.tb_len = (int16_t)n;
-  }
+  if (TB && bc >= 0 && o.score > k.t && st.S >= k.t)
+    tb_walk<IM, COLOR, 64>(g, st.pl, st.L, st.win, k.NC, bc, tbw, tb, o);
   return o;
 }
 

@@ -3,10 +3,12 @@
 // Semantics: alignment__align + alignment__traceback (alignment.c:326-360, :488-632, :642-734)
 // on the window max(h - I, -d) of length L + 2I (run.c:667, :855), nucleotide or color space.
 // One thread owns one candidate that passed the filter; the 2I+1 band columns of a row live in
-// registers (template bound IM on I).  A score pass decides whether the record can ever be
-// accepted (score > t and S >= t); only those records are aligned again with the per-cell
-// directions in scratch words (4 bits per cell, interleaved across the lanes of a wavefront),
-// which the same thread walks back.
+// registers (template bound IM on I).  One pass computes the rows, stores each row's directions in
+// scratch words (4 bits per cell, interleaved across the lanes of a wavefront) and stops at the
+// first row where the record can no longer be accepted; a record that can (score > t and S >= t)
+// is walked back by the same thread.  (Scoring first and aligning only the acceptable records
+// again costs more: the direction stores are cheap next to the cells, and one pass does every
+// cell once.)
 //
 // The kernel is threshold-free: alongside the score it returns the survival bound
 //   S = min over rows i >= 1 of (max_on_line(i) + best_possible(i)),
@@ -18,39 +20,39 @@
 
 namespace storm {
 
+// The record's window and the look-ahead words of its planes lie inside the genome (a record
+// from a caller that the library did not build)
+__device__ __forceinline__ bool record_in_bounds(const AlignArgs& a, int64_t rec) {
+  const int rs = a.rec_rs[rec];
+  if (rs < 0 || rs >= 2 * a.r.n) return false;
+  const int64_t h = a.rec_diag[rec];
+  const int64_t win = (h - a.sc.I > -a.sc.d) ? h - a.sc.I : -a.sc.d;
+  return win + a.r.len[rs >> 1] + 2 * a.sc.I + 96 <= a.g.size + GPAD;
+}
+
+__device__ __forceinline__ AlignRec invalid_record() {
+  AlignRec x;
+  x.score = MIN_SCORE;
+  x.S = MIN_SCORE;
+  x.win = 0;
+  x.best_j = -2;
+  x.best_i0 = x.best_j0 = -1;
+  x.tb_len = 0;
+  return x;
+}
+
 template <int IM, bool COLOR>
 __global__ __launch_bounds__(128) void k_align(AlignArgs a) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= a.n) return;
   const int64_t rec = a.base + k;
-  if (a.check_bounds) {
-    const int rs = a.rec_rs[rec];
-    bool ok = rs >= 0 && rs < 2 * a.r.n;
-    if (ok) {   // window codes win .. win + L + 2I - 1 and the look-ahead words inside the planes
-      const int64_t h = a.rec_diag[rec];
-      const int64_t win = (h - a.sc.I > -a.sc.d) ? h - a.sc.I : -a.sc.d;
-      ok = win + a.r.len[rs >> 1] + 2 * a.sc.I + 96 <= a.g.size + GPAD;
-    }
-    if (!ok) {
-      AlignRec x;
-      x.score = MIN_SCORE;
-      x.S = MIN_SCORE;
-      x.win = 0;
-      x.best_j = -2;
-      x.best_i0 = x.best_j0 = -1;
-      x.tb_len = 0;
-      a.out[rec] = x;
-      return;
-    }
+  if (a.check_bounds && !record_in_bounds(a, rec)) {
+    a.out[rec] = invalid_record();
+    return;
   }
-  // score pass without the direction matrix; the records that can still be accepted (a few per
-  // cent of the filter passes) are aligned again with directions and traceback
-  AlignRec x = dp_one<IM, COLOR, false>(a.g, a.r, a.sc, a.rec_rs[rec], a.rec_diag[rec], nullptr, nullptr);
-  if (x.score > a.sc.t && x.S >= a.sc.t)
-    x = dp_one<IM, COLOR, true>(a.g, a.r, a.sc, a.rec_rs[rec], a.rec_diag[rec],
-                                (uint32_t*)(a.scratch + (k >> 6) * 64 * a.scratch_stride) + (k & 63),
-                                a.tb + rec * TBB);
-  a.out[rec] = x;
+  a.out[rec] = dp_one<IM, COLOR, true>(a.g, a.r, a.sc, a.rec_rs[rec], a.rec_diag[rec],
+                                       (uint32_t*)(a.scratch + (k >> 6) * 64 * a.scratch_stride) + (k & 63),
+                                       a.tb + rec * TBB);
 }
 
 hipError_t launch_align(const AlignArgs& a, hipStream_t st) {

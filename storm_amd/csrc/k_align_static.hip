@@ -60,11 +60,13 @@ extern "C" int storm_align_run_records_noalloc(storm_align_plan* plan, const int
   if (n_rec < 0) return fail(STORM_ERR_INVALID, "negative record count");
   if (n_rec == 0) return STORM_OK;
   if (!d_rec_rs || !d_rec_diag || !d_results || !d_tb) return fail(STORM_ERR_INVALID, "null device buffer");
+  if (((uintptr_t)d_tb & 3) || ((uintptr_t)d_results & 3))   // tracebacks are stored as 32-bit words
+    return fail(STORM_ERR_INVALID, "d_tb and d_results must be 4-byte aligned");
   const int64_t stride = (int64_t)std::max(maxL, 1) * dp_dir_words(a.sc.I) * 4;
-  const int64_t chunk = ((int64_t)ALIGN_STATIC_SCRATCH_BYTES / stride) & ~63LL;
-  if (chunk < 64)
+  const int64_t chunk = ((int64_t)ALIGN_STATIC_SCRATCH_BYTES / stride) & ~(int64_t)(ALIGN_WAVE_RECORDS - 1);
+  if (chunk < ALIGN_WAVE_RECORDS)
     return fail(STORM_ERR_INVALID, "reads of %d codes at -i %d need %lld bytes of direction words per wavefront, "
-                "more than the %zu-byte static scratch", maxL, a.sc.I, (long long)(64 * stride),
+                "more than the %zu-byte static scratch", maxL, a.sc.I, (long long)(ALIGN_WAVE_RECORDS * stride),
                 ALIGN_STATIC_SCRATCH_BYTES);
   HIPOK(hipSetDevice(device));
   uint8_t* sp = nullptr;

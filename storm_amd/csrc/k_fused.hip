@@ -16,7 +16,7 @@
 // record buffer, is flagged in `overflow` and left untouched; the host sends it through the
 // staged path.
 #include "dev_align.h"
-#include "dev_align2.h"
+#include "dev_align.h"
 #include "dev_filter.h"
 #include "dev_seed.h"
 #include "storm_kernels.h"
@@ -31,11 +31,13 @@ constexpr int F_PENDING = -3;        // Slot.pass while the entry's traceback is
 static_assert(TBB % 16 == 0, "traceback records are copied in 16-byte units");
 static_assert(F_CCAP >= 2 * F_HCAP, "a flush must hold both strands of one read");
 
-// Three wavefronts per SIMD for the bands of the two-records-per-lane DP (up to 168 VGPRs: the
-// pair's columns, color states and both genome windows stay in registers; at four the compiler
-// spills them), two for the wide bands.
+// Wavefronts per SIMD: four for bands up to 15 columns in nucleotide mode (at W = 8 the kernel then
+// keeps 128 VGPRs and spills a few bytes, measured 4 % faster on 100-nt reads than three), three
+// for color mode at W = 8 (its color state costs more registers: four measured 2 % slower), two for
+// the wide bands (up to 256 VGPRs).
 template <int W, int IM, bool COLOR>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 3 : 2))) void k_fused(FusedArgs a) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? (COLOR ? 3 : 4) : 2)))
+void k_fused(FusedArgs a) {
   __shared__ int32_t hbuf[F_HCAP];
   __shared__ int32_t lk[192];
   __shared__ int32_t cstart[2 * F_GMAX + 1];
@@ -50,7 +52,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 3 :
   const int I = a.sc.I;
   const FilterConst fk = filter_const(a.sc.fmatch, a.sc.fmismatch, a.sc.fopen, a.sc.fext, a.sc.fthr);
   // direction words of this lane, interleaved with the other lanes' (64 lanes x scratch_stride bytes
-  // per workgroup: two records per lane in the pair pass, one in the single-record pass)
+  // per workgroup)
   uint32_t* tbw = (uint32_t*)(a.scratch + (int64_t)blockIdx.x * 64 * a.scratch_stride) + lane;
   uint8_t* tbbuf = a.tbbuf + (int64_t)blockIdx.x * F_PCAP * TBB;
   uint8_t* wb = a.work + (int64_t)blockIdx.x * FUSED_WORK_BYTES;
@@ -104,11 +106,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 3 :
       __syncthreads();
     }
     if (lane == 0) { stl[0] += (unsigned long long)nc; stl[1] += (unsigned long long)np; }
-    // DP (stage d), score pass: lane l takes records l, l + 64, ...; nothing is stored per cell
-    // and a record stops at the first row where it can no longer be accepted
+    // DP (stage d) over the flush's filter passes
     const int npl = np < F_PCAP ? np : F_PCAP;
-    // the record needs its score pass (0), is skipped (1: its filter score bounds its DP score at
-    // or below t, so it is written as never accepted), or belongs to an overflowing read (2)
+    // the record runs (0), is skipped (1: its filter score bounds its DP score at or below t, so
+    // it is written as never accepted), or belongs to an overflowing read (2)
     auto triage = [&](int k, int& rs, int& h) -> int {
       const int c = rec2cand[k];
       if (rovf[crs[c] >> 1]) return 2;
@@ -127,44 +128,31 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 3 :
       }
       return 0;
     };
-    if constexpr (IM <= 7) {
-      // two records per lane, one per 16-bit half (records 2l, 2l+1, then 2l+128, ...), directions
-      // stored as they are computed and the traceback of a record that can be accepted walked at
-      // once, so no record is aligned twice
-      // the records that run, compacted in record order into hbuf (free during a flush), so that
-      // skipped and overflowing records leave no lane idle
-      int nrun = 0;
-      for (int b0 = 0; b0 < npl; b0 += 64) {
-        const int k = b0 + lane;
-        int rs = 0, h = 0, r = 0;
-        if (k < npl) r = triage(k, rs, h) == 0;
-        const int incl = wave_scan_incl_i32(r);
-        if (r) hbuf[nrun + incl - 1] = k;
-        nrun += __shfl(incl, 63, 64);
-      }
-      __syncthreads();
-      for (int j0 = 2 * lane; j0 < nrun; j0 += 128) {
-        const int k0 = hbuf[j0];
-        const bool two = j0 + 1 < nrun;   // a lone record runs in both halves; the copy writes nothing
-        const int k1 = two ? hbuf[j0 + 1] : k0;
-        const int c0 = rec2cand[k0], c1 = rec2cand[k1];
-        AlignRec x0, x1;
-        dp_score_pair<IM, COLOR, true>(a.g, a.r, a.sc, (int)(2 * base + crs[c0]), cdiag[c0], (int)(2 * base + crs[c1]),
-                                       cdiag[c1], x0, x1, tbw, tbbuf + (int64_t)k0 * TBB,
-                                       two ? tbbuf + (int64_t)k1 * TBB : nullptr);
-        recs[k0] = x0;
-        if (two) recs[k1] = x1;
-      }
-      __syncthreads();   // hbuf is rewritten below
-    } else {
-      for (int k = lane; k < npl; k += 64) {
-        int rs = 0, h = 0;
-        if (triage(k, rs, h) == 0) recs[k] = dp_one<IM, COLOR, false>(a.g, a.r, a.sc, rs, h, nullptr, nullptr);
-      }
+    // the records that run, compacted in record order into hbuf (free during a flush), so that
+    // skipped and overflowing records leave no lane idle
+    int nrun = 0;
+    for (int b0 = 0; b0 < npl; b0 += 64) {
+      const int k = b0 + lane;
+      int rs = 0, h = 0, r = 0;
+      if (k < npl) r = triage(k, rs, h) == 0;
+      const int incl = wave_scan_incl_i32(r);
+      if (r) hbuf[nrun + incl - 1] = k;
+      nrun += __shfl(incl, 63, 64);
     }
     __syncthreads();
+    // One pass per record, lane l taking the running records hbuf[l], hbuf[l + 64], ...: the rows'
+    // directions are stored as they are computed (the lane's scratch words), a record stops at the
+    // first row where it can no longer be accepted, and one that can (score > t, S >= t) is walked
+    // back at once (codes to tbbuf), so no record is aligned twice.
+    for (int j = lane; j < nrun; j += 64) {
+      const int k = hbuf[j];
+      const int c = rec2cand[k];
+      recs[k] = dp_one<IM, COLOR, true>(a.g, a.r, a.sc, (int)(2 * base + crs[c]), cdiag[c], tbw,
+                                        tbbuf + (int64_t)k * TBB);
+    }
+    __syncthreads();   // hbuf is rewritten below
     // the records that can still be accepted (score > t, S >= t), ascending, as
-    // (read strand << 16) | record in the hit buffer (free during a flush)
+    // (read strand << 16) | record in the hit buffer
     int nv = 0;
     for (int b0 = 0; b0 < npl; b0 += 64) {
       const int k = b0 + lane;
@@ -179,16 +167,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? 3 :
       const int incl = wave_scan_incl_i32(v);
       if (v) hbuf[nv + incl - 1] = (j << 16) | k;
       nv += __shfl(incl, 63, 64);
-    }
-    __syncthreads();
-    // traceback pass over those records only (bands past 2IM+1 = 15 columns; the pair pass above
-    // has done it for narrower ones): directions in the wavefront's scratch words, codes to tbbuf
-    if constexpr (IM > 7)
-    for (int e = lane; e < nv; e += 64) {
-      const int k = hbuf[e] & 0xffff;
-      const int c = rec2cand[k];
-      const int64_t rs = 2 * base + crs[c];
-      recs[k] = dp_one<IM, COLOR, true>(a.g, a.r, a.sc, (int)rs, cdiag[c], tbw, tbbuf + (int64_t)k * TBB);
     }
     __syncthreads();
     // hit map replay (run.c:660-697, hit_map.c:50-144): one lane per read

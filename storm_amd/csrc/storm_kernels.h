@@ -137,7 +137,7 @@ struct FusedArgs {
   uint8_t* cells;            // 4 * TBB bytes per read: the traceback cells
   uint8_t* overflow;         // per read
   uint8_t* scratch;          // per workgroup: 64 * scratch_stride bytes of direction words
-  int64_t scratch_stride;    // bytes per lane: 2 x maxL x dp_dir_words(I) x 4
+  int64_t scratch_stride;    // bytes per lane: maxL x dp_dir_words(I) x 4
   uint8_t* tbbuf;            // per workgroup: FUSED_PCAP * TBB traceback bytes of the current flush
   uint8_t* work;             // per workgroup: FUSED_WORK_BYTES of candidates and records
   int ref_id;
@@ -181,6 +181,8 @@ hipError_t launch_serve_scatter(const Slot* slots, const ServeRow* rows, int64_t
 // The DP kernels are compiled for band bounds IM = 1, 3, 7, 15, 31 (2IM + 1 columns); -i I uses the
 // smallest IM >= I.  Traceback directions are 4 bits per cell, 8 cells per 32-bit word.
 __host__ __device__ inline int dp_band_bound(int I) { return I <= 1 ? 1 : I <= 3 ? 3 : I <= 7 ? 7 : I <= 15 ? 15 : 31; }
+// records whose direction words one wavefront of k_align interleaves
+constexpr int ALIGN_WAVE_RECORDS = 64;
 __host__ __device__ inline int dp_dir_words(int I) { return (2 * dp_band_bound(I) + 1 + 7) / 8; }
 
 // FASTQ reads parsed on the device (k_ingest.hip)

@@ -121,9 +121,11 @@ def _case(seed, color, I, t, with_qual, n_reads=160, G=12000):
         np.array(rs, np.int32), np.array(diags, np.int32)
 
 
-def _oracle_records(o, genome, mask, reads, quals, lens, fq, rs, diags, color, I, t):
+def _oracle_records(o, genome, mask, reads, quals, lens, fq, rs, diags, color, I, t, scores=None):
     op = OrcParams()
     o.orc_params_default(C.byref(op), color)
+    if scores:
+        op.match, op.mismatch, op.gap_open, op.gap_extend = scores
     op.allowed_indels = I
     op.simd_diags = min(I, 15)
     op.t = t
@@ -172,13 +174,26 @@ CASES = [(0, 0, 115, True), (0, 1, 40, True), (0, 3, 115, True), (0, 3, 60, Fals
          (1, 31, 60, True)]
 
 
+# (color, I, t, scores): score sets far from the reference's defaults (larger steps per cell, a
+# color refund of twice a large mismatch, gap open equal to the match)
+SCORE_CASES = [(0, 3, 200, (20, -12, -25, -6)), (1, 7, 150, (10, -12, -14, -5)), (0, 1, 60, (19, -3, -5, -2)),
+               (1, 3, 90, (18, -9, -18, -18))]
+
+
+@pytest.mark.parametrize("color,I,t,scores", SCORE_CASES, ids=lambda v: str(v))
+def test_align_batch_scores_equal_oracle(color, I, t, scores, oracle_built):
+    test_align_batch_equals_oracle(color, I, t, True, oracle_built, scores)
+
+
 @pytest.mark.parametrize("color,I,t,with_qual", CASES, ids=lambda v: str(v))
-def test_align_batch_equals_oracle(color, I, t, with_qual, oracle_built):
+def test_align_batch_equals_oracle(color, I, t, with_qual, oracle_built, scores=None):
     o = _oracle()
-    seed = 1000 * color + 10 * I + (t % 7) + (0 if with_qual else 5)
+    seed = 1000 * color + 10 * I + (t % 7) + (0 if with_qual else 5) + (sum(scores) if scores else 0)
     genome, mask, reads, quals, lens, fq, rs, diags = _case(seed, color, I, t, with_qual)
     p = lib.params(color, allowed_indels=I, t=t)
-    print(f"align batch color={color} -i {I} -t {t} qual={with_qual}: {len(rs)} records", flush=True)
+    if scores:
+        p.match, p.mismatch, p.gap_open, p.gap_extend = scores
+    print(f"align batch color={color} -i {I} -t {t} qual={with_qual} scores={scores}: {len(rs)} records", flush=True)
     plan = lib.AlignPlan(genome, mask, reads, quals, lens, rs, diags, p, first_qual=fq if color else None)
     plan.run(1)
     res, tb = plan.fetch()
@@ -186,7 +201,7 @@ def test_align_batch_equals_oracle(color, I, t, with_qual, oracle_built):
     res2, tb2 = plan.fetch()
     plan.close()
     assert np.array_equal(res, res2)
-    want = _oracle_records(o, genome, mask, reads, quals, lens, fq, rs, diags, color, I, t)
+    want = _oracle_records(o, genome, mask, reads, quals, lens, fq, rs, diags, color, I, t, scores)
     n_tb = n_exit = 0
     for k, (sc, S, full, win, bj, bi0, bj0, otb, otl) in enumerate(want):
         g = res[k]
