@@ -142,6 +142,71 @@ void length_order(const storm_reads_view* v, std::vector<int64_t>& order, bool& 
 
 // Uploads the view's reads and arranges them on the device in order[0..) (a permutation of the
 // view).  The caller's arrays go to the device as they are; a reordering is a device gather.
+// memcpy on up to 8 host threads, 1 MiB or more each (a copy between pinned memory and the
+// caller's pageable memory runs at the host's memory bandwidth only with several threads)
+static void host_copy(void* dst, const void* src, size_t bytes) {
+  const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, bytes >> 20));
+  if (nt <= 1) {
+    memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t per = (bytes + nt - 1) / nt;
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t) {
+    const size_t off = (size_t)t * per;
+    if (off >= bytes) break;
+    th.emplace_back([=] { memcpy((char*)dst + off, (const char*)src + off, std::min(per, bytes - off)); });
+  }
+  for (auto& x : th) x.join();
+}
+
+struct CopySeg {
+  void* dst;
+  const void* src;
+  size_t bytes;
+};
+
+// Copies between the caller's (pageable) host memory and device memory on the mapper's stream at
+// pinned-copy speed: the segments are cut into pieces of up to 8 MiB that pass through two pinned
+// halves in turn, and the host threads copying one piece overlap the DMA of the next.
+// to_host: device -> host, returns when every dst holds its bytes; else host -> device, returns
+// once every byte is in pinned memory or on the device (the DMA completes in stream order; the
+// halves are not reused before their events say so).
+static int staged_copy(storm_mapper* m, const std::vector<CopySeg>& segs, bool to_host) {
+  const size_t CH = (size_t)8 << 20;
+  HIPOK(m->out_stage.ensure(2 * CH));
+  for (int k = 0; k < 2; ++k)
+    if (!m->stage_ev[k]) HIPOK(hipEventCreateWithFlags(&m->stage_ev[k], hipEventDisableTiming));
+  struct Piece {
+    char* dst;
+    const char* src;
+    size_t len;
+  };
+  std::vector<Piece> ps;
+  for (const CopySeg& g : segs)
+    for (size_t off = 0; off < g.bytes; off += CH)
+      ps.push_back({(char*)g.dst + off, (const char*)g.src + off, std::min(CH, g.bytes - off)});
+  for (size_t c = 0; c <= ps.size(); ++c) {
+    if (c < ps.size()) {
+      char* half = (char*)m->out_stage.p + (c & 1) * CH;
+      if (to_host) {   // piece c into half c & 1 (its previous piece, c - 2, was copied out already)
+        HIPOK(hipMemcpyAsync(half, ps[c].src, ps[c].len, hipMemcpyDeviceToHost, m->st));
+      } else {         // the half's previous DMA has read it; fill it and send it
+        HIPOK(hipEventSynchronize(m->stage_ev[c & 1]));
+        host_copy(half, ps[c].src, ps[c].len);
+        HIPOK(hipMemcpyAsync(ps[c].dst, half, ps[c].len, hipMemcpyHostToDevice, m->st));
+      }
+      HIPOK(hipEventRecord(m->stage_ev[c & 1], m->st));
+    }
+    if (to_host && c >= 1) {   // piece c - 1 out of its half while piece c is in flight
+      const size_t q = c - 1;
+      HIPOK(hipEventSynchronize(m->stage_ev[q & 1]));
+      host_copy(ps[q].dst, (const char*)m->out_stage.p + (q & 1) * CH, ps[q].len);
+    }
+  }
+  return STORM_OK;
+}
+
 int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order, DevReads& d) {
   const int64_t n = v->n_reads;
   if ((int64_t)order.size() != n) return fail(STORM_ERR_INVALID, "read order does not cover the view");
@@ -170,10 +235,11 @@ int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<i
     if (qs) HIPOK(src.qual.ensure((size_t)(n * qs)));
     if (has_fq) HIPOK(src.first_qual.ensure((size_t)n));
   }
-  HIPOK(hipMemcpyAsync(src.codes.p, v->codes, (size_t)(n * cs), hipMemcpyHostToDevice, m->st));
-  HIPOK(hipMemcpyAsync(src.len.p, v->len, (size_t)n * 4, hipMemcpyHostToDevice, m->st));
-  if (qs) HIPOK(hipMemcpyAsync(src.qual.p, v->qual, (size_t)(n * qs), hipMemcpyHostToDevice, m->st));
-  if (has_fq) HIPOK(hipMemcpyAsync(src.first_qual.p, v->first_qual, (size_t)n, hipMemcpyHostToDevice, m->st));
+  std::vector<CopySeg> up = {{src.codes.p, v->codes, (size_t)(n * cs)}, {src.len.p, v->len, (size_t)n * 4}};
+  if (qs) up.push_back({src.qual.p, v->qual, (size_t)(n * qs)});
+  if (has_fq) up.push_back({src.first_qual.p, v->first_qual, (size_t)n});
+  int rc = staged_copy(m, up, false);
+  if (rc) return rc;
   if (!identity) {
     std::vector<int32_t> o32((size_t)n);
     for (int64_t i = 0; i < n; ++i) o32[(size_t)i] = (int32_t)order[(size_t)i];
@@ -209,11 +275,12 @@ static int hits_to_host(storm_mapper* m, const Slot* slots, const std::vector<in
   int32_t* ref_host = ho.ref_id;
   if (!ref_host) { ref_tmp.resize((size_t)(4 * n)); ref_host = ref_tmp.data(); }
   const size_t bytes = (size_t)(4 * n) * 4;
-  if (ho.score) HIPOK(hipMemcpyAsync(ho.score, hd, bytes, hipMemcpyDeviceToHost, m->st));
-  HIPOK(hipMemcpyAsync(ref_host, hd + 4 * n, bytes, hipMemcpyDeviceToHost, m->st));
-  if (ho.ref_start) HIPOK(hipMemcpyAsync(ho.ref_start, hd + 8 * n, bytes, hipMemcpyDeviceToHost, m->st));
-  if (ho.sense) HIPOK(hipMemcpyAsync(ho.sense, hd + 12 * n, bytes, hipMemcpyDeviceToHost, m->st));
-  HIPOK(hipStreamSynchronize(m->st));
+  std::vector<CopySeg> out = {{ref_host, hd + 4 * n, bytes}};
+  if (ho.score) out.push_back({ho.score, hd, bytes});
+  if (ho.ref_start) out.push_back({ho.ref_start, hd + 8 * n, bytes});
+  if (ho.sense) out.push_back({ho.sense, hd + 12 * n, bytes});
+  const int rc = staged_copy(m, out, true);
+  if (rc) return rc;
   mapped = 0;
   for (int64_t i = 0; i < n; ++i) mapped += ref_host[(size_t)(4 * i)] != -1;
   return STORM_OK;
@@ -540,9 +607,10 @@ int finish_view(storm_mapper* m, const Slot* slots, const uint8_t* cells, const 
     hits = hv.data();
     if (want_tb) { tv.resize((size_t)(n * M) * TBB); tb = tv.data(); }
   }
-  HIPOK(hipMemcpyAsync(hits, m->rec_dev.p, (size_t)(4 * n) * sizeof(HitRec), hipMemcpyDeviceToHost, m->st));
-  if (want_tb) HIPOK(hipMemcpyAsync(tb, m->tb_dev.p, (size_t)(n * M) * TBB, hipMemcpyDeviceToHost, m->st));
-  HIPOK(hipStreamSynchronize(m->st));
+  std::vector<CopySeg> out = {{hits, m->rec_dev.p, (size_t)(4 * n) * sizeof(HitRec)}};
+  if (want_tb) out.push_back({tb, m->tb_dev.p, (size_t)(n * M) * TBB});
+  const int rc = staged_copy(m, out, true);
+  if (rc) return rc;
   mapped = 0;
   for (int64_t i = 0; i < n; ++i) mapped += hits[4 * i].ref_id != -1;
   if (ro) return STORM_OK;
@@ -1010,6 +1078,8 @@ void storm_mapper_destroy(storm_mapper* m) {
   (void)hipSetDevice(m->device);
   m->fs.drop_graphs();
   m->ps = PassScratch();
+  for (hipEvent_t& e : m->stage_ev)
+    if (e) (void)hipEventDestroy(e);
   m->res.reset();
   if (m->st) (void)hipStreamDestroy(m->st);
   delete m;

@@ -235,6 +235,8 @@ struct storm_mapper {
   storm::DevMem sam_names, sam_name_off, sam_no_qual, sam_lut, sam_inv, sam_lens, sam_offs, sam_text, sam_mapped;
   int sam_lut_dm = 0;
   storm::PinnedMem sam_host;
+  storm::PinnedMem out_stage;          // two pinned halves the results pass through to the caller
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};
   double finish_seconds = 0;   // time spent in a map call's finish step (SAM formatting)
   storm::DevReads sub_reads;   // the batched path's staged fallback: the handed-back reads' planes,
   storm::DevMem sub_ids, sub_slots;   // their ids in the batch and their slots
