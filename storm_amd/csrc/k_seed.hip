@@ -65,18 +65,6 @@ __global__ __launch_bounds__(256) void k_prep_reads(PrepArgs a) {
 }
 
 // --- seeds -----------------------------------------------------------------------------------
-__device__ __forceinline__ int seed_key(const SeedDev& s, const uint32_t* pl, int rp) {
-  int key = 0;
-  for (int k = 0; k < s.weight; ++k) key = (key << 2) | rcode(pl, rp + s.offs[k]);
-  return key;
-}
-
-__device__ __forceinline__ int seed_npos(const SeedDev& s, int L) {
-  if (s.n_pos > 0) return s.n_pos;
-  const int n = L - s.span + 1;
-  return n > 0 ? n : 0;
-}
-
 // wave-wide inclusive scan (64 lanes)
 __device__ __forceinline__ int64_t wave_scan_incl(int64_t v) {
   const int lane = threadIdx.x & 63;

@@ -95,6 +95,35 @@ __device__ __forceinline__ int rcode(const uint32_t* pl, int i) {
   return (int)(((pl[i >> 5] >> (i & 31)) & 1u) | (((pl[RW + (i >> 5)] >> (i & 31)) & 1u) << 1));
 }
 
+// The key of seed s at read position rp (seed.c:203-214): the codes at rp + offs[k], two bits
+// each, first symbol highest.  A seed spanning at most 32 codes reads one 32-bit window per read
+// plane (two words each, joined by alignbit) and takes its codes from registers; a longer one
+// reads every code from the planes.
+__device__ __forceinline__ int seed_key(const SeedDev& s, const uint32_t* pl, int rp) {
+  int key = 0;
+  const int w = rp >> 5, sh = rp & 31;
+  if (s.span <= 32 && (w + 1 < RW || sh + s.span <= 32)) {
+    // the second word: w + 1, or (last word of the plane, window inside it) w again, so no word
+    // outside the read's planes is read
+    const int w1 = w + 1 < RW ? w + 1 : w;
+    const uint32_t lo = __builtin_amdgcn_alignbit(pl[w1], pl[w], sh);
+    const uint32_t hi = __builtin_amdgcn_alignbit(pl[RW + w1], pl[RW + w], sh);
+    for (int k = 0; k < s.weight; ++k) {
+      const int o = s.offs[k];
+      key = (key << 2) | (int)(((lo >> o) & 1u) | (((hi >> o) & 1u) << 1));
+    }
+    return key;
+  }
+  for (int k = 0; k < s.weight; ++k) key = (key << 2) | rcode(pl, rp + s.offs[k]);
+  return key;
+}
+
+__device__ __forceinline__ int seed_npos(const SeedDev& s, int L) {
+  if (s.n_pos > 0) return s.n_pos;
+  const int n = L - s.span + 1;
+  return n > 0 ? n : 0;
+}
+
 __device__ __forceinline__ int rqual(const ReadsDev& r, int64_t rs, int i) {
   if (!r.has_qual) return 3;   // alignment.c:151-162: no quality -> level of Q93
   return (int)((r.qual[rs * QW + (i >> 4)] >> ((i & 15) << 1)) & 3u);
