@@ -112,6 +112,10 @@ __device__ int wave_sort_keep(const int32_t* buf, int n, int I, int32_t* out, in
 // consecutive entries of a bucket.  Returns the number of candidates, or -1 when the raw hits
 // exceed hcap or the candidates exceed `cap` (the caller sends the read to the staged path, which
 // has no such limits).  *raw_hits receives the bucket entries read (the lookups' total count).
+// REG2048: sort up to 2048 raw hits in registers (32 per lane) rather than in LDS, five times
+// fewer instructions for reads with many hits (repeats), but it raises the caller's register
+// count; the callers whose occupancy it would lower leave it off.
+template <bool REG2048>
 __device__ int wave_candidates(const ReadsDev& r, const SeedSetDev& seeds, int64_t rs, int I, int32_t* hbuf, int hcap,
                                int32_t* lk, int32_t* out, int cap, int* raw_hits) {
   const int lane = threadIdx.x & 63;
@@ -194,6 +198,7 @@ __device__ int wave_candidates(const ReadsDev& r, const SeedSetDev& seeds, int64
   if (run <= 256) return wave_sort_keep<4>(hbuf, run, I, out, cap);
   if (run <= 512) return wave_sort_keep<8>(hbuf, run, I, out, cap);
   if (run <= 1024) return wave_sort_keep<16>(hbuf, run, I, out, cap);
+  if (REG2048 && run <= 2048) return wave_sort_keep<32>(hbuf, run, I, out, cap);
   wave_bitonic_sort(hbuf, run);
   int placed = 0;
   for (int e0 = 0; e0 < run; e0 += 64) {
