@@ -635,6 +635,21 @@ def config5(lib, g, gpath, n_req=40, load=0.8, lanes=3, max_batch=1 << 18):
                       "sam_bytes": int(sum(x[2] for x in dig_b))}
     log(f"config5: batched {total / wall_b:.0f} reads/s, p50 {np.percentile(lat_b, 50):.3f}s "
         f"p99 {np.percentile(lat_b, 99):.3f}s")
+    # the same arrivals on a batched server that serves requests in admission order
+    srv = lib.Server(gpath, lanes=lanes, max_batch_reads=max_batch, schedule=lib.STORM_SCHED_FIFO)
+    for _ in range(lanes):
+        for rid in [srv.submit(pr, **kw) for pr, kw in warm]:
+            srv.wait_sam(rid)
+    lat_f, wall_f, _, dig_f = serve_run(srv, reqs, arrivals)
+    srv.close()
+    res["batched"]["schedule"] = "deadline = submit + 4 x estimated remaining device time (STORM_SCHED_SIZE)"
+    res["batched_fifo"] = {"reads_per_s": round(total / wall_f, 1), "wall_s": round(wall_f, 3),
+                           "p50_latency_s": round(float(np.percentile(lat_f, 50)), 4),
+                           "p99_latency_s": round(float(np.percentile(lat_f, 99)), 4),
+                           "schedule": "admission order (STORM_SCHED_FIFO)",
+                           "sam_equals_batched": all(a[1] == b[1] for a, b in zip(dig_f, dig_b))}
+    log(f"config5: batched FIFO {total / wall_f:.0f} reads/s, p50 {np.percentile(lat_f, 50):.3f}s "
+        f"p99 {np.percentile(lat_f, 99):.3f}s")
     t0 = time.perf_counter()
     srv = lib.Server(gpath, mode=lib.STORM_MODE_PLAIN)
     log(f"config5: plain server created in {time.perf_counter() - t0:.1f}s")

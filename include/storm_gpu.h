@@ -329,7 +329,16 @@ typedef struct storm_server_config {
   int ignore_lowercase;       /* -j */
   int sam_records;            /* 1 (default): each request also gets its -M SAM records (hit_map.c:325-504,
                                  formatted on the device with its batches); 0: hits only */
+  int schedule;               /* which admitted request the next batch serves first:
+                                 STORM_SCHED_SIZE (default): the earliest of submit time + 4 x the
+                                 request's estimated remaining device time (read count x mean length^2
+                                 x filter lanes, priced by the measured time of earlier batches), so
+                                 a small request overtakes a large one and a large one is overtaken
+                                 only by requests whose own estimate puts them first;
+                                 STORM_SCHED_FIFO: admission order.  Results do not depend on it. */
 } storm_server_config;
+#define STORM_SCHED_FIFO 0
+#define STORM_SCHED_SIZE 1
 
 typedef struct storm_server_stats {
   int64_t submitted, completed, failed, reads_done;

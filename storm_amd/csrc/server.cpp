@@ -9,8 +9,12 @@
 // `lanes` worker threads each own a stream and the buffers of one batch (a storm_mapper over the
 // shared resident genome).  A lane that becomes free admits what fits, then takes the next batch:
 // up to max_batch_reads reads, all of one parameter class (encoding, -i, -z, -t, scores, strands),
-// drawn in admission order from every admitted request of that class that still has unbatched
-// reads -- the tail of one request and the head of the next share a launch.  The batch's reads
+// drawn from every admitted request of that class that still has unbatched reads -- the tail of
+// one request and the head of the next share a launch.  Requests are served in order of a
+// deadline, submit time + 4 x their estimated remaining device time (read count x mean length^2 x
+// filter lanes, priced by the measured time of earlier batches): a small request does not wait
+// behind a large one, and a large one is only passed by requests that the same rule puts first,
+// so it is never starved (or in admission order, STORM_SCHED_FIFO).  The batch's reads
 // are gathered from their regions on the device, sorted by length, mapped by the persistent
 // fused kernel (its overflow reads by the staged kernels), and the slots scattered back to each
 // request's region.  Lanes run concurrently, so batches of different requests and classes
@@ -129,6 +133,7 @@ struct Request {
   hipEvent_t ready = nullptr;             // recorded after the region's upload on the admitting lane
   int64_t next_row = 0;                   // reads not yet given to a batch
   int64_t rows_done = 0;
+  double units_per_read = 0;              // work estimate: mean length^2 x filter lanes
   int batches = 0;
   double t_submit = 0, t_admit = 0, t_first = 0, t_done = 0;
   bool done = false;
@@ -180,6 +185,10 @@ struct storm_server {
   int busy = 0;
   storm_server_stats st{};
   double t0 = 0;
+  // device seconds per work unit (a read of length L in a band of W filter lanes is L^2 W units:
+  // its candidates grow with L and each costs L x 2W filter cells), from the batches run so far
+  // (their wall time on a lane, so it includes the sharing of the device between lanes)
+  double sec_per_unit = 4e-11;
   struct Lane {
     std::unique_ptr<storm_mapper> m;
     std::thread th;
@@ -470,10 +479,23 @@ static void form_batch(storm_server* s, const std::vector<Request*>& own_uploads
     head->t_first = now;
     return;
   }
+  // the requests a batch may draw from, in the order it serves them: admission order (FIFO), or
+  // by deadline = submit time + 4 x estimated remaining device time (STORM_SCHED_SIZE)
+  std::vector<Request*> cand;
+  for (Request* r : s->active)
+    if (r->next_row < r->n && r->status == STORM_OK && ready(r)) cand.push_back(r);
+  if (s->cfg.schedule == STORM_SCHED_SIZE) {
+    auto deadline = [&](const Request* r) {
+      return r->t_submit + 4.0 * (double)(r->n - r->next_row) * r->units_per_read * s->sec_per_unit;
+    };
+    std::stable_sort(cand.begin(), cand.end(),
+                     [&](const Request* a, const Request* b) { return deadline(a) < deadline(b); });
+    head = cand.front();
+  }
   int64_t room = s->cfg.max_batch_reads;
-  for (Request* r : s->active) {
+  for (Request* r : cand) {
     if (room == 0) break;
-    if (r->next_row >= r->n || r->status != STORM_OK || !ready(r) || !(r->cls == head->cls)) continue;
+    if (!(r->cls == head->cls)) continue;
     const int64_t k = std::min(room, r->n - r->next_row);
     segs.push_back(Segment{r, r->next_row, r->next_row + k});
     if (r->next_row == 0) r->t_first = now;
@@ -481,6 +503,13 @@ static void form_batch(storm_server* s, const std::vector<Request*>& own_uploads
     room -= k;
     if (!s->cfg.cross_request) break;
   }
+}
+
+// under s->mu: the work units of a batch's segments
+static double batch_units(const std::vector<Segment>& segs) {
+  double u = 0;
+  for (const Segment& g : segs) u += (double)(g.row1 - g.row0) * g.r->units_per_read;
+  return u;
 }
 
 static bool has_work(storm_server* s) {
@@ -575,7 +604,10 @@ static void lane_loop(storm_server* s, storm_server::Lane* ln) {
       std::lock_guard<std::mutex> lk(s->mu);
       s->busy--;
       s->st.batches++;
-      s->st.gpu_busy_seconds += now_s() - tb;
+      const double dt = now_s() - tb, units = batch_units(run);
+      s->st.gpu_busy_seconds += dt;
+      if (rc == STORM_OK && units > 0 && s->cfg.mode != STORM_MODE_PLAIN)   // running price of a work unit
+        s->sec_per_unit = 0.7 * s->sec_per_unit + 0.3 * (dt / units);
       int64_t nb = 0;
       std::vector<Request*> seen;
       for (Segment& g : segs) {
@@ -613,6 +645,7 @@ void storm_server_config_default(storm_server_config* c) {
   c->encodings = 3;
   c->stdev_distance = 100;
   c->sam_records = 1;
+  c->schedule = STORM_SCHED_SIZE;
 }
 
 int storm_server_create(int device, const char* genome_path, const storm_server_config* cfg, storm_server** out) {
@@ -623,6 +656,8 @@ int storm_server_create(int device, const char* genome_path, const storm_server_
     return fail(STORM_ERR_INVALID, "max_batch_reads must be in 1..2^24");
   if (cfg->mode != STORM_MODE_PLAIN && cfg->mode != STORM_MODE_BATCHED) return fail(STORM_ERR_INVALID, "bad mode");
   if (cfg->encodings < 1 || cfg->encodings > 3) return fail(STORM_ERR_INVALID, "encodings must be 1, 2 or 3");
+  if (cfg->schedule != STORM_SCHED_FIFO && cfg->schedule != STORM_SCHED_SIZE)
+    return fail(STORM_ERR_INVALID, "schedule must be STORM_SCHED_FIFO or STORM_SCHED_SIZE");
   std::unique_ptr<storm_server> s(new storm_server());
   s->device = device;
   s->cfg = *cfg;
@@ -713,7 +748,9 @@ int storm_server_submit(storm_server* s, const storm_reads_view* v, const storm_
     if (L <= 0 || L > MAXL) return fail(STORM_ERR_INVALID, "read %lld has length %d (1..%d supported)", (long long)i, L, MAXL);
     if (L < wl) return fail(STORM_ERR_INVALID, "read %lld is shorter (%d) than the filter band lanes (%d)", (long long)i, L, wl);
     if (r->cs * 4 < L) return fail(STORM_ERR_INVALID, "code_stride %lld too small for length %d", (long long)r->cs, L);
+    r->units_per_read += (double)L * L;
   }
+  r->units_per_read *= (double)wl / (double)r->n;
   if (s->cfg.sam_records) {
     if (v->names) {
       r->name_off.resize((size_t)r->n + 1);

@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(HERE, "_build", "libstorm_hip.so")
 
 STORM_MODE_PLAIN = 0
 STORM_MODE_BATCHED = 1
+STORM_SCHED_FIFO = 0
+STORM_SCHED_SIZE = 1
 
 
 class StormError(RuntimeError):
@@ -52,7 +54,8 @@ class ServerConfig(C.Structure):
     _fields_ = [("mode", C.c_int), ("lanes", C.c_int), ("cross_request", C.c_int), ("max_batch_reads", C.c_int64),
                 ("hbm_budget_bytes", C.c_int64), ("hbm_reserve_bytes", C.c_int64), ("encodings", C.c_int),
                 ("seeds_nucleotide", C.c_char_p), ("seeds_color", C.c_char_p), ("stdev_distance", C.c_int),
-                ("ignore_lowercase", C.c_int), ("sam_records", C.c_int)]
+                ("ignore_lowercase", C.c_int), ("sam_records", C.c_int),
+                ("schedule", C.c_int)]
 
 
 class ServerStats(C.Structure):

@@ -249,3 +249,29 @@ def test_failed_upload_fails_only_its_request(serve_case, alone, monkeypatch):
     st = s.stats()
     s.close()
     assert len(failed) == 1 and st["failed"] == 1 and st["completed"] == 5
+
+
+@pytest.mark.parametrize("schedule", ["size", "fifo"])
+def test_schedule_order(serve_case, alone, schedule):
+    """A small request submitted just after a large one of the same class (one lane, 1000-read
+    batches, the large one needs 9): under STORM_SCHED_SIZE it completes first, under
+    STORM_SCHED_FIFO after the large one; either way every read gets the slots it gets alone."""
+    from storm_amd import lib
+    _, gpath, _, reqs = serve_case
+    big, kw = reqs[3]
+    assert len(big.len) == 9000
+    sched = lib.STORM_SCHED_SIZE if schedule == "size" else lib.STORM_SCHED_FIFO
+    s = lib.Server(gpath, lanes=1, max_batch_reads=1000, schedule=sched)
+    print(f"server: schedule {schedule}, 9000-read request then a 100-read one", flush=True)
+    ib = s.submit(big, **kw)
+    ism = s.submit(big, 0, 100, **kw)
+    hb, rb = s.wait(ib)
+    hs, rs = s.wait(ism)
+    s.close()
+    assert rb["status"] == 0 and rs["status"] == 0
+    assert _same(hb, alone[3][0])
+    assert _same(hs, {f: alone[3][0][f][:100] for f in hs})
+    if schedule == "size":
+        assert rs["t_done"] < rb["t_done"], (rs, rb)
+    else:
+        assert rs["t_done"] >= rb["t_done"], (rs, rb)
