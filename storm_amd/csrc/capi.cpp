@@ -746,6 +746,7 @@ static FusedArgs fused_args(storm_mapper* m, size_t ref, int group) {
   a.ref_id = (int)ref;
   a.sense_mask = m->p.sense_mask;
   a.stats = F.stats.as<unsigned long long>();
+  a.phase = F.phase.as<unsigned long long>();
   return a;
 }
 
@@ -852,6 +853,12 @@ int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback,
   int rc = ensure_fused_buffers(m, n, stride, max_grid);
   if (rc) return rc;
   HIPOK(launch_slots_init(F.slots.as<Slot>(), 4 * n, m->st));
+#ifdef STORM_FUSED_PHASES
+  if (!F.phase.p) {
+    HIPOK(F.phase.ensure(FUSED_PHASES * sizeof(unsigned long long)));
+    HIPOK(hipMemsetAsync(F.phase.p, 0, FUSED_PHASES * sizeof(unsigned long long), m->st));
+  }
+#endif
   const int32_t n32 = (int32_t)n;
   const size_t n_refs = m->res->refs.size();
   int pass_id = 0;
@@ -891,6 +898,11 @@ int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback,
       (void)hipGraphDestroy(g);
       F.add_graph(sig, exec);
     }
+#ifdef STORM_FUSED_PHASES
+    unsigned long long ph0[FUSED_PHASES], ph1[FUSED_PHASES];
+    HIPOK(hipMemcpyAsync(ph0, F.phase.p, sizeof ph0, hipMemcpyDeviceToHost, m->st));
+    HIPOK(hipStreamSynchronize(m->st));
+#endif
     hipEvent_t e0, e1;
     HIPOK(hipEventCreate(&e0));
     HIPOK(hipEventCreate(&e1));
@@ -906,6 +918,17 @@ int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback,
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     ps.ms_filter += ms;   // the fused kernel carries every stage
+#ifdef STORM_FUSED_PHASES
+    {   // wavefront-time shares of the launch's phases (constant 100 MHz clock)
+      HIPOK(hipMemcpy(ph1, F.phase.p, sizeof ph1, hipMemcpyDeviceToHost));
+      double tot = 0, d[FUSED_PHASES];
+      for (int q = 0; q < FUSED_PHASES; ++q) { d[q] = (double)(ph1[q] - ph0[q]); tot += d[q]; }
+      fprintf(stderr, "fused phases W=%d I=%d color=%d reads=%lld grid=%d G=%d launch=%.2f ms wave-s=%.3f: candidates %.1f%%, "
+              "filter %.1f%%, scan+triage %.1f%%, dp+traceback %.1f%%, replay+cells %.1f%%, task %.1f%%\n",
+              sc.W, sc.I, m->p.color, (long long)n, tune.grid, tune.group, ms, tot * 1e-8, 100 * d[0] / tot,
+              100 * d[1] / tot, 100 * d[2] / tot, 100 * d[3] / tot, 100 * d[4] / tot, 100 * d[5] / tot);
+    }
+#endif
     for (int q = 0; q < 4; ++q) totals[q] += st3[q];
     std::vector<int64_t> over;
     for (int64_t i = 0; i < n; ++i) if (ovf[(size_t)i]) over.push_back(i);

@@ -31,6 +31,50 @@ struct DPConst {
   __device__ int mismatch(int q) const { return msel(q & 2, msel(q & 1, px3, px2), msel(q & 1, px1, px0)); }
 };
 
+// Color space: the diagonal move's color state as a table.  A cell's state is its checksum,
+// auto-correction and mismatch freshness (2 bits each, freshness <= 3: it starts at a quality
+// level and only decreases), packed as chk | auto << 2 | fresh << 4 (the cell's meta bits 8-13).
+// Entry [corner state | chk0 << 6 | q << 8] -- chk0 the cell's own read ^ genome color (0 for a
+// masked cell), q its quality level -- holds the state a diagonal move from that corner gives the
+// cell (alignment.c:533-578) and, at bit 6, whether the corner's state turns the cell's mismatch
+// penalty into a refund (alignment.c:465-468: pending checksum, and the composed checksum is the
+// identity or the corner's auto-correction; the caller adds "penalty < 0, cell not masked").
+// The kernels that run color DPs fill it in LDS at start (fill_color_table).
+constexpr int COLOR_TAB = 1024;
+
+__device__ __forceinline__ uint8_t color_diag_entry(int idx) {
+  const int c_chk = idx & 3, c_auto = (idx >> 2) & 3, c_fresh = (idx >> 4) & 3;
+  const int chk0 = (idx >> 6) & 3, q = (idx >> 8) & 3;
+  const int x = c_chk ^ chk0;   // COMPOSE(crt checksum, corner checksum)
+  int chk = x, autoc, fresh;
+  if (chk0 != 0) {              // the corner's checksum changes: a color mismatch in this cell
+    if (c_chk == 0) { autoc = x; fresh = q; }                  // a SNP starts or a read error
+    else if (x == c_auto) { chk = 0; autoc = 0; fresh = 0; }   // the SNP ends
+    else if (x == 0) {
+      if (c_fresh - 1 > 0) { autoc = c_auto; fresh = c_fresh - 1; }
+      else { autoc = 0; fresh = 0; }
+    } else {
+      if (c_fresh - 1 > 0) { autoc = c_auto; fresh = c_fresh - 1; }
+      else { autoc = x; fresh = q; }
+    }
+  } else {                      // match: the pending mismatch wears off
+    autoc = 0;                  // the cell's own (initial) state, chk0 = 0
+    fresh = 0;
+    if (c_auto != 0) {
+      if (c_fresh - 1 > 0 && x != 0) { autoc = c_auto; fresh = c_fresh - 1; }
+    } else {
+      chk = 0;
+    }
+  }
+  const int refund_ok = (c_chk != 0) && (x == 0 || x == c_auto);
+  return (uint8_t)(chk | (autoc << 2) | (fresh << 4) | (refund_ok << 6));
+}
+
+// the table in LDS, filled by the nt threads of a workgroup (the caller synchronises after)
+__device__ __forceinline__ void fill_color_table(uint8_t* tab, int tid, int nt) {
+  for (int e = tid; e < COLOR_TAB; e += nt) tab[e] = color_diag_entry(e);
+}
+
 // The DP of one candidate (read strand rs = 2*read + strand, diagonal h) as a state machine over
 // rows.  The 2I+1 band columns of the current row, the genome planes and the read's codes and
 // quality levels are held in registers: bit c of the row-i genome words is code win + i + c; per
@@ -55,10 +99,13 @@ struct DPLane {
   uint32_t rlo_n, rhi_n, rq_n;   // the next read and quality words, loaded a word ahead
   int has_qual;
   int L, i, win, S, bp;
+  const uint8_t* ctab;   // color mode: the diagonal-move table in LDS (COLOR_TAB entries)
 
   // row 0 (alignment.c:326-360); tbw (TB) receives the row's directions
   template <bool TB>
-  __device__ void begin(const GenomeDev& g, const ReadsDev& r, const DPConst& k, int rs, int h, uint32_t* tbw) {
+  __device__ void begin(const GenomeDev& g, const ReadsDev& r, const DPConst& k, int rs, int h, uint32_t* tbw,
+                        const uint8_t* color_tab) {
+    ctab = color_tab;
     L = r.len[rs >> 1];
     win = (h - k.I > -k.d) ? h - k.I : -k.d;
     pl = read_planes(r, rs);
@@ -182,14 +229,12 @@ struct DPLane {
         // The recurrence in select form (alignment.c:465-604): every lane runs the same
         // instructions whatever its record, with no divergent branches per cell.
         int match_score = cs;
-        // color state of the corner, kept in place in its meta word: chk bits 8-9, auto 10-11,
-        // fresh 12-14; xw = the checksum this cell would carry on a diagonal move, at bits 8-9
-        const int xw = COLOR ? ((cmeta ^ (cchk << 8)) & 0x300) : 0;
+        // color state of the corner, kept in place in its meta word (chk bits 8-9, auto 10-11,
+        // fresh 12-13): the diagonal move's new state and refund from the table
+        const int dtab = COLOR ? (int)ctab[((cmeta >> 8) & 0x3F) | (cchk << 6) | (qi << 8)] : 0;
         const int iauto_w = COLOR ? (cmeta & 0xC00) : 0;
-        const bool ichk0 = COLOR ? ((cmeta & 0x300) == 0) : true;     // no pending checksum
-        const bool x_is_auto = (xw << 2) == iauto_w;                  // corrected by the pending auto
-        if (COLOR) {   // alignment.c:465-468; & and | (not && ||): no divergent branches
-          const bool refund = (!ichk0) & ((xw == 0) | x_is_auto) & (cs != MIN_SCORE) & (cs < 0);
+        if (COLOR) {   // alignment.c:465-468; & (not &&): no divergent branches
+          const bool refund = ((dtab & 0x40) != 0) & (cs != MIN_SCORE) & (cs < 0);
           match_score = msel(refund, -2 * cs, cs);
         }
         const int diag = csc + match_score;
@@ -210,29 +255,22 @@ struct DPLane {
         const int l_lo = (l_lo8 & 0xfc) + 5;
         const int n_lo = msel(take_l, l_lo, msel(take_t, t_lo, d_lo));
         const int nfrom = n_lo & 3;
-        int nhi = 0;   // color state bits 8-14 of the new meta word
+        int nhi = 0;   // color state bits 8-13 of the new meta word
         if (COLOR) {
-          // diagonal move (alignment.c:533-578), the reference's cases in order, each as a whole
-          // state word; K keeps the corner's auto with its freshness decremented
-          const int fw = cmeta & 0x7000;
-          const bool f1 = fw > 0x1000;                                  // ifresh > 1
-          const int K = iauto_w | (fw - 0x1000);
-          const int A1w = xw | (xw << 2) | (qi << 12);                  // (x, x, qi)
-          const bool a2 = (!ichk0) & x_is_auto, a3 = (!ichk0) & (!a2) & (xw == 0);
-          const int m_a = msel(ichk0, A1w, msel(a2, 0, msel(f1, xw | K, msel(a3, 0, A1w))));
-          const bool b1 = iauto_w != 0, b_keep = b1 & f1 & (xw != 0);
-          const int m_b = msel(b1, msel(b_keep, xw | K, xw), 0);
-          const int d_hi = msel(cchk != 0, m_a, m_b);
-          // gap from the top (alignment.c:580-589) or from the left (alignment.c:590-599): chk of
-          // the neighbour xor the read / genome code; auto of the corner and the neighbour's
-          // freshness - 1 while it is > 1, else the cell's own initial state
+          const int d_hi = (dtab & 0x3F) << 8;   // diagonal move (alignment.c:533-578)
+          // gap from the top (alignment.c:580-589) or from the left (alignment.c:590-599), the
+          // neighbour chosen first: chk of the neighbour xor the read / genome code; auto of the
+          // corner and the neighbour's freshness - 1 while it is > 1, else the cell's own initial
+          // state
           const int c_af = (cauto << 10) | (cfresh << 12);
-          const int tfw = tmeta & 0x7000, lfw = left_meta & 0x7000;
-          const int t_hi = ((tmeta ^ (ri << 8)) & 0x300) | msel(tfw > 0x1000, iauto_w | (tfw - 0x1000), c_af);
-          const int l_hi = ((left_meta ^ (gj << 8)) & 0x300) | msel(lfw > 0x1000, iauto_w | (lfw - 0x1000), c_af);
-          nhi = msel(take_l, l_hi, msel(take_t, t_hi, d_hi));
+          const int nb = msel(take_l, left_meta, tmeta), code = msel(take_l, gj, ri);
+          const int nfw = nb & 0x7000;
+          const int g_hi = ((nb ^ (code << 8)) & 0x300) | msel(nfw > 0x1000, iauto_w | (nfw - 0x1000), c_af);
+          nhi = msel(take_l | take_t, g_hi, d_hi);
         }
-        if (TB) dw[c >> 3] |= (uint32_t)(nfrom | ((nsc > csc) ? 4 : 0)) << ((c & 7) * 4);
+        // direction nibble: from, and in color mode whether the score rose (the walk's
+        // correction test, alignment.c:684-705)
+        if (TB) dw[c >> 3] |= (uint32_t)(nfrom | ((COLOR && nsc > csc) ? 4 : 0)) << ((c & 7) * 4);
         sc[c] = nsc;
         meta[c] = COLOR ? (n_lo | nhi) : n_lo;
         left_sc = nsc;
@@ -391,10 +429,10 @@ __device__ __forceinline__ void tb_walk(const GenomeDev& g, const uint32_t* pl, 
 // best_j.
 template <int IM, bool COLOR, bool TB = true>
 __device__ AlignRec dp_one(const GenomeDev& g, const ReadsDev& r, const ScoreDev& scd, int rs, int h, uint32_t* tbw,
-                           uint8_t* tb) {
+                           uint8_t* tb, const uint8_t* color_tab) {
   const DPConst k(scd);
   DPLane<IM, COLOR> st;
-  st.template begin<TB>(g, r, k, rs, h, tbw);
+  st.template begin<TB>(g, r, k, rs, h, tbw, color_tab);
   while (st.more()) {
     st.template row<TB>(g, k, tbw);
     if (st.S < k.t) return st.rejected();   // the rest of the rows cannot raise S

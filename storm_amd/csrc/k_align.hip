@@ -43,6 +43,11 @@ __device__ __forceinline__ AlignRec invalid_record() {
 
 template <int IM, bool COLOR>
 __global__ __launch_bounds__(128) void k_align(AlignArgs a) {
+  __shared__ uint8_t ctab[COLOR ? COLOR_TAB : 1];
+  if (COLOR) {
+    fill_color_table(ctab, threadIdx.x, blockDim.x);
+    __syncthreads();
+  }
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= a.n) return;
   const int64_t rec = a.base + k;
@@ -52,7 +57,7 @@ __global__ __launch_bounds__(128) void k_align(AlignArgs a) {
   }
   a.out[rec] = dp_one<IM, COLOR, true>(a.g, a.r, a.sc, a.rec_rs[rec], a.rec_diag[rec],
                                        (uint32_t*)(a.scratch + (k >> 6) * 64 * a.scratch_stride) + (k & 63),
-                                       a.tb + rec * TBB);
+                                       a.tb + rec * TBB, ctab);
 }
 
 hipError_t launch_align(const AlignArgs& a, hipStream_t st) {

@@ -21,6 +21,19 @@
 #include "dev_seed.h"
 #include "storm_kernels.h"
 
+// Measurement build (make phases, -DSTORM_FUSED_PHASES): each wavefront sums the constant-clock
+// time it spends in the kernel's phases and adds it to FusedArgs::phase at exit; the product build
+// compiles none of it.
+#ifdef STORM_FUSED_PHASES
+#define PH_START() uint64_t ph_t = wall_clock64(); uint64_t ph[FUSED_PHASES] = {}
+#define PH_MARK(k) do { const uint64_t t_ = wall_clock64(); ph[k] += t_ - ph_t; ph_t = t_; } while (0)
+#define PH_FLUSH() do { if (a.phase && lane == 0) for (int q_ = 0; q_ < FUSED_PHASES; ++q_) atomicAdd(&a.phase[q_], (unsigned long long)ph[q_]); } while (0)
+#else
+#define PH_START() do {} while (0)
+#define PH_MARK(k) do {} while (0)
+#define PH_FLUSH() do {} while (0)
+#endif
+
 namespace storm {
 
 constexpr int F_HCAP = 2048;         // raw hits of one read strand (LDS sort buffer)
@@ -45,6 +58,7 @@ void k_fused(FusedArgs a) {
   __shared__ uint8_t rovf[F_GMAX];
   __shared__ int32_t task_sh;
   __shared__ unsigned long long stl[4];   // candidates, filter passes, accepted, raw hits
+  __shared__ uint8_t ctab[COLOR ? COLOR_TAB : 1];   // color DP: the diagonal-move table
 
   const int lane = threadIdx.x;
   const int n = *a.n_reads;
@@ -63,12 +77,15 @@ void k_fused(FusedArgs a) {
   uint8_t* crs = (uint8_t*)(rec2cand + F_PCAP);
   uint8_t* cpass = crs + F_CCAP;
   if (lane < 4) stl[lane] = 0;
+  if (COLOR) fill_color_table(ctab, lane, 64);   // read after the first flush's barriers
 
   int64_t base = 0;
+  PH_START();
   // filter, DP and hit-map replay of the nc candidates of reads [f0, f1) of the task
   auto flush = [&](int f0, int f1, int nc) {
     if (lane == 0) cstart[2 * f1] = nc;
     __syncthreads();
+    PH_MARK(0);
     // filter (stage c): two candidates per lane
     for (int b0 = 0; b0 < nc; b0 += 128) {
       const int c0 = b0 + 2 * lane;
@@ -85,6 +102,7 @@ void k_fused(FusedArgs a) {
       }
     }
     __syncthreads();
+    PH_MARK(1);
     // record index of every pass, in candidate order
     int np = 0;
     for (int b0 = 0; b0 < nc; b0 += 64) {
@@ -140,6 +158,7 @@ void k_fused(FusedArgs a) {
       nrun += __shfl(incl, 63, 64);
     }
     __syncthreads();
+    PH_MARK(2);
     // One pass per record, lane l taking the running records hbuf[l], hbuf[l + 64], ...: the rows'
     // directions are stored as they are computed (the lane's scratch words), a record stops at the
     // first row where it can no longer be accepted, and one that can (score > t, S >= t) is walked
@@ -148,9 +167,10 @@ void k_fused(FusedArgs a) {
       const int k = hbuf[j];
       const int c = rec2cand[k];
       recs[k] = dp_one<IM, COLOR, true>(a.g, a.r, a.sc, (int)(2 * base + crs[c]), cdiag[c], tbw,
-                                        tbbuf + (int64_t)k * TBB);
+                                        tbbuf + (int64_t)k * TBB, ctab);
     }
     __syncthreads();   // hbuf is rewritten below
+    PH_MARK(3);
     // the records that can still be accepted (score > t, S >= t), ascending, as
     // (read strand << 16) | record in the hit buffer
     int nv = 0;
@@ -276,6 +296,7 @@ void k_fused(FusedArgs a) {
     }
     if (lane == 0) { stl[2] += (unsigned long long)st_acc; stl[3] += (unsigned long long)st_hits; }
     __syncthreads();
+    PH_MARK(4);
   };
 
   for (;;) {
@@ -284,6 +305,7 @@ void k_fused(FusedArgs a) {
     const int task = task_sh;
     __syncthreads();
     base = (int64_t)task * G;
+    PH_MARK(5);
     if (base >= n) break;
     const int nr = (n - base) < G ? (int)(n - base) : G;
     for (int j = lane; j < nr; j += 64) rovf[j] = 0;
@@ -322,6 +344,7 @@ void k_fused(FusedArgs a) {
     }
   }
   if (a.stats && lane < 4) atomicAdd(&a.stats[lane], stl[lane]);
+  PH_FLUSH();
 }
 
 // clears the task counter and the stats inside a captured graph (no memset nodes)

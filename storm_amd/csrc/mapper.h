@@ -159,6 +159,7 @@ struct FusedTune {
 struct FusedState {
   DevReads reads;
   DevMem slots, cells, overflow, n_dev, counter, stats, scratch, tbbuf, work, gids, gtb, gcount;
+  DevMem phase;   // measurement build only: k_fused's per-phase clock ticks
   int64_t cap_reads = 0, scratch_stride = 0;
   int grid = 0;
   // captured graphs (reset + fused kernel) keyed by the buffers, settings and reference they were

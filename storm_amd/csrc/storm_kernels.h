@@ -143,7 +143,12 @@ struct FusedArgs {
   int ref_id;
   int sense_mask;
   unsigned long long* stats; // [0] candidates, [1] filter passes, [2] accepted, [3] raw hits
+  unsigned long long* phase; // measurement build only (STORM_FUSED_PHASES): wavefront clock ticks
+                             // per phase, FUSED_PHASES entries; null otherwise
 };
+// phases of k_fused: candidates (seed keys, lookups, gather, sort, keep rule), filter, pass scan +
+// DP triage, DP + traceback, hit-map replay + traceback cells, task fetch
+constexpr int FUSED_PHASES = 6;
 
 constexpr int FUSED_PCAP = 1024;   // DP records per flush of k_fused
 constexpr int FUSED_CCAP = 8192;   // candidates per flush of k_fused

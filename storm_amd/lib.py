@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "libstorm_hip.so")
+# STORM_LIB names another build of the library (the phase-timer measurement build:
+# make -C storm_amd/csrc phases)
+LIB_PATH = os.environ.get("STORM_LIB") or os.path.join(HERE, "_build", "libstorm_hip.so")
 
 STORM_MODE_PLAIN = 0
 STORM_MODE_BATCHED = 1
