@@ -1,5 +1,6 @@
 // dev_align.h -- banded glocal DP + traceback (used by k_align and k_fused); semantics in k_align.hip.
 #pragma once
+#include <type_traits>
 #include "storm_kernels.h"
 
 namespace storm {
@@ -211,73 +212,80 @@ struct DPLane {
     uint32_t dw[DW];
 #pragma unroll
     for (int w = 0; w < DW; ++w) dw[w] = 0;
+    // the cells of the row; a row in which no lane's band touches a masked genome code (N runs:
+    // rare) takes the copy without the masked-cell selects
+    auto cells = [&](auto mk) {
+      constexpr bool MK = decltype(mk)::value;
 #pragma unroll
-    for (int c = 0; c < NCM; ++c) {
-      if (c < NC) {
-        const uint32_t Lw = c < 32 ? wl0 : wl1, Hw = c < 32 ? wh0 : wh1, Kw = c < 32 ? wm0 : wm1;
-        const int gj = (int)(((Lw >> (c & 31)) & 1u) | (((Hw >> (c & 31)) & 1u) << 1));
-        const bool masked = ((Kw >> (c & 31)) & 1u) != 0;
-        // this cell before the recurrence (init_matrix): pair score, or MIN_SCORE when masked;
-        // in color mode the checksum, auto-correction and freshness of a mismatch
-        const bool mis = (((c < 32 ? mis0 : mis1) >> (c & 31)) & 1u) != 0;
-        const int cchk = msel(masked, 0, ri ^ gj);
-        const int cs = msel(masked, MIN_SCORE, msel(mis, s_x, s_m));
-        const int cauto = cchk, cfresh = msel(cchk != 0, qi, 0);
-        const int csc = sc[c], cmeta = meta[c];                          // corner (i-1, j-1)
-        const int tsc = (c + 1 < NCM) ? sc[(c + 1) < NCM ? c + 1 : c] : MIN_SCORE;   // top (i-1, j)
-        const int tmeta = (c + 1 < NCM) ? meta[(c + 1) < NCM ? c + 1 : c] : m_make(3, I, 0, 0, 0);
-        // The recurrence in select form (alignment.c:465-604): every lane runs the same
-        // instructions whatever its record, with no divergent branches per cell.
-        int match_score = cs;
-        // color state of the corner, kept in place in its meta word (chk bits 8-9, auto 10-11,
-        // fresh 12-13): the diagonal move's new state and refund from the table
-        const int dtab = COLOR ? (int)ctab[((cmeta >> 8) & 0x3F) | (cchk << 6) | (qi << 8)] : 0;
-        const int iauto_w = COLOR ? (cmeta & 0xC00) : 0;
-        if (COLOR) {   // alignment.c:465-468; & (not &&): no divergent branches
-          const bool refund = ((dtab & 0x40) != 0) & (cs != MIN_SCORE) & (cs < 0);
-          match_score = msel(refund, -2 * cs, cs);
-        }
-        const int diag = csc + match_score;
-        const int t_lo8 = COLOR ? (tmeta & 0xff) : tmeta, l_lo8 = COLOR ? (left_meta & 0xff) : left_meta;
-        // a cell's from is 1 (left), 2 (top) or 3 (corner; also row 0 and the band border): a gap
-        // extends from the top cell iff its from is 2 (bit 0 clear), from the left iff 1 (bit 1 clear)
-        const int t_try = tsc + gap_ext + (t_lo8 & 1) * (gap_open - gap_ext);
-        const bool take_t = (t_lo8 < I4) & (t_try > diag);   // indels < I
+      for (int c = 0; c < NCM; ++c) {
+        if (c < NC) {
+          const uint32_t Lw = c < 32 ? wl0 : wl1, Hw = c < 32 ? wh0 : wh1, Kw = c < 32 ? wm0 : wm1;
+          const int gj = (int)(((Lw >> (c & 31)) & 1u) | (((Hw >> (c & 31)) & 1u) << 1));
+          const bool masked = MK && ((Kw >> (c & 31)) & 1u) != 0;
+          // this cell before the recurrence (init_matrix): pair score, or MIN_SCORE when masked;
+          // in color mode the checksum, auto-correction and freshness of a mismatch
+          const bool mis = (((c < 32 ? mis0 : mis1) >> (c & 31)) & 1u) != 0;
+          const int cchk = msel(masked, 0, ri ^ gj);
+          const int cs = msel(masked, MIN_SCORE, msel(mis, s_x, s_m));
+          const int cauto = cchk, cfresh = msel(cchk != 0, qi, 0);
+          const int csc = sc[c], cmeta = meta[c];                          // corner (i-1, j-1)
+          const int tsc = (c + 1 < NCM) ? sc[(c + 1) < NCM ? c + 1 : c] : MIN_SCORE;   // top (i-1, j)
+          const int tmeta = (c + 1 < NCM) ? meta[(c + 1) < NCM ? c + 1 : c] : m_make(3, I, 0, 0, 0);
+          // The recurrence in select form (alignment.c:465-604): every lane runs the same
+          // instructions whatever its record, with no divergent branches per cell.
+          int match_score = cs;
+          // color state of the corner, kept in place in its meta word (chk bits 8-9, auto 10-11,
+          // fresh 12-13): the diagonal move's new state and refund from the table
+          const int dtab = COLOR ? (int)ctab[((cmeta >> 8) & 0x3F) | (cchk << 6) | (qi << 8)] : 0;
+          const int iauto_w = COLOR ? (cmeta & 0xC00) : 0;
+          if (COLOR) {   // alignment.c:465-468; & (not &&): no divergent branches
+            const bool refund = ((dtab & 0x40) != 0) & (cs != MIN_SCORE) & (cs < 0);
+            match_score = msel(refund, -2 * cs, cs);
+          }
+          const int diag = csc + match_score;
+          const int t_lo8 = COLOR ? (tmeta & 0xff) : tmeta, l_lo8 = COLOR ? (left_meta & 0xff) : left_meta;
+          // a cell's from is 1 (left), 2 (top) or 3 (corner; also row 0 and the band border): a gap
+          // extends from the top cell iff its from is 2 (bit 0 clear), from the left iff 1 (bit 1 clear)
+          const int t_try = tsc + gap_ext + (t_lo8 & 1) * (gap_open - gap_ext);
+          const bool take_t = (t_lo8 < I4) & (t_try > diag);   // indels < I
 
-        const int after_t = msel(take_t, t_try, diag);
-        const int l_try = left_sc + gap_ext + ((l_lo8 >> 1) & 1) * (gap_open - gap_ext);
-        const bool take_l = (l_lo8 < I4) & (l_try > after_t);
-        const int nsc = msel(take_l, l_try, after_t);
-        // low byte of the new meta word (from | indels << 2): the corner's indels with from = 3, or a
-        // neighbour's indels + 1 with from = 2 (top) / 1 (left); indels <= 31, so +4 never carries out
-        const int d_lo = (COLOR ? (cmeta & 0xff) : cmeta) | 3;
-        const int t_lo = (t_lo8 & 0xfc) + 6;
-        const int l_lo = (l_lo8 & 0xfc) + 5;
-        const int n_lo = msel(take_l, l_lo, msel(take_t, t_lo, d_lo));
-        const int nfrom = n_lo & 3;
-        int nhi = 0;   // color state bits 8-13 of the new meta word
-        if (COLOR) {
-          const int d_hi = (dtab & 0x3F) << 8;   // diagonal move (alignment.c:533-578)
-          // gap from the top (alignment.c:580-589) or from the left (alignment.c:590-599), the
-          // neighbour chosen first: chk of the neighbour xor the read / genome code; auto of the
-          // corner and the neighbour's freshness - 1 while it is > 1, else the cell's own initial
-          // state
-          const int c_af = (cauto << 10) | (cfresh << 12);
-          const int nb = msel(take_l, left_meta, tmeta), code = msel(take_l, gj, ri);
-          const int nfw = nb & 0x7000;
-          const int g_hi = ((nb ^ (code << 8)) & 0x300) | msel(nfw > 0x1000, iauto_w | (nfw - 0x1000), c_af);
-          nhi = msel(take_l | take_t, g_hi, d_hi);
+          const int after_t = msel(take_t, t_try, diag);
+          const int l_try = left_sc + gap_ext + ((l_lo8 >> 1) & 1) * (gap_open - gap_ext);
+          const bool take_l = (l_lo8 < I4) & (l_try > after_t);
+          const int nsc = msel(take_l, l_try, after_t);
+          // low byte of the new meta word (from | indels << 2): the corner's indels with from = 3, or a
+          // neighbour's indels + 1 with from = 2 (top) / 1 (left); indels <= 31, so +4 never carries out
+          const int d_lo = (COLOR ? (cmeta & 0xff) : cmeta) | 3;
+          const int t_lo = (t_lo8 & 0xfc) + 6;
+          const int l_lo = (l_lo8 & 0xfc) + 5;
+          const int n_lo = msel(take_l, l_lo, msel(take_t, t_lo, d_lo));
+          const int nfrom = n_lo & 3;
+          int nhi = 0;   // color state bits 8-13 of the new meta word
+          if (COLOR) {
+            const int d_hi = (dtab & 0x3F) << 8;   // diagonal move (alignment.c:533-578)
+            // gap from the top (alignment.c:580-589) or from the left (alignment.c:590-599), the
+            // neighbour chosen first: chk of the neighbour xor the read / genome code; auto of the
+            // corner and the neighbour's freshness - 1 while it is > 1, else the cell's own initial
+            // state
+            const int c_af = (cauto << 10) | (cfresh << 12);
+            const int nb = msel(take_l, left_meta, tmeta), code = msel(take_l, gj, ri);
+            const int nfw = nb & 0x7000;
+            const int g_hi = ((nb ^ (code << 8)) & 0x300) | msel(nfw > 0x1000, iauto_w | (nfw - 0x1000), c_af);
+            nhi = msel(take_l | take_t, g_hi, d_hi);
+          }
+          // direction nibble: from, and in color mode whether the score rose (the walk's
+          // correction test, alignment.c:684-705)
+          if (TB) dw[c >> 3] |= (uint32_t)(nfrom | ((COLOR && nsc > csc) ? 4 : 0)) << ((c & 7) * 4);
+          sc[c] = nsc;
+          meta[c] = COLOR ? (n_lo | nhi) : n_lo;
+          left_sc = nsc;
+          left_meta = meta[c];
+          if (max_on_line < nsc) max_on_line = nsc;
         }
-        // direction nibble: from, and in color mode whether the score rose (the walk's
-        // correction test, alignment.c:684-705)
-        if (TB) dw[c >> 3] |= (uint32_t)(nfrom | ((COLOR && nsc > csc) ? 4 : 0)) << ((c & 7) * 4);
-        sc[c] = nsc;
-        meta[c] = COLOR ? (n_lo | nhi) : n_lo;
-        left_sc = nsc;
-        left_meta = meta[c];
-        if (max_on_line < nsc) max_on_line = nsc;
       }
-    }
+    };
+    if (__builtin_amdgcn_ballot_w64((wm0 | wm1) != 0u)) cells(std::true_type{});
+    else cells(std::false_type{});
     if (TB) {
 #pragma unroll
       for (int w = 0; w < DW; ++w) tbw[64 * (i * DW + w)] = dw[w];
