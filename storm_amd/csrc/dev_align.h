@@ -15,9 +15,9 @@ __device__ __forceinline__ int m_make(int from, int ind, int chk, int autoc, int
   return from | (ind << 2) | (chk << 8) | (autoc << 10) | (fresh << 12);
 }
 
-// a ? x : y as mask arithmetic: the compiler keeps it a straight-line select instead of a
-// divergent branch around the computation of the unused operand
-__device__ __forceinline__ int msel(bool a, int x, int y) { return y ^ ((y ^ x) & -(int)a); }
+// a ? x : y over operands that are both computed: one v_cndmask on the compare's lane mask (no
+// branch; mask arithmetic would first turn the mask into a 0 / -1 register)
+__device__ __forceinline__ int msel(bool a, int x, int y) { return a ? x : y; }
 
 // Scoring constants of the DP in registers (a lane-indexed read of the argument block would be a
 // memory round trip per cell).
@@ -199,6 +199,9 @@ struct DPLane {
     const int ri = (int)(((rlo >> (i & 31)) & 1u) | (((rhi >> (i & 31)) & 1u) << 1));
     const int qi = (int)((rq >> ((i & 15) << 1)) & 3u);
     const int s_m = k.match(qi), s_x = k.mismatch(qi);
+    // color: a pair score after the correction refund (-2 x a negative score, alignment.c:465-468)
+    const int r_m = COLOR ? (s_m < 0 ? -2 * s_m : s_m) : 0, r_x = COLOR ? (s_x < 0 ? -2 * s_x : s_x) : 0;
+    const int q12 = qi << 12;
     const uint32_t wl0 = __builtin_amdgcn_alignbit(gl[1], gl[0], sh), wh0 = __builtin_amdgcn_alignbit(gh[1], gh[0], sh);
     const uint32_t wm0 = __builtin_amdgcn_alignbit(gm[1], gm[0], sh);
     const uint32_t wl1 = NCM > 32 ? __builtin_amdgcn_alignbit(gl[2], gl[1], sh) : 0u;
@@ -212,22 +215,23 @@ struct DPLane {
     uint32_t dw[DW];
 #pragma unroll
     for (int w = 0; w < DW; ++w) dw[w] = 0;
-    // the cells of the row; a row in which no lane's band touches a masked genome code (N runs:
-    // rare) takes the copy without the masked-cell selects
-    auto cells = [&](auto mk) {
-      constexpr bool MK = decltype(mk)::value;
+    // the cells of the row, in one of three copies: a row in which no lane's band touches a masked
+    // genome code (N runs: rare) takes a copy without the masked-cell selects, and when the band
+    // has the kernel's full width (I = IM, the usual case) one without the per-column width test,
+    // so the cells form one block the compiler schedules as a whole
+    auto cells = [&](auto mk, auto full) {
+      constexpr bool MK = decltype(mk)::value, FULL = decltype(full)::value;
 #pragma unroll
       for (int c = 0; c < NCM; ++c) {
-        if (c < NC) {
+        if (FULL || c < NC) {
           const uint32_t Lw = c < 32 ? wl0 : wl1, Hw = c < 32 ? wh0 : wh1, Kw = c < 32 ? wm0 : wm1;
           const int gj = (int)(((Lw >> (c & 31)) & 1u) | (((Hw >> (c & 31)) & 1u) << 1));
           const bool masked = MK && ((Kw >> (c & 31)) & 1u) != 0;
           // this cell before the recurrence (init_matrix): pair score, or MIN_SCORE when masked;
           // in color mode the checksum, auto-correction and freshness of a mismatch
           const bool mis = (((c < 32 ? mis0 : mis1) >> (c & 31)) & 1u) != 0;
-          const int cchk = msel(masked, 0, ri ^ gj);
-          const int cs = msel(masked, MIN_SCORE, msel(mis, s_x, s_m));
-          const int cauto = cchk, cfresh = msel(cchk != 0, qi, 0);
+          const int cchk = MK ? msel(masked, 0, ri ^ gj) : (ri ^ gj);
+          const int cs = MK ? msel(masked, MIN_SCORE, msel(mis, s_x, s_m)) : msel(mis, s_x, s_m);
           const int csc = sc[c], cmeta = meta[c];                          // corner (i-1, j-1)
           const int tsc = (c + 1 < NCM) ? sc[(c + 1) < NCM ? c + 1 : c] : MIN_SCORE;   // top (i-1, j)
           const int tmeta = (c + 1 < NCM) ? meta[(c + 1) < NCM ? c + 1 : c] : m_make(3, I, 0, 0, 0);
@@ -239,8 +243,12 @@ struct DPLane {
           const int dtab = COLOR ? (int)ctab[((cmeta >> 8) & 0x3F) | (cchk << 6) | (qi << 8)] : 0;
           const int iauto_w = COLOR ? (cmeta & 0xC00) : 0;
           if (COLOR) {   // alignment.c:465-468; & (not &&): no divergent branches
-            const bool refund = ((dtab & 0x40) != 0) & (cs != MIN_SCORE) & (cs < 0);
-            match_score = msel(refund, -2 * cs, cs);
+            if (MK) {
+              const bool refund = ((dtab & 0x40) != 0) & (cs != MIN_SCORE) & (cs < 0);
+              match_score = msel(refund, -2 * cs, cs);
+            } else {   // no masked cell: the refunded score of the cell's pair, where the table allows it
+              match_score = msel((dtab & 0x40) != 0, msel(mis, r_x, r_m), cs);
+            }
           }
           const int diag = csc + match_score;
           const int t_lo8 = COLOR ? (tmeta & 0xff) : tmeta, l_lo8 = COLOR ? (left_meta & 0xff) : left_meta;
@@ -253,25 +261,28 @@ struct DPLane {
           const int l_try = left_sc + gap_ext + ((l_lo8 >> 1) & 1) * (gap_open - gap_ext);
           const bool take_l = (l_lo8 < I4) & (l_try > after_t);
           const int nsc = msel(take_l, l_try, after_t);
-          // low byte of the new meta word (from | indels << 2): the corner's indels with from = 3, or a
-          // neighbour's indels + 1 with from = 2 (top) / 1 (left); indels <= 31, so +4 never carries out
+          // low byte of the new meta word (from | indels << 2): the corner's indels with from = 3, or
+          // the gap neighbour's indels + 1 with from = 2 (top) / 1 (left); indels <= 31, so +4 never
+          // carries out
+          const bool gap = take_l | take_t;
+          const int nb = msel(take_l, left_meta, tmeta);   // the gap neighbour
           const int d_lo = (COLOR ? (cmeta & 0xff) : cmeta) | 3;
-          const int t_lo = (t_lo8 & 0xfc) + 6;
-          const int l_lo = (l_lo8 & 0xfc) + 5;
-          const int n_lo = msel(take_l, l_lo, msel(take_t, t_lo, d_lo));
+          const int g_lo = (nb & 0xfc) + msel(take_l, 5, 6);
+          const int n_lo = msel(gap, g_lo, d_lo);
           const int nfrom = n_lo & 3;
           int nhi = 0;   // color state bits 8-13 of the new meta word
           if (COLOR) {
             const int d_hi = (dtab & 0x3F) << 8;   // diagonal move (alignment.c:533-578)
-            // gap from the top (alignment.c:580-589) or from the left (alignment.c:590-599), the
-            // neighbour chosen first: chk of the neighbour xor the read / genome code; auto of the
-            // corner and the neighbour's freshness - 1 while it is > 1, else the cell's own initial
-            // state
-            const int c_af = (cauto << 10) | (cfresh << 12);
-            const int nb = msel(take_l, left_meta, tmeta), code = msel(take_l, gj, ri);
+            // gap from the top (alignment.c:580-589) or from the left (alignment.c:590-599): chk of
+            // the neighbour xor the read / genome code; auto of the corner and the neighbour's
+            // freshness - 1 while it is > 1, else the cell's own initial state (auto = chk0, fresh =
+            // the level when chk0 != 0)
+            const bool c_mis = MK ? (cchk != 0) : mis;
+            const int c_af = msel(c_mis, (cchk << 10) | q12, 0);
+            const int code = msel(take_l, gj, ri);
             const int nfw = nb & 0x7000;
             const int g_hi = ((nb ^ (code << 8)) & 0x300) | msel(nfw > 0x1000, iauto_w | (nfw - 0x1000), c_af);
-            nhi = msel(take_l | take_t, g_hi, d_hi);
+            nhi = msel(gap, g_hi, d_hi);
           }
           // direction nibble: from, and in color mode whether the score rose (the walk's
           // correction test, alignment.c:684-705)
@@ -284,8 +295,9 @@ struct DPLane {
         }
       }
     };
-    if (__builtin_amdgcn_ballot_w64((wm0 | wm1) != 0u)) cells(std::true_type{});
-    else cells(std::false_type{});
+    if (__builtin_amdgcn_ballot_w64((wm0 | wm1) != 0u)) cells(std::true_type{}, std::false_type{});
+    else if (NC == NCM) cells(std::false_type{}, std::true_type{});
+    else cells(std::false_type{}, std::false_type{});
     if (TB) {
 #pragma unroll
       for (int w = 0; w < DW; ++w) tbw[64 * (i * DW + w)] = dw[w];
