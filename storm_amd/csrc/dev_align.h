@@ -40,14 +40,15 @@ struct DPConst {
 // cell (alignment.c:533-578) and, at bit 6, whether the corner's state turns the cell's mismatch
 // penalty into a refund (alignment.c:465-468: pending checksum, and the composed checksum is the
 // identity or the corner's auto-correction; the caller adds "penalty < 0, cell not masked").
-// The kernels that run color DPs fill it in LDS at start (fill_color_table).
+// The table is built at compile time; the kernels that run color DPs copy it to LDS
+// (copy_color_table) and look entries up there.
 constexpr int COLOR_TAB = 1024;
 
-__device__ __forceinline__ uint8_t color_diag_entry(int idx) {
+constexpr uint8_t color_diag_entry(int idx) {
   const int c_chk = idx & 3, c_auto = (idx >> 2) & 3, c_fresh = (idx >> 4) & 3;
   const int chk0 = (idx >> 6) & 3, q = (idx >> 8) & 3;
   const int x = c_chk ^ chk0;   // COMPOSE(crt checksum, corner checksum)
-  int chk = x, autoc, fresh;
+  int chk = x, autoc = 0, fresh = 0;
   if (chk0 != 0) {              // the corner's checksum changes: a color mismatch in this cell
     if (c_chk == 0) { autoc = x; fresh = q; }                  // a SNP starts or a read error
     else if (x == c_auto) { chk = 0; autoc = 0; fresh = 0; }   // the SNP ends
@@ -58,10 +59,8 @@ __device__ __forceinline__ uint8_t color_diag_entry(int idx) {
       if (c_fresh - 1 > 0) { autoc = c_auto; fresh = c_fresh - 1; }
       else { autoc = x; fresh = q; }
     }
-  } else {                      // match: the pending mismatch wears off
-    autoc = 0;                  // the cell's own (initial) state, chk0 = 0
-    fresh = 0;
-    if (c_auto != 0) {
+  } else {                      // match: the pending mismatch wears off (else the cell's own
+    if (c_auto != 0) {          // initial state: auto = chk0 = 0, fresh = 0)
       if (c_fresh - 1 > 0 && x != 0) { autoc = c_auto; fresh = c_fresh - 1; }
     } else {
       chk = 0;
@@ -71,9 +70,21 @@ __device__ __forceinline__ uint8_t color_diag_entry(int idx) {
   return (uint8_t)(chk | (autoc << 2) | (fresh << 4) | (refund_ok << 6));
 }
 
-// the table in LDS, filled by the nt threads of a workgroup (the caller synchronises after)
-__device__ __forceinline__ void fill_color_table(uint8_t* tab, int tid, int nt) {
-  for (int e = tid; e < COLOR_TAB; e += nt) tab[e] = color_diag_entry(e);
+struct alignas(16) ColorTable {
+  uint8_t v[COLOR_TAB];
+};
+constexpr ColorTable make_color_table() {
+  ColorTable t{};
+  for (int e = 0; e < COLOR_TAB; ++e) t.v[e] = color_diag_entry(e);
+  return t;
+}
+__device__ constexpr ColorTable kColorTable = make_color_table();
+
+// copies the table to LDS (16-byte aligned) with the nt threads of a workgroup; the caller
+// synchronises before the first lookup
+__device__ __forceinline__ void copy_color_table(uint8_t* lds, int tid, int nt) {
+  for (int e = 16 * tid; e < COLOR_TAB; e += 16 * nt)
+    *(uint4*)(lds + e) = *(const uint4*)(kColorTable.v + e);
 }
 
 // The DP of one candidate (read strand rs = 2*read + strand, diagonal h) as a state machine over
@@ -295,9 +306,13 @@ struct DPLane {
         }
       }
     };
-    if (__builtin_amdgcn_ballot_w64((wm0 | wm1) != 0u)) cells(std::true_type{}, std::false_type{});
-    else if (NC == NCM) cells(std::false_type{}, std::true_type{});
-    else cells(std::false_type{}, std::false_type{});
+    if constexpr (NCM > 16) {   // wide bands: one copy (three would spill registers)
+      cells(std::true_type{}, std::false_type{});
+    } else {
+      if (__builtin_amdgcn_ballot_w64((wm0 | wm1) != 0u)) cells(std::true_type{}, std::false_type{});
+      else if (NC == NCM) cells(std::false_type{}, std::true_type{});
+      else cells(std::false_type{}, std::false_type{});
+    }
     if (TB) {
 #pragma unroll
       for (int w = 0; w < DW; ++w) tbw[64 * (i * DW + w)] = dw[w];

@@ -43,9 +43,9 @@ __device__ __forceinline__ AlignRec invalid_record() {
 
 template <int IM, bool COLOR>
 __global__ __launch_bounds__(128) void k_align(AlignArgs a) {
-  __shared__ uint8_t ctab[COLOR ? COLOR_TAB : 1];
+  __shared__ __align__(16) uint8_t ctab[COLOR ? COLOR_TAB : 16];
   if (COLOR) {
-    fill_color_table(ctab, threadIdx.x, blockDim.x);
+    copy_color_table(ctab, threadIdx.x, blockDim.x);
     __syncthreads();
   }
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -43,22 +43,23 @@ constexpr int F_GMAX = 64;           // reads per task
 constexpr int F_PENDING = -3;        // Slot.pass while the entry's traceback is still in the flush buffer
 static_assert(TBB % 16 == 0, "traceback records are copied in 16-byte units");
 static_assert(F_CCAP >= 2 * F_HCAP, "a flush must hold both strands of one read");
+static_assert((F_HCAP - F_PCAP) * 4 >= COLOR_TAB, "the color table lives in hbuf past the record lists");
 
 // Wavefronts per SIMD: four for bands up to 15 columns in nucleotide mode (at W = 8 the kernel then
-// keeps 128 VGPRs and spills a few bytes, measured 4 % faster on 100-nt reads than three), three
+// keeps 128 VGPRs and spills a few bytes, measured 4 % faster on 100-nt reads than three) and for
+// color mode at W <= 4 (128 VGPRs; 10 KB of LDS per workgroup keeps 16 workgroups on a CU), three
 // for color mode at W = 8 (its color state costs more registers: four measured 2 % slower), two for
 // the wide bands (up to 256 VGPRs).
 template <int W, int IM, bool COLOR>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 8 ? (COLOR ? 3 : 4) : 2)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(W <= 4 ? 4 : W <= 8 ? (COLOR ? 3 : 4) : 2)))
 void k_fused(FusedArgs a) {
-  __shared__ int32_t hbuf[F_HCAP];
+  __shared__ __align__(16) int32_t hbuf[F_HCAP];
   __shared__ int32_t lk[192];
   __shared__ int32_t cstart[2 * F_GMAX + 1];
   __shared__ int32_t rawh[2 * F_GMAX];
   __shared__ uint8_t rovf[F_GMAX];
   __shared__ int32_t task_sh;
   __shared__ unsigned long long stl[4];   // candidates, filter passes, accepted, raw hits
-  __shared__ uint8_t ctab[COLOR ? COLOR_TAB : 1];   // color DP: the diagonal-move table
 
   const int lane = threadIdx.x;
   const int n = *a.n_reads;
@@ -77,7 +78,6 @@ void k_fused(FusedArgs a) {
   uint8_t* crs = (uint8_t*)(rec2cand + F_PCAP);
   uint8_t* cpass = crs + F_CCAP;
   if (lane < 4) stl[lane] = 0;
-  if (COLOR) fill_color_table(ctab, lane, 64);   // read after the first flush's barriers
 
   int64_t base = 0;
   PH_START();
@@ -157,6 +157,10 @@ void k_fused(FusedArgs a) {
       if (r) hbuf[nrun + incl - 1] = k;
       nrun += __shfl(incl, 63, 64);
     }
+    // color DP: the diagonal-move table in the upper half of hbuf, which the flush does not use
+    // (its record lists hold at most F_PCAP entries); the next task's candidates overwrite it
+    uint8_t* ctab = (uint8_t*)(hbuf + F_PCAP);
+    if (COLOR) copy_color_table(ctab, lane, 64);
     __syncthreads();
     PH_MARK(2);
     // One pass per record, lane l taking the running records hbuf[l], hbuf[l + 64], ...: the rows'
