@@ -6,6 +6,9 @@
 #include <algorithm>
 #include <cstddef>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <functional>
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
@@ -17,6 +20,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "host_pool.h"
 #include "mapper.h"
 
 namespace storm {
@@ -142,7 +146,7 @@ void length_order(const storm_reads_view* v, std::vector<int64_t>& order, bool& 
 
 // Uploads the view's reads and arranges them on the device in order[0..) (a permutation of the
 // view).  The caller's arrays go to the device as they are; a reordering is a device gather.
-// memcpy on up to 8 host threads, 1 MiB or more each (a copy between pinned memory and the
+// memcpy in parts of 1 MiB or more on up to 8 host threads (a copy between pinned memory and the
 // caller's pageable memory runs at the host's memory bandwidth only with several threads)
 static void host_copy(void* dst, const void* src, size_t bytes) {
   const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, bytes >> 20));
@@ -151,13 +155,10 @@ static void host_copy(void* dst, const void* src, size_t bytes) {
     return;
   }
   const size_t per = (bytes + nt - 1) / nt;
-  std::vector<std::thread> th;
-  for (int t = 0; t < nt; ++t) {
+  HostPool::get().run(nt, [&](int t) {
     const size_t off = (size_t)t * per;
-    if (off >= bytes) break;
-    th.emplace_back([=] { memcpy((char*)dst + off, (const char*)src + off, std::min(per, bytes - off)); });
-  }
-  for (auto& x : th) x.join();
+    if (off < bytes) memcpy((char*)dst + off, (const char*)src + off, std::min(per, bytes - off));
+  });
 }
 
 struct CopySeg {
