@@ -783,22 +783,34 @@ static int tune_fused(storm_mapper* m, int64_t n, int maxL, int max_grid, FusedT
     HIPOK(launch_fused(a, max_grid, m->st));
     HIPOK(hipStreamSynchronize(m->st));
   }
-  const int groups[] = {8, 16, 32, 64};
-  const int grids[] = {max_grid, max_grid / 2};
-  for (int gi = 0; gi < 2; ++gi) {
-    for (int G : groups) {
-      const int grid = std::max(1, grids[gi]);
-      FusedArgs a = fused_args(m, 0, G);
-      HIPOK(launch_slots_init(F.slots.as<Slot>(), 4 * sample, m->st));
-      HIPOK(launch_fused_reset(a.task_counter, a.stats, m->st));
-      HIPOK(hipEventRecord(e0, m->st));
-      HIPOK(launch_fused(a, grid, m->st));
-      HIPOK(hipEventRecord(e1, m->st));
-      HIPOK(hipEventSynchronize(e1));
-      float ms = 0;
-      HIPOK(hipEventElapsedTime(&ms, e0, e1));
-      if (ms < best.ms) { best.ms = ms; best.group = G; best.grid = grid; }
-    }
+  auto timed = [&](int G, int grid, float& ms) -> int {
+    FusedArgs a = fused_args(m, 0, G);
+    HIPOK(launch_slots_init(F.slots.as<Slot>(), 4 * sample, m->st));
+    HIPOK(launch_fused_reset(a.task_counter, a.stats, m->st));
+    HIPOK(hipEventRecord(e0, m->st));
+    HIPOK(launch_fused(a, grid, m->st));
+    HIPOK(hipEventRecord(e1, m->st));
+    HIPOK(hipEventSynchronize(e1));
+    HIPOK(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best.ms) { best.ms = ms; best.group = G; best.grid = grid; }
+    return STORM_OK;
+  };
+  // reads per task G = 8, 16, 32, 64 on the occupancy-sized grid, stopping at the first G slower
+  // than the one before (fewer, longer tasks leave workgroups idle on the sample: a larger G was
+  // never faster once one was slower, and such runs take several times the best one), then the
+  // best G on half the grid
+  float prev = 1e30f;
+  for (int G = 8; G <= 64; G *= 2) {
+    float ms = 0;
+    int rc = timed(G, max_grid, ms);
+    if (rc) return rc;
+    if (ms > prev) break;
+    prev = ms;
+  }
+  if (max_grid / 2 >= 1) {
+    float ms = 0;
+    int rc = timed(best.group, max_grid / 2, ms);
+    if (rc) return rc;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
