@@ -5,7 +5,9 @@
 
 namespace storm {
 
-// meta word of a cell: from (2 bits) | indels << 2 (6 bits) | chk << 8 | autoc << 10 | fresh << 12
+// meta word of a cell: from (2 bits) | indels << 2 (6 bits) | chk << 8 | autoc << 10 | fresh << 12,
+// and in bits 16-31 the signed gap penalty of a vertical gap step out of the cell (extend when the
+// cell was reached from the top, else open: alignment.c:386-388), so the cell below adds it as is
 __device__ __forceinline__ int m_from(int m) { return m & 3; }
 __device__ __forceinline__ int m_ind(int m) { return (m >> 2) & 63; }
 __device__ __forceinline__ int m_chk(int m) { return (m >> 8) & 3; }
@@ -23,11 +25,17 @@ __device__ __forceinline__ int msel(bool a, int x, int y) { return a ? x : y; }
 // memory round trip per cell).
 struct DPConst {
   int I, d, NC, I4, t, top_match, gap_open, gap_ext;
+  int m_diag, m_top, m_left;   // the new meta word's constant part: from, indels step, vertical gap penalty
   int pm0, pm1, pm2, pm3, px0, px1, px2, px3;   // pair_scores[match / mismatch][level]
   __device__ explicit DPConst(const ScoreDev& s)
       : I(s.I), d(s.d), NC(2 * s.I + 1), I4(4 * s.I), t(s.t), top_match(s.ps[0][3]), gap_open(s.gap_open),
         gap_ext(s.gap_ext), pm0(s.ps[0][0]), pm1(s.ps[0][1]), pm2(s.ps[0][2]), pm3(s.ps[0][3]), px0(s.ps[1][0]),
-        px1(s.ps[1][1]), px2(s.ps[1][2]), px3(s.ps[1][3]) {}
+        px1(s.ps[1][1]), px2(s.ps[1][2]), px3(s.ps[1][3]) {
+    const int vo = (int)((uint32_t)(uint16_t)(short)s.gap_open << 16), ve = (int)((uint32_t)(uint16_t)(short)s.gap_ext << 16);
+    m_diag = 3 | vo;       // from the corner: indels kept, a vertical step out opens a gap
+    m_top = 6 | ve;        // from the top: indels + 1, a vertical step out extends the gap
+    m_left = 5 | vo;       // from the left: indels + 1, a vertical step out opens a gap
+  }
   __device__ int match(int q) const { return msel(q & 2, msel(q & 1, pm3, pm2), msel(q & 1, pm1, pm0)); }
   __device__ int mismatch(int q) const { return msel(q & 2, msel(q & 1, px3, px2), msel(q & 1, px1, px0)); }
 };
@@ -154,14 +162,14 @@ struct DPLane {
           if (COLOR) {
             const int chk = r0 ^ gj;
             sc[c] = chk ? s_x : s_m;
-            meta[c] = m_make(3, 0, chk, chk, chk ? q0 : 0);
+            meta[c] = m_make(3, 0, chk, chk, chk ? q0 : 0) | (k.m_diag & ~0xFFFF);
           } else {
             sc[c] = (r0 != gj) ? s_x : s_m;
-            meta[c] = m_make(3, 0, 0, 0, 0);
+            meta[c] = k.m_diag;
           }
         } else {
           sc[c] = MIN_SCORE;
-          meta[c] = m_make(3, 0, 0, 0, 0);
+          meta[c] = k.m_diag;
         }
       } else {   // columns past the band hold the border, so the top neighbour needs no test
         sc[c] = MIN_SCORE;
@@ -222,7 +230,8 @@ struct DPLane {
     const uint32_t rb0 = 0u - (uint32_t)(ri & 1), rb1 = 0u - (uint32_t)((ri >> 1) & 1);
     const uint32_t mis0 = (wl0 ^ rb0) | (wh0 ^ rb1), mis1 = NCM > 32 ? ((wl1 ^ rb0) | (wh1 ^ rb1)) : 0u;
     int max_on_line = MIN_SCORE;
-    int left_sc = MIN_SCORE, left_meta = m_make(3, I, 0, 0, 0);   // border (alignment.c:303-318)
+    // the left neighbour: its meta word and the score a horizontal gap step out of it gives
+    int left_try = MIN_SCORE, left_meta = m_make(3, I, 0, 0, 0);   // border (alignment.c:303-318)
     uint32_t dw[DW];
 #pragma unroll
     for (int w = 0; w < DW; ++w) dw[w] = 0;
@@ -262,23 +271,24 @@ struct DPLane {
             }
           }
           const int diag = csc + match_score;
-          const int t_lo8 = COLOR ? (tmeta & 0xff) : tmeta, l_lo8 = COLOR ? (left_meta & 0xff) : left_meta;
+          const int t_lo8 = tmeta & 0xff, l_lo8 = left_meta & 0xff;
           // a cell's from is 1 (left), 2 (top) or 3 (corner; also row 0 and the band border): a gap
-          // extends from the top cell iff its from is 2 (bit 0 clear), from the left iff 1 (bit 1 clear)
-          const int t_try = tsc + gap_ext + (t_lo8 & 1) * (gap_open - gap_ext);
+          // extends from the top cell iff its from is 2, from the left iff 1; the top cell carries
+          // its vertical step's penalty in its meta word, the left one's step was scored with it
+          const int t_try = tsc + (int)(short)(tmeta >> 16);
           const bool take_t = (t_lo8 < I4) & (t_try > diag);   // indels < I
 
           const int after_t = msel(take_t, t_try, diag);
-          const int l_try = left_sc + gap_ext + ((l_lo8 >> 1) & 1) * (gap_open - gap_ext);
+          const int l_try = left_try;
           const bool take_l = (l_lo8 < I4) & (l_try > after_t);
           const int nsc = msel(take_l, l_try, after_t);
-          // low byte of the new meta word (from | indels << 2): the corner's indels with from = 3, or
-          // the gap neighbour's indels + 1 with from = 2 (top) / 1 (left); indels <= 31, so +4 never
-          // carries out
+          // low byte and vertical penalty of the new meta word: the corner's indels with from = 3, or
+          // the gap neighbour's indels + 1 with from = 2 (top) / 1 (left) (indels <= 31, so +4 never
+          // carries out of the byte), and the penalty of a vertical step out of this cell
           const bool gap = take_l | take_t;
           const int nb = msel(take_l, left_meta, tmeta);   // the gap neighbour
-          const int d_lo = (COLOR ? (cmeta & 0xff) : cmeta) | 3;
-          const int g_lo = (nb & 0xfc) + msel(take_l, 5, 6);
+          const int d_lo = (cmeta & 0xff) | k.m_diag;
+          const int g_lo = (nb & 0xfc) + msel(take_l, k.m_left, k.m_top);
           const int n_lo = msel(gap, g_lo, d_lo);
           const int nfrom = n_lo & 3;
           int nhi = 0;   // color state bits 8-13 of the new meta word
@@ -300,7 +310,7 @@ struct DPLane {
           if (TB) dw[c >> 3] |= (uint32_t)(nfrom | ((COLOR && nsc > csc) ? 4 : 0)) << ((c & 7) * 4);
           sc[c] = nsc;
           meta[c] = COLOR ? (n_lo | nhi) : n_lo;
-          left_sc = nsc;
+          left_try = nsc + msel(take_l, gap_ext, gap_open);   // a horizontal step out of this cell
           left_meta = meta[c];
           if (max_on_line < nsc) max_on_line = nsc;
         }
