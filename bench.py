@@ -56,6 +56,10 @@ VALU_INSTR_PER_DP_CELL_I7_L75 = 37.75
 # candidates, the factors calibrated on a device-to-device copy of known size in the same run
 # (profiles/k_filter_W8_L75_10Mcand.md)
 HBM_BYTES_PER_CAND_W8_L75 = 45.6
+# HBM bytes fetched per index lookup by k_seed_count (FETCH_SIZE x 2.00 x 1024 / lookups, the factor
+# calibrated on a device-to-device copy in the same run; profiles/k_seed_count_lookups.md): each
+# random first[key] read takes a 128-byte line
+LOOKUP_FETCH_BYTES = {"config1": 131.3, "config3": 131.7}
 # algorithmic bytes per candidate: packed read (19 B at L = 75) + genome window (L + 2W - 1 = 90
 # codes, 23 B) + the u8 score and the pass byte
 ALG_BYTES_PER_CAND_W8_L75 = 44.0
@@ -461,6 +465,11 @@ def map_config(lib, name, gpath, pr, kw, color, chunk, sam_reads, cpu=True, ref_
         if sp["gpu_ms_lookup"] > 0 else None}
     if res["plain"]["index_lookup_GBps"]:
         res["plain"]["index_lookup_hbm_frac"] = round(res["plain"]["index_lookup_GBps"] / HBM_PEAK_GBPS, 4)
+        fb = LOOKUP_FETCH_BYTES.get(name)
+        if fb:   # the bytes the lookups move: a cache line per random first[key] read
+            gbps = fb * sp["n_lookups"] / (sp["gpu_ms_lookup"] * 1e-3) / 1e9
+            res["plain"]["index_lookup_fetched"] = {"bytes_per_lookup": fb, "GBps": round(gbps, 1),
+                                                    "hbm_frac": round(gbps / HBM_PEAK_GBPS, 3)}
     res["batched_over_plain"] = round(tp / tb_, 2)
     res["plain_records_equal_batched"] = bool(same)
     log(f"{name}: batched {n / tb_:.0f} reads/s ({tb_:.3f}s; first call {first[1]:.3f}s), plain {n / tp:.0f} reads/s "
