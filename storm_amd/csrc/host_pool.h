@@ -14,7 +14,8 @@ namespace storm {
 // Host worker threads shared by every mapper of the process, started once: a call hands them the
 // parts of one job and works on it itself until every part is done (a thread created per copy
 // costs tens of microseconds, as much as copying a megabyte).  Jobs of concurrent callers (the
-// serving loop's lanes) queue and are served in turn.
+// serving loop's lanes) queue and are served in turn.  The workers live until the process ends
+// (the library is not meant to be unloaded while the process runs).
 class HostPool {
  public:
   static HostPool& get() {
