@@ -8,12 +8,14 @@ namespace storm {
 typedef short s2 __attribute__((ext_vector_type(2)));
 typedef unsigned short u2 __attribute__((ext_vector_type(2)));
 
+// One row of a candidate pair: P[d] holds, for diagonal d, the mismatch bits of 16 consecutive
+// rows of candidate a (low half) and candidate b (high half); this row is bit t of each half.
 template <int D, bool MASKED>
-__device__ __forceinline__ void filter_row(s2 (&M)[D], u2 (&G)[D], u2& best, uint32_t mlo, uint32_t mhi,
+__device__ __forceinline__ void filter_row(s2 (&M)[D], u2 (&G)[D], u2& best, const uint32_t (&P)[D], int t,
                                            s2 NEG, s2 MAT, u2 EXT, u2 OPN, u2 vm) {
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    uint32_t nzb = ((d < 16 ? mlo : mhi) >> (d & 15)) & 0x00010001u;
+    uint32_t nzb = (P[d] >> t) & 0x00010001u;
     s2 nz = __builtin_bit_cast(s2, nzb);
     s2 Dv = M[d] + nz * NEG + MAT;                         // corner + (match ? m : -x)
     u2 I;
@@ -68,19 +70,27 @@ __device__ __forceinline__ void filter_pair(const GenomeDev& g, const uint32_t* 
     gbit_pair(g.lh, bb, B0l, B0h);
     gbit_pair(g.lh, bb + 32, B1l, B1h);
     const uint32_t ral = pa[blk], rah = pa[RW + blk], rbl = pb[blk], rbh = pb[RW + blk];
-    const int rows = (Lmax - 32 * blk) < 32 ? (Lmax - 32 * blk) : 32;
-    for (int t = 0; t < rows; ++t) {
-      const uint32_t gal = __builtin_amdgcn_alignbit(A1l, A0l, t), gah = __builtin_amdgcn_alignbit(A1h, A0h, t);
-      const uint32_t gbl = __builtin_amdgcn_alignbit(B1l, B0l, t), gbh = __builtin_amdgcn_alignbit(B1h, B0h, t);
-      const uint32_t sal = (uint32_t)__builtin_amdgcn_sbfe((int)ral, t, 1), sah = (uint32_t)__builtin_amdgcn_sbfe((int)rah, t, 1);
-      const uint32_t sbl = (uint32_t)__builtin_amdgcn_sbfe((int)rbl, t, 1), sbh = (uint32_t)__builtin_amdgcn_sbfe((int)rbh, t, 1);
-      const uint32_t ma = (gal ^ sal) | (gah ^ sah);      // bit delta = mismatch on diagonal delta
-      const uint32_t mb = (gbl ^ sbl) | (gbh ^ sbh);
-      const uint32_t mlo = __builtin_amdgcn_perm(mb, ma, 0x05040100u);   // [ma.lo16 | mb.lo16]
-      const uint32_t mhi = (D > 16) ? __builtin_amdgcn_perm(mb, ma, 0x07060302u) : 0u;
-      if (32 * blk + t < Lmin) filter_row<D, false>(M, G, best, mlo, mhi, k.NEG, k.MAT, k.EXT, k.OPN, vm);
-      else filter_row<D, true>(M, G, best, mlo, mhi, k.NEG, k.MAT, k.EXT, k.OPN, vm);
+    // mismatch bits of the block's 32 rows per diagonal: bit t of diagonal d compares read code
+    // 32*blk + t with genome code ba + t + d (one funnel shift of each plane per diagonal, not
+    // per row); Q holds rows 0-15 and R rows 16-31, candidate a in the low half, b in the high
+    uint32_t Q[D], R[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const uint32_t ma = (__builtin_amdgcn_alignbit(A1l, A0l, d) ^ ral) | (__builtin_amdgcn_alignbit(A1h, A0h, d) ^ rah);
+      const uint32_t mb = (__builtin_amdgcn_alignbit(B1l, B0l, d) ^ rbl) | (__builtin_amdgcn_alignbit(B1h, B0h, d) ^ rbh);
+      Q[d] = __builtin_amdgcn_perm(mb, ma, 0x05040100u);
+      R[d] = __builtin_amdgcn_perm(mb, ma, 0x07060302u);
     }
+    // each half: its rows below both lengths, then the rows only the longer candidate has (its
+    // partner's maximum frozen), as two loops, so no row branches between the two forms
+    const int r0 = 32 * blk;
+    const int rows = (Lmax - r0) < 32 ? (Lmax - r0) : 32;
+    const int n0 = rows < 16 ? rows : 16, n1 = rows - n0;
+    const int u0 = min(max(Lmin - r0, 0), n0), u1 = min(max(Lmin - r0 - 16, 0), n1);
+    for (int t = 0; t < u0; ++t) filter_row<D, false>(M, G, best, Q, t, k.NEG, k.MAT, k.EXT, k.OPN, vm);
+    for (int t = u0; t < n0; ++t) filter_row<D, true>(M, G, best, Q, t, k.NEG, k.MAT, k.EXT, k.OPN, vm);
+    for (int t = 0; t < u1; ++t) filter_row<D, false>(M, G, best, R, t, k.NEG, k.MAT, k.EXT, k.OPN, vm);
+    for (int t = u1; t < n1; ++t) filter_row<D, true>(M, G, best, R, t, k.NEG, k.MAT, k.EXT, k.OPN, vm);
   }
   sa = best.x > 255 ? 255 : best.x;
   sb = best.y > 255 ? 255 : best.y;
