@@ -47,7 +47,7 @@ VALU_PEAK_PK16 = 78.6        # T 16-bit ops/s with packed VALU instructions (2 p
 HBM_PEAK_GBPS = 8000.0
 # VALU lane-instructions k_filter<8> issues per cell at L=75 (SQ_INSTS_VALU x 64 / cells, rocprofv3 --pmc,
 # profiles/k_filter_W8_L75_10Mcand.md); the kernel's own work measure for the VALU-issue roofline
-VALU_INSTR_PER_CELL_W8_L75 = 5.438
+VALU_INSTR_PER_CELL_W8_L75 = 5.254
 # VALU lane-instructions k_align<7,false> issues per DP cell (L x 15 band columns) on the DP stage
 # workload (one pass per record with directions, the walk of every accepted record), measured the
 # same way (profiles/k_align_I7_L75_2M_records.md)
@@ -55,7 +55,7 @@ VALU_INSTR_PER_DP_CELL_I7_L75 = 37.75
 # HBM bytes per candidate of the same dispatch: (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00) x 1024 /
 # candidates, the factors calibrated on a device-to-device copy of known size in the same run
 # (profiles/k_filter_W8_L75_10Mcand.md)
-HBM_BYTES_PER_CAND_W8_L75 = 45.6
+HBM_BYTES_PER_CAND_W8_L75 = 45.1
 # HBM bytes fetched per index lookup by k_seed_count (FETCH_SIZE x 2.00 x 1024 / lookups, the factor
 # calibrated on a device-to-device copy in the same run; profiles/k_seed_count_lookups.md): each
 # random first[key] read takes a 128-byte line
