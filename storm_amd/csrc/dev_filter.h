@@ -10,21 +10,35 @@ typedef unsigned short u2 __attribute__((ext_vector_type(2)));
 
 // One row of a candidate pair: P[d] holds, for diagonal d, the mismatch bits of 16 consecutive
 // rows of candidate a (low half) and candidate b (high half); this row is bit t of each half.
-template <int D, bool MASKED>
+// FAST (gap open - gap extend == match, gap open >= gap extend: the default scores): with
+// X = corner + m - (m + x) * mismatch - (open - ext) = corner - (m + x) * mismatch, the gap state
+// max(subs(I, ext), subs(M, open)) is subs(max(I, X), ext) and the cell's score X + (open - ext),
+// one instruction fewer per cell pair (M = max(X + open - ext, I) >= I, so subs(M, open) adds
+// nothing to max(I - ext, X + open - ext - open)).
+template <int D, bool MASKED, bool FAST>
 __device__ __forceinline__ void filter_row(s2 (&M)[D], u2 (&G)[D], u2& best, const uint32_t (&P)[D], int t,
                                            s2 NEG, s2 MAT, u2 EXT, u2 OPN, u2 vm) {
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     uint32_t nzb = (P[d] >> t) & 0x00010001u;
     s2 nz = __builtin_bit_cast(s2, nzb);
-    s2 Dv = M[d] + nz * NEG + MAT;                         // corner + (match ? m : -x)
     u2 I;
     if (d == 0) I = G[1];
     else if (d == D - 1) I = G[D - 2];
     else I = __builtin_elementwise_max(G[d + 1], G[d - 1]);   // top (old row), left (new row)
-    s2 Mv = __builtin_elementwise_max(Dv, __builtin_bit_cast(s2, I));
-    u2 Mu = __builtin_bit_cast(u2, Mv);
-    G[d] = __builtin_elementwise_max(__builtin_elementwise_sub_sat(I, EXT), __builtin_elementwise_sub_sat(Mu, OPN));
+    const s2 Is = __builtin_bit_cast(s2, I);
+    s2 Mv;
+    if (FAST) {
+      const s2 X = M[d] + nz * NEG;                         // corner + (match ? m : -x) - (open - ext)
+      Mv = __builtin_elementwise_max(X + MAT, Is);
+      G[d] = __builtin_elementwise_sub_sat(__builtin_bit_cast(u2, __builtin_elementwise_max(Is, X)), EXT);
+    } else {
+      const s2 Dv = M[d] + nz * NEG + MAT;                  // corner + (match ? m : -x)
+      Mv = __builtin_elementwise_max(Dv, Is);
+      G[d] = __builtin_elementwise_max(__builtin_elementwise_sub_sat(I, EXT),
+                                       __builtin_elementwise_sub_sat(__builtin_bit_cast(u2, Mv), OPN));
+    }
+    const u2 Mu = __builtin_bit_cast(u2, Mv);
     M[d] = Mv;
     if (MASKED) best = __builtin_elementwise_max(best, Mu & vm);
     else best = __builtin_elementwise_max(best, Mu);
@@ -36,6 +50,7 @@ struct FilterConst {
   s2 NEG, MAT;
   u2 EXT, OPN;
   int thr;
+  bool fast;   // open - ext == m and open >= ext: filter_row's FAST form
 };
 
 __device__ __forceinline__ FilterConst filter_const(int mt, int mm, int op, int ex, int thr) {
@@ -45,14 +60,15 @@ __device__ __forceinline__ FilterConst filter_const(int mt, int mm, int op, int 
   k.EXT = (u2){(unsigned short)ex, (unsigned short)ex};
   k.OPN = (u2){(unsigned short)op, (unsigned short)op};
   k.thr = thr;
+  k.fast = (op - ex == mt) && op >= ex;
   return k;
 }
 
 // Two candidates: read planes pa/pb of lengths La/Lb, windows starting at genome codes xa/xb
 // (already offset by GPAD).  Returns the reference's u8 running maxima in sa/sb.
-template <int W>
-__device__ __forceinline__ void filter_pair(const GenomeDev& g, const uint32_t* pa, const uint32_t* pb, int La, int Lb,
-                                            int64_t xa, int64_t xb, const FilterConst& k, int& sa, int& sb) {
+template <int W, bool FAST>
+__device__ __forceinline__ void filter_pair_f(const GenomeDev& g, const uint32_t* pa, const uint32_t* pb, int La, int Lb,
+                                              int64_t xa, int64_t xb, const FilterConst& k, int& sa, int& sb) {
   constexpr int D = 2 * W;
   const int Lmin = La < Lb ? La : Lb, Lmax = La < Lb ? Lb : La;
   const u2 vm = {(unsigned short)(La > Lb ? 0xFFFF : 0), (unsigned short)(Lb > La ? 0xFFFF : 0)};
@@ -87,13 +103,20 @@ __device__ __forceinline__ void filter_pair(const GenomeDev& g, const uint32_t* 
     const int rows = (Lmax - r0) < 32 ? (Lmax - r0) : 32;
     const int n0 = rows < 16 ? rows : 16, n1 = rows - n0;
     const int u0 = min(max(Lmin - r0, 0), n0), u1 = min(max(Lmin - r0 - 16, 0), n1);
-    for (int t = 0; t < u0; ++t) filter_row<D, false>(M, G, best, Q, t, k.NEG, k.MAT, k.EXT, k.OPN, vm);
-    for (int t = u0; t < n0; ++t) filter_row<D, true>(M, G, best, Q, t, k.NEG, k.MAT, k.EXT, k.OPN, vm);
-    for (int t = 0; t < u1; ++t) filter_row<D, false>(M, G, best, R, t, k.NEG, k.MAT, k.EXT, k.OPN, vm);
-    for (int t = u1; t < n1; ++t) filter_row<D, true>(M, G, best, R, t, k.NEG, k.MAT, k.EXT, k.OPN, vm);
+    for (int t = 0; t < u0; ++t) filter_row<D, false, FAST>(M, G, best, Q, t, k.NEG, k.MAT, k.EXT, k.OPN, vm);
+    for (int t = u0; t < n0; ++t) filter_row<D, true, FAST>(M, G, best, Q, t, k.NEG, k.MAT, k.EXT, k.OPN, vm);
+    for (int t = 0; t < u1; ++t) filter_row<D, false, FAST>(M, G, best, R, t, k.NEG, k.MAT, k.EXT, k.OPN, vm);
+    for (int t = u1; t < n1; ++t) filter_row<D, true, FAST>(M, G, best, R, t, k.NEG, k.MAT, k.EXT, k.OPN, vm);
   }
   sa = best.x > 255 ? 255 : best.x;
   sb = best.y > 255 ? 255 : best.y;
+}
+
+template <int W>
+__device__ __forceinline__ void filter_pair(const GenomeDev& g, const uint32_t* pa, const uint32_t* pb, int La, int Lb,
+                                            int64_t xa, int64_t xb, const FilterConst& k, int& sa, int& sb) {
+  if (k.fast) filter_pair_f<W, true>(g, pa, pb, La, Lb, xa, xb, k, sa, sb);
+  else filter_pair_f<W, false>(g, pa, pb, La, Lb, xa, xb, k, sa, sb);
 }
 
 }  // namespace storm

@@ -8,15 +8,19 @@
 //
 // gfx950 mapping.  Each lane owns TWO candidates, one per 16-bit half of every VGPR, and walks
 // their rows; the 2W diagonals of a row are unrolled in registers, so no cross-lane traffic is
-// needed.  Per pair of cells the chain is ~10 packed VALU ops (v_pk_mad_u16, v_pk_add_u16,
-// v_pk_max_u16/i16, v_pk_sub_u16 clamp).  Two rewrites keep it short and exact:
+// needed.  Per pair of cells the chain is 9-10 packed VALU ops (v_pk_mad_u16, v_pk_add_u16,
+// v_pk_max_u16/i16, v_pk_sub_u16 clamp).  Three rewrites keep it short and exact:
 //   * the gap state is carried as G = max(subs(I, ext), subs(M, open)) per cell, so a cell's
 //     I is max(G_top, G_left) -- one op instead of four;
+//   * when open - ext == match and open >= ext (the default scores), G = subs(max(I, X), ext)
+//     with X = corner - (match + mismatch) * [mismatch], the cell's score being X + match: one
+//     op fewer (dev_filter.h, filter_row FAST);
 //   * the upper u8 clamp is dropped: scores are kept unclamped in 16 bits and the final
 //     maximum is clamped to 255.  The first cell that would saturate makes the clamped maximum
 //     255 and every cell before it is unchanged, so min(255, max) is the reference's result.
-// Read and genome codes arrive as bit planes; a row's 2W match/mismatch bits for one candidate
-// cost 7 ops (two funnel shifts of the genome planes, two sign-extended read bits, xor/xor/or).
+// Read and genome codes arrive as bit planes; the mismatch bits of 32 rows of one diagonal come
+// from one funnel shift of each genome plane and an xor with the read's planes, packed 16 rows of
+// both candidates per register, so a cell takes its bit with a shift and a mask.
 #include "dev_filter.h"
 #include "storm_kernels.h"
 

@@ -97,3 +97,37 @@ def test_filter_mixed_lengths_in_one_launch(oracle_built):
         want.append(min(255, ol.orc_filter_score(C.c_void_p(gp.ctypes.data + pad), int(cp[c]), C.c_void_p(rpk.ctypes.data),
                                                  int(lens[cr[c]]), W, 5, -4, -9, -4)))
     assert np.array_equal(s.astype(np.int64), np.array(want))
+
+
+@pytest.mark.parametrize("scores", [(5, -4, -9, -4), (4, -3, -7, -2), (3, -2, -5, -2), (2, -5, -3, -4)],
+                         ids=["default", "open-ext!=m", "open-ext=m", "open<ext"])
+def test_filter_score_sets_vs_oracle(oracle_built, scores):
+    """Both forms of the filter cell against the CPU restatement: the short form taken when
+    open - extend == match and open >= extend (the default scores, and 3/2/5/2), and the general
+    form otherwise (4/3/7/2, and 2/5/3/4 with open < extend), every band class."""
+    from storm_amd import lib, synth
+    m, x, o, e = scores
+    ol = C.CDLL(os.path.join(oracle_built, "liboracle.so"))
+    ol.orc_filter_score.restype = C.c_int
+    ol.orc_filter_score.argtypes = [C.c_void_p, C.c_long, C.c_void_p, C.c_int, C.c_int] + [C.c_int] * 4
+    g = synth.random_genome(100_000, 21)
+    pad = 64
+    gp = np.zeros(pad + (len(g) + 3) // 4 + pad, np.uint8)
+    gp[pad:pad + (len(g) + 3) // 4] = synth.pack_codes(g)
+    for W in (2, 4, 8, 16):
+        n_reads, L, d = 600, 60 + 3 * W, min(2 * W - 1, 15)
+        reads, pos = synth.filter_candidates(g, n_reads, L, W, d, seed=22 + W)
+        plan = lib.FilterPlan(g, list(reads), np.full(n_reads, L), np.repeat(np.arange(n_reads), 2), pos.reshape(-1), W,
+                              match=m, mismatch=x, gap_open=o, gap_extend=e)
+        plan.run(1)
+        s, _ = plan.fetch()
+        plan.close()
+        stride = (L + 3) // 4 + 8
+        rp = np.zeros(n_reads * stride + 64, np.uint8)
+        for k in range(n_reads):
+            rp[k * stride:k * stride + (L + 3) // 4] = synth.pack_codes(reads[k])
+        sel = np.arange(0, 2 * n_reads, 3)
+        want = np.array([min(255, ol.orc_filter_score(C.c_void_p(gp.ctypes.data + pad), int(pos.reshape(-1)[c]),
+                                                      C.c_void_p(rp.ctypes.data + int(c // 2) * stride), L, W, m, x, o, e))
+                         for c in sel])
+        assert np.array_equal(s[sel].astype(np.int64), want), f"W={W}"
