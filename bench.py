@@ -47,7 +47,7 @@ VALU_PEAK_PK16 = 78.6        # T 16-bit ops/s with packed VALU instructions (2 p
 HBM_PEAK_GBPS = 8000.0
 # VALU lane-instructions k_filter<8> issues per cell at L=75 (SQ_INSTS_VALU x 64 / cells, rocprofv3 --pmc,
 # profiles/k_filter_W8_L75_10Mcand.md); the kernel's own work measure for the VALU-issue roofline
-VALU_INSTR_PER_CELL_W8_L75 = 5.254
+VALU_INSTR_PER_CELL_W8_L75 = 4.755
 # VALU lane-instructions k_align<7,false> issues per DP cell (L x 15 band columns) on the DP stage
 # workload (one pass per record with directions, the walk of every accepted record), measured the
 # same way (profiles/k_align_I7_L75_2M_records.md)
