@@ -130,7 +130,7 @@ static SeedSetDev seed_set(const RefState& rs) {
 // reports whether that order is the view's own.
 // Longest reads first (stable within a length): the persistent kernel hands out the costliest
 // tasks first and the short ones fill the tail, and equal lengths stay contiguous for the plain path.
-void length_order(const storm_reads_view* v, std::vector<int64_t>& order, bool& identity) {
+void length_order(const storm_reads_view* v, std::vector<int64_t>& order, bool& identity, std::vector<int64_t>* hist) {
   const int64_t n = v->n_reads;
   auto bucket = [&](int64_t i) { return (size_t)(MAXL + 1 - std::min(std::max((int)v->len[i], 0), MAXL + 1)); };
   std::vector<int64_t> start(MAXL + 3, 0);
@@ -138,6 +138,10 @@ void length_order(const storm_reads_view* v, std::vector<int64_t>& order, bool& 
   for (int64_t i = 0; i < n; ++i) {
     start[bucket(i) + 1]++;
     if (i && v->len[i] > v->len[i - 1]) identity = false;
+  }
+  if (hist) {   // bucket b holds length MAXL + 1 - b
+    hist->assign(MAXL + 2, 0);
+    for (int L = 0; L <= MAXL + 1; ++L) (*hist)[(size_t)L] = start[(size_t)(MAXL + 1 - L) + 1];
   }
   for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
   order.resize((size_t)n);
@@ -208,19 +212,27 @@ static int staged_copy(storm_mapper* m, const std::vector<CopySeg>& segs, bool t
   return STORM_OK;
 }
 
-int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order, DevReads& d) {
+int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order, DevReads& d,
+                 const std::vector<int64_t>* hist, const bool* identity_in) {
   const int64_t n = v->n_reads;
   if ((int64_t)order.size() != n) return fail(STORM_ERR_INVALID, "read order does not cover the view");
   const int w = filter_lanes(m->p.allowed_indels >= 16 ? 15 : m->p.allowed_indels);
-  bool identity = true;
-  for (int64_t i = 0; i < n; ++i) {
-    const int L = v->len[i];
-    if (L <= 0 || L > MAXL)
-      return fail(STORM_ERR_INVALID, "read %lld has length %d (1..%d supported)", (long long)i, L, MAXL);
-    if (L < w)
-      return fail(STORM_ERR_INVALID, "read %lld is shorter (%d) than the filter band lanes (%d)", (long long)i, L, w);
-    if (order[(size_t)i] != i) identity = false;
-  }
+  // lengths outside [w, MAXL] are rejected; with the call's length histogram the reads are only
+  // scanned (for the message) when one is
+  bool bad = !hist;
+  if (hist)
+    for (int L = 0; L <= MAXL + 1; ++L)
+      if ((*hist)[(size_t)L] && (L <= 0 || L > MAXL || L < w)) bad = true;
+  bool identity = identity_in ? *identity_in : true;
+  if (bad || !identity_in)
+    for (int64_t i = 0; i < n; ++i) {
+      const int L = v->len[i];
+      if (L <= 0 || L > MAXL)
+        return fail(STORM_ERR_INVALID, "read %lld has length %d (1..%d supported)", (long long)i, L, MAXL);
+      if (L < w)
+        return fail(STORM_ERR_INVALID, "read %lld is shorter (%d) than the filter band lanes (%d)", (long long)i, L, w);
+      if (order[(size_t)i] != i) identity = false;
+    }
   d.n = n;
   d.has_qual = v->qual != nullptr;
   const int64_t cs = v->code_stride, qs = v->qual ? v->qual_stride : 0;
@@ -638,7 +650,7 @@ static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode,
   // order groups reads by length, stable within a length
   std::vector<int64_t> order;
   bool sorted_view = true;
-  length_order(v, order, sorted_view);
+  length_order(v, order, sorted_view, &m->len_hist);
   std::vector<std::pair<int64_t, int64_t>> batches;
   if (mode == STORM_MODE_PLAIN) {
     int64_t s = 0;
@@ -649,7 +661,7 @@ static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode,
   }
   const double t0 = now_s();
   DevReads& dr = m->plain_reads;   // kept across calls: a call allocates only when it outgrows them
-  int rc = upload_reads(m, v, order, dr);
+  int rc = upload_reads(m, v, order, dr, &m->len_hist, &sorted_view);
   if (rc) return rc;
   DevMem& slots = m->plain_slots;
   HIPOK(slots.ensure((size_t)(4 * n) * sizeof(Slot)));
@@ -970,12 +982,13 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
   // device order: by length (tasks then hold reads of similar length)
   std::vector<int64_t> order;
   bool sorted_view = true;
-  length_order(v, order, sorted_view);
+  length_order(v, order, sorted_view, &m->len_hist);
   int maxL = 0;
-  for (int64_t i = 0; i < n; ++i) maxL = std::max(maxL, (int)v->len[i]);
+  for (int L = MAXL + 1; L > 0 && !maxL; --L)
+    if (m->len_hist[(size_t)L]) maxL = L;
   const double t0 = now_s();
   const double ti = m->res->index_seconds;
-  int rc = upload_reads(m, v, order, F.reads);
+  int rc = upload_reads(m, v, order, F.reads, &m->len_hist, &sorted_view);
   if (rc) return rc;
   PassStats ps;
   unsigned long long totals[4] = {0, 0, 0, 0};
@@ -1027,12 +1040,17 @@ int load_resident(const char* genome_path, const storm_params& p, std::shared_pt
   return STORM_OK;
 }
 
+// index lookups of a call, from its length histogram (length_order)
 static int64_t count_lookups(const storm_mapper* m, const storm_reads_view* v) {
+  (void)v;
   const int strands = ((m->p.sense_mask & 1) ? 1 : 0) + ((m->p.sense_mask & 2) ? 1 : 0);
   int64_t n = 0;
-  for (int64_t i = 0; i < v->n_reads; ++i)
+  for (int L = 1; L <= MAXL && L < (int)m->len_hist.size(); ++L) {
+    const int64_t c = m->len_hist[(size_t)L];
+    if (!c) continue;
     for (const SeedSpec& s : m->res->seeds)
-      n += s.pos.empty() ? std::max<int64_t>(0, v->len[i] - s.span + 1) : (int64_t)s.pos.size();
+      n += c * (s.pos.empty() ? std::max<int64_t>(0, L - s.span + 1) : (int64_t)s.pos.size());
+  }
   return n * strands * (int64_t)m->res->refs.size();
 }
 

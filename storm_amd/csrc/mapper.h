@@ -230,6 +230,7 @@ struct storm_mapper {
   storm::DevReads plain_reads; // the plain path's reads in device order, reused across calls
   storm::DevMem plain_slots;   // the plain path's hit map slots, reused across calls
   storm::DevMem plain_cells;   // the plain path's traceback cells (4 per read), reused across calls
+  std::vector<int64_t> len_hist;   // reads per length of the current call (length_order)
   storm::DevMem rec_dev, tb_dev;   // the hit map and tracebacks in the caller's order, on their way out
   // SAM records formatted on the device: the caller's read names, the MAPQ table of the scores,
   // caller row -> device row, per-row byte counts and offsets, the text and its host copy
@@ -289,7 +290,8 @@ int align_plan_view(storm_align_plan* pl, AlignArgs& a, int& maxL, int& device, 
 int64_t align_scratch_bytes(int I, int maxL, int64_t n_rec);
 int prep_device_reads(storm_mapper* m, int64_t n, int64_t code_stride, int64_t qual_stride, bool has_qual,
                       bool has_first_qual, DevReads& d);
-int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order, DevReads& d);
+int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order, DevReads& d,
+                 const std::vector<int64_t>* hist = nullptr, const bool* identity = nullptr);
 // one staged pass over reads [r0, r1) of dr; PASS_SPLIT (nothing written) when its raw index hits
 // exceed PASS_HIT_LIMIT
 constexpr int PASS_SPLIT = 1;
@@ -360,7 +362,11 @@ struct RecordsOut {
   storm_hit* hits;
   uint8_t* tb;
 };
-void length_order(const storm_reads_view* v, std::vector<int64_t>& order, bool& identity);
+// The view's reads ordered by length, longest first, stable (a counting sort); identity = the
+// view is already in that order; hist (MAXL + 2 entries), when given, receives the number of reads
+// of each length 0 .. MAXL + 1 (lengths outside that range counted at its ends).
+void length_order(const storm_reads_view* v, std::vector<int64_t>& order, bool& identity,
+                  std::vector<int64_t>* hist = nullptr);
 // ho != null: the slots go straight to the caller's arrays (no host slot assembly, no tracebacks);
 // ro != null: the records and tracebacks go to the caller's arrays instead of out_slots
 int map_view(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
