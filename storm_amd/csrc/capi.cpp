@@ -784,9 +784,17 @@ static int tune_fused(storm_mapper* m, int64_t n, int maxL, int max_grid, FusedT
   const int64_t sample = std::min<int64_t>(n, 32768);
   const int32_t ns = (int32_t)sample;
   HIPOK(hipMemcpyAsync(F.n_dev.p, &ns, 4, hipMemcpyHostToDevice, m->st));
-  hipEvent_t e0, e1;
-  HIPOK(hipEventCreate(&e0));
-  HIPOK(hipEventCreate(&e1));
+  // the two timing events, destroyed on every return
+  struct Events {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~Events() {
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } ev;
+  HIPOK(hipEventCreate(&ev.e[0]));
+  HIPOK(hipEventCreate(&ev.e[1]));
+  hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
   FusedTune best;
   best.ms = 1e30f;
   {   // warm-up launch (code object load, caches)
@@ -824,8 +832,6 @@ static int tune_fused(storm_mapper* m, int64_t n, int maxL, int max_grid, FusedT
     int rc = timed(best.group, max_grid / 2, ms);
     if (rc) return rc;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   HIPOK(launch_slots_init(F.slots.as<Slot>(), 4 * sample, m->st));
   F.tune[key] = best;
   if (m->shared_tune) {
@@ -928,9 +934,9 @@ int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback,
     HIPOK(hipMemcpyAsync(ph0, F.phase.p, sizeof ph0, hipMemcpyDeviceToHost, m->st));
     HIPOK(hipStreamSynchronize(m->st));
 #endif
-    hipEvent_t e0, e1;
-    HIPOK(hipEventCreate(&e0));
-    HIPOK(hipEventCreate(&e1));
+    if (!m->ev0) HIPOK(hipEventCreate(&m->ev0));
+    if (!m->ev1) HIPOK(hipEventCreate(&m->ev1));
+    hipEvent_t e0 = m->ev0, e1 = m->ev1;
     HIPOK(hipEventRecord(e0, m->st));
     HIPOK(hipGraphLaunch(exec, m->st));
     HIPOK(hipEventRecord(e1, m->st));
@@ -940,8 +946,6 @@ int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback,
     HIPOK(hipStreamSynchronize(m->st));
     float ms = 0;
     (void)hipEventElapsedTime(&ms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     ps.ms_filter += ms;   // the fused kernel carries every stage
 #ifdef STORM_FUSED_PHASES
     {   // wavefront-time shares of the launch's phases (constant 100 MHz clock)
@@ -1132,8 +1136,6 @@ void storm_mapper_destroy(storm_mapper* m) {
   (void)hipSetDevice(m->device);
   m->fs.drop_graphs();
   m->ps = PassScratch();
-  for (hipEvent_t& e : m->stage_ev)
-    if (e) (void)hipEventDestroy(e);
   m->res.reset();
   if (m->st) (void)hipStreamDestroy(m->st);
   delete m;

@@ -224,7 +224,7 @@ struct storm_mapper {
   std::string unmapped_str;    // the -O path p.unmapped_path points to (empty: none)
   std::shared_ptr<storm::Resident> res;
   storm::PassScratch ps;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;   // the fused launch's timing events (made on first use)
   storm::FusedState fs;
   storm::DevReads stage;       // the caller's reads as uploaded, before the device reordering
   storm::DevReads plain_reads; // the plain path's reads in device order, reused across calls
@@ -246,6 +246,13 @@ struct storm_mapper {
   storm::DevMem hits_dev;      // per-slot results on their way to the caller
   storm::SharedTune* shared_tune = nullptr;   // tuner results of sibling contexts, or null
   bool want_tb = true;   // false while a call returns hits only: tracebacks stay on the device
+  storm_mapper() = default;
+  storm_mapper(const storm_mapper&) = delete;
+  storm_mapper& operator=(const storm_mapper&) = delete;
+  ~storm_mapper() {
+    for (hipEvent_t e : {ev0, ev1, stage_ev[0], stage_ev[1]})
+      if (e) (void)hipEventDestroy(e);
+  }
 };
 
 namespace storm {
