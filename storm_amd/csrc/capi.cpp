@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstddef>
 #include <chrono>
 #include <condition_variable>
@@ -130,6 +131,31 @@ static SeedSetDev seed_set(const RefState& rs) {
 // reports whether that order is the view's own.
 // Longest reads first (stable within a length): the persistent kernel hands out the costliest
 // tasks first and the short ones fill the tail, and equal lengths stay contiguous for the plain path.
+// Device order of a chunked call: the reads of each chunk [bounds[k], bounds[k + 1]) of the view
+// by length within the chunk (longest first, stable), so that chunk k's device rows are its own
+// reads; hist counts the reads per length over the whole view, identity tells whether the order
+// is the view's own.
+static void length_order_chunks(const storm_reads_view* v, const std::vector<int64_t>& bounds, std::vector<int64_t>& order,
+                                bool& identity, std::vector<int64_t>& hist) {
+  const int64_t n = v->n_reads;
+  auto bucket = [&](int64_t i) { return (size_t)(MAXL + 1 - std::min(std::max((int)v->len[i], 0), MAXL + 1)); };
+  order.resize((size_t)n);
+  hist.assign(MAXL + 2, 0);
+  identity = true;
+  std::vector<int64_t> start(MAXL + 3);
+  for (size_t k = 0; k + 1 < bounds.size(); ++k) {
+    std::fill(start.begin(), start.end(), 0);
+    for (int64_t i = bounds[k]; i < bounds[k + 1]; ++i) {
+      start[bucket(i) + 1]++;
+      if (i > bounds[k] && v->len[i] > v->len[i - 1]) identity = false;
+    }
+    for (int L = 0; L <= MAXL + 1; ++L) hist[(size_t)L] += start[(size_t)(MAXL + 1 - L) + 1];
+    start[0] = bounds[k];
+    for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
+    for (int64_t i = bounds[k]; i < bounds[k + 1]; ++i) order[(size_t)start[bucket(i)]++] = i;
+  }
+}
+
 void length_order(const storm_reads_view* v, std::vector<int64_t>& order, bool& identity, std::vector<int64_t>* hist) {
   const int64_t n = v->n_reads;
   auto bucket = [&](int64_t i) { return (size_t)(MAXL + 1 - std::min(std::max((int)v->len[i], 0), MAXL + 1)); };
@@ -171,17 +197,20 @@ struct CopySeg {
   size_t bytes;
 };
 
-// Copies between the caller's (pageable) host memory and device memory on the mapper's stream at
-// pinned-copy speed: the segments are cut into pieces of up to 8 MiB that pass through two pinned
+// Copies between the caller's (pageable) host memory and device memory on stream st (default: the
+// mapper's) at pinned-copy speed: the segments are cut into pieces of up to 8 MiB that pass through two pinned
 // halves in turn, and the host threads copying one piece overlap the DMA of the next.
 // to_host: device -> host, returns when every dst holds its bytes; else host -> device, returns
 // once every byte is in pinned memory or on the device (the DMA completes in stream order; the
 // halves are not reused before their events say so).
-static int staged_copy(storm_mapper* m, const std::vector<CopySeg>& segs, bool to_host) {
+static int staged_copy(storm_mapper* m, const std::vector<CopySeg>& segs, bool to_host, hipStream_t st = nullptr) {
+  if (!st) st = m->st;
   const size_t CH = (size_t)8 << 20;
   HIPOK(m->out_stage.ensure(2 * CH));
-  for (int k = 0; k < 2; ++k)
+  for (int k = 0; k < 2; ++k) {
     if (!m->stage_ev[k]) HIPOK(hipEventCreateWithFlags(&m->stage_ev[k], hipEventDisableTiming));
+    HIPOK(hipStreamWaitEvent(st, m->stage_ev[k], 0));   // the half's last copy, on whichever stream
+  }
   struct Piece {
     char* dst;
     const char* src;
@@ -195,13 +224,13 @@ static int staged_copy(storm_mapper* m, const std::vector<CopySeg>& segs, bool t
     if (c < ps.size()) {
       char* half = (char*)m->out_stage.p + (c & 1) * CH;
       if (to_host) {   // piece c into half c & 1 (its previous piece, c - 2, was copied out already)
-        HIPOK(hipMemcpyAsync(half, ps[c].src, ps[c].len, hipMemcpyDeviceToHost, m->st));
+        HIPOK(hipMemcpyAsync(half, ps[c].src, ps[c].len, hipMemcpyDeviceToHost, st));
       } else {         // the half's previous DMA has read it; fill it and send it
         HIPOK(hipEventSynchronize(m->stage_ev[c & 1]));
         host_copy(half, ps[c].src, ps[c].len);
-        HIPOK(hipMemcpyAsync(ps[c].dst, half, ps[c].len, hipMemcpyHostToDevice, m->st));
+        HIPOK(hipMemcpyAsync(ps[c].dst, half, ps[c].len, hipMemcpyHostToDevice, st));
       }
-      HIPOK(hipEventRecord(m->stage_ev[c & 1], m->st));
+      HIPOK(hipEventRecord(m->stage_ev[c & 1], st));
     }
     if (to_host && c >= 1) {   // piece c - 1 out of its half while piece c is in flight
       const size_t q = c - 1;
@@ -609,7 +638,7 @@ int finish_view(storm_mapper* m, const Slot* slots, const uint8_t* cells, const 
   }
   HIPOK(m->rec_dev.ensure((size_t)(4 * n) * sizeof(HitRec)));
   if (want_tb) HIPOK(m->tb_dev.ensure((size_t)(n * M) * TBB));
-  HIPOK(launch_hit_records(slots, identity ? nullptr : m->order_dev.as<int32_t>(), n, M, cells, m->rec_dev.as<HitRec>(),
+  HIPOK(launch_hit_records(slots, identity ? nullptr : m->order_dev.as<int32_t>(), 0, n, M, cells, m->rec_dev.as<HitRec>(),
                            want_tb ? m->tb_dev.as<uint8_t>() : nullptr, m->st));
   std::vector<storm_hit> hv;
   std::vector<uint8_t> tv;
@@ -765,25 +794,37 @@ static FusedArgs fused_args(storm_mapper* m, size_t ref, int group) {
 
 // Autotuner: reads per task and persistent grid for this band class and length bucket, timed on
 // a sample of the batch (the sample's slots are reset afterwards).
-static int tune_fused(storm_mapper* m, int64_t n, int maxL, int max_grid, FusedTune& out) {
-  FusedState& F = m->fs;
+// the tuner's key: band class and 32-nt length bucket of the longest read
+static int64_t tune_key(const storm_mapper* m, int maxL) {
   ScoreDev sc{};
   fill_score(m->p, sc);
-  const int64_t key = ((int64_t)sc.W << 40) | ((int64_t)sc.I << 32) | ((int64_t)m->p.color << 24) | (int64_t)((maxL + 31) / 32);
+  return ((int64_t)sc.W << 40) | ((int64_t)sc.I << 32) | ((int64_t)m->p.color << 24) | (int64_t)((maxL + 31) / 32);
+}
+
+// a tuning of this context or, on the same device, of a sibling context
+static bool cached_tune(storm_mapper* m, int64_t key, FusedTune& out) {
+  FusedState& F = m->fs;
   auto it = F.tune.find(key);
-  if (it != F.tune.end()) { out = it->second; return STORM_OK; }
-  if (m->shared_tune) {   // another context on the same device tuned this class already
+  if (it != F.tune.end()) { out = it->second; return true; }
+  if (m->shared_tune) {
     std::lock_guard<std::mutex> lk(m->shared_tune->mu);
     auto s = m->shared_tune->best.find(key);
     if (s != m->shared_tune->best.end()) {
       F.tune[key] = s->second;
       out = s->second;
-      return STORM_OK;
+      return true;
     }
   }
+  return false;
+}
+
+static int tune_fused(storm_mapper* m, int64_t n, int maxL, int max_grid, FusedTune& out) {
+  FusedState& F = m->fs;
+  const int64_t key = tune_key(m, maxL);
+  if (cached_tune(m, key, out)) return STORM_OK;
   const int64_t sample = std::min<int64_t>(n, 32768);
-  const int32_t ns = (int32_t)sample;
-  HIPOK(hipMemcpyAsync(F.n_dev.p, &ns, 4, hipMemcpyHostToDevice, m->st));
+  const int32_t ns[2] = {(int32_t)sample, 0};
+  HIPOK(hipMemcpyAsync(F.n_dev.p, ns, 8, hipMemcpyHostToDevice, m->st));
   // the two timing events, destroyed on every return
   struct Events {
     hipEvent_t e[2] = {nullptr, nullptr};
@@ -872,6 +913,35 @@ int staged_subset(storm_mapper* m, const std::vector<int64_t>& dev_ids, int ref,
   return STORM_OK;
 }
 
+// One graph per reference: reset + fused kernel, replayed for every batch with the same buffers
+// and scoring (the kernel arguments are part of the captured graph; the read range is read from
+// the device).  sig is the graph's key in the mapper's cache.
+static int fused_graph(storm_mapper* m, size_t r, const FusedTune& tune, std::vector<const void*>& sig,
+                       hipGraphExec_t& exec) {
+  FusedState& F = m->fs;
+  FusedArgs a = fused_args(m, r, tune.group);
+  sig = {a.g.lh, a.r.planes, a.r.qual, a.r.len, a.slots, a.cells, a.scratch, a.tbbuf, a.work,
+         a.seeds.s[0].first, (const void*)(intptr_t)tune.group, (const void*)(intptr_t)tune.grid,
+         (const void*)(intptr_t)a.scratch_stride, (const void*)(intptr_t)a.sense_mask};
+  const int* scw = (const int*)&a.sc;
+  for (size_t q = 0; q < sizeof(ScoreDev) / sizeof(int); ++q) sig.push_back((const void*)(intptr_t)scw[q]);
+  sig.push_back((const void*)r);
+  exec = F.find_graph(sig);
+  if (!exec) {
+    hipGraph_t g;
+    HIPOK(hipStreamBeginCapture(m->st, hipStreamCaptureModeThreadLocal));
+    hipError_t e1 = launch_fused_reset(a.task_counter, a.stats, m->st);
+    hipError_t e2 = launch_fused(a, tune.grid, m->st);
+    HIPOK(hipStreamEndCapture(m->st, &g));
+    HIPOK(e1);
+    HIPOK(e2);
+    HIPOK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    F.add_graph(sig, exec);
+  }
+  return STORM_OK;
+}
+
 int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback, PassStats& ps,
                unsigned long long totals[4], int64_t& n_fallback) {
   FusedState& F = m->fs;
@@ -890,7 +960,7 @@ int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback,
     HIPOK(hipMemsetAsync(F.phase.p, 0, FUSED_PHASES * sizeof(unsigned long long), m->st));
   }
 #endif
-  const int32_t n32 = (int32_t)n;
+  const int32_t n32[2] = {(int32_t)n, 0};   // reads [0, n)
   const size_t n_refs = m->res->refs.size();
   int pass_id = 0;
   for (int q = 0; q < 4; ++q) totals[q] = 0;
@@ -906,29 +976,11 @@ int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback,
     } else {
       tune_fused(m, n, maxL, max_grid, tune);
     }
-    HIPOK(hipMemcpyAsync(F.n_dev.p, &n32, 4, hipMemcpyHostToDevice, m->st));
-    // one graph per reference: reset + fused kernel, replayed for every batch with the same
-    // buffers and scoring (the kernel arguments are part of the captured graph)
-    FusedArgs a = fused_args(m, r, tune.group);
-    std::vector<const void*> sig = {a.g.lh, a.r.planes, a.r.qual, a.r.len, a.slots, a.cells, a.scratch, a.tbbuf, a.work,
-                                    a.seeds.s[0].first, (const void*)(intptr_t)tune.group, (const void*)(intptr_t)tune.grid,
-                                    (const void*)(intptr_t)a.scratch_stride, (const void*)(intptr_t)a.sense_mask};
-    const int* scw = (const int*)&a.sc;
-    for (size_t q = 0; q < sizeof(ScoreDev) / sizeof(int); ++q) sig.push_back((const void*)(intptr_t)scw[q]);
-    sig.push_back((const void*)r);
-    hipGraphExec_t exec = F.find_graph(sig);
-    if (!exec) {
-      hipGraph_t g;
-      HIPOK(hipStreamBeginCapture(m->st, hipStreamCaptureModeThreadLocal));
-      hipError_t e1 = launch_fused_reset(a.task_counter, a.stats, m->st);
-      hipError_t e2 = launch_fused(a, tune.grid, m->st);
-      HIPOK(hipStreamEndCapture(m->st, &g));
-      HIPOK(e1);
-      HIPOK(e2);
-      HIPOK(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
-      (void)hipGraphDestroy(g);
-      F.add_graph(sig, exec);
-    }
+    HIPOK(hipMemcpyAsync(F.n_dev.p, n32, 8, hipMemcpyHostToDevice, m->st));
+    std::vector<const void*> sig;
+    hipGraphExec_t exec = nullptr;
+    rc = fused_graph(m, r, tune, sig, exec);
+    if (rc) return rc;
 #ifdef STORM_FUSED_PHASES
     unsigned long long ph0[FUSED_PHASES], ph1[FUSED_PHASES];
     HIPOK(hipMemcpyAsync(ph0, F.phase.p, sizeof ph0, hipMemcpyDeviceToHost, m->st));
@@ -977,12 +1029,168 @@ int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback,
 static int batched_stats(storm_mapper* m, int64_t n, int64_t mapped, const unsigned long long totals[4],
                          const PassStats& ps, int64_t n_fallback, double t0, double ti, storm_map_stats* st_out);
 
+// Chunks of a batched records call.  Splitting a launch in K hides (K - 1) / K of the results'
+// copy (measured 36-60 GB/s through the pinned halves) and adds K - 1 launch tails (the last tasks
+// of a persistent launch run with the chip partly idle: measured 0.4-1.0 times the tuner's sample
+// launch, one task per workgroup), so K = sqrt(copy / tail) with the tail taken at its upper
+// figure, rounded down; the tuner's figure is known from the first call of a band class on, which
+// itself runs in one launch.  Measured: 5M color reads of 34-74 on 50 Mbp go in 2 chunks (host
+// share 73 -> 51 ms, launches 411 -> 418 ms); 4M x 100 nt at -i 7 -M 4 (a 25 ms tail against
+// 60 ms of copies) and 100k x 50 nt (two launches 0.26 ms longer than one, copies 0.45 ms) stay in
+// one.  The measurement build with phase timers always maps in one launch (one phase line per
+// call).  The environment variable STORM_RECORD_CHUNKS = K >= 1 fixes the count instead.
+static int record_chunks(storm_mapper* m, const storm_reads_view* v, bool want_tb) {
+#ifdef STORM_FUSED_PHASES
+  (void)m;
+  (void)v;
+  (void)want_tb;
+  return 1;
+#else
+  const int64_t n = v->n_reads;
+  if (m->res->refs.size() != 1) return 1;
+  if (const char* e = getenv("STORM_RECORD_CHUNKS")) {   // a fixed count (1 = one launch), for tests
+    const int k = atoi(e);
+    if (k >= 1) return (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)k, 8, n}));
+  }
+  if (n < 262144) return 1;
+  int maxL = 0;
+  for (int64_t i = 0; i < n; ++i) maxL = std::max(maxL, (int)v->len[i]);
+  if (maxL <= 0 || maxL > MAXL) return 1;   // the one-launch path reports the bad length
+  FusedTune t;
+  if (!cached_tune(m, tune_key(m, maxL), t) || t.ms <= 0) return 1;
+  const double tail_ms = t.ms;
+  const double copy_ms = (double)n * (4.0 * sizeof(HitRec) + (want_tb ? (double)tb_ranks(m->p) * TBB : 0.0)) / 40e6;
+  const int K = (int)std::floor(std::sqrt(copy_ms / tail_ms));
+  return (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)K, 8, n / 32768}));
+#endif
+}
+
+// A batched call with whole records for the caller, in K chunks of consecutive reads of the view:
+// each chunk is one fused launch over its reads (by length within the chunk), and the hit map and
+// tracebacks of chunk k - 1 go to the caller (DMA on the mapper's second stream, host threads
+// copying out of the pinned halves) while chunk k runs on the device, so the copies of all but
+// the last chunk leave the call's critical path.  A read's results do not depend on the other reads
+// of its launch, so they are those of one launch over the view.
+static int map_records_chunked(storm_mapper* m, const storm_reads_view* v, const RecordsOut& ro, int K,
+                               storm_map_stats* st_out) {
+  FusedState& F = m->fs;
+  const int64_t n = v->n_reads;
+  std::vector<int64_t> bounds((size_t)K + 1);
+  for (int k = 0; k <= K; ++k) bounds[(size_t)k] = n * k / K;
+  std::vector<int64_t> order;
+  bool sorted_view = true;
+  length_order_chunks(v, bounds, order, sorted_view, m->len_hist);
+  int maxL = 0;
+  for (int L = MAXL + 1; L > 0 && !maxL; --L)
+    if (m->len_hist[(size_t)L]) maxL = L;
+  const double t0 = now_s();
+  const double ti = m->res->index_seconds;
+  int rc = upload_reads(m, v, order, F.reads, &m->len_hist, &sorted_view);
+  if (rc) return rc;
+  ScoreDev sc{};
+  fill_score(m->p, sc);
+  const int64_t stride = (int64_t)maxL * dp_dir_words(sc.I) * 4;   // direction words per lane
+  const int bpc = fused_blocks_per_cu(sc.W, sc.I, m->p.color);
+  if (bpc <= 0) return fail(STORM_ERR_DEVICE, "fused kernel: no occupancy for this band class");
+  const int max_grid = device_cus(m->device) * bpc;
+  rc = ensure_fused_buffers(m, n, stride, max_grid);
+  if (rc) return rc;
+  HIPOK(launch_slots_init(F.slots.as<Slot>(), 4 * n, m->st));
+  rc = ensure_ref_resident(m, 0);
+  if (rc) return rc;
+  FusedTune tune;
+  rc = tune_fused(m, n, maxL, max_grid, tune);
+  if (rc) return rc;
+  std::vector<const void*> sig;
+  hipGraphExec_t exec = nullptr;
+  rc = fused_graph(m, 0, tune, sig, exec);
+  if (rc) return rc;
+  // the records kernel's device row -> caller row map
+  bool identity = true;
+  for (int64_t i = 0; i < n && identity; ++i) identity = order[(size_t)i] == i;
+  std::vector<int32_t> o32;
+  if (!identity) {
+    o32.resize((size_t)n);
+    for (int64_t i = 0; i < n; ++i) o32[(size_t)i] = (int32_t)order[(size_t)i];
+    HIPOK(m->order_dev.ensure((size_t)n * 4));
+    HIPOK(hipMemcpyAsync(m->order_dev.p, o32.data(), (size_t)n * 4, hipMemcpyHostToDevice, m->st));
+  }
+  const int M = tb_ranks(m->p);
+  const bool want_tb = m->want_tb && ro.tb;
+  HIPOK(m->rec_dev.ensure((size_t)(4 * n) * sizeof(HitRec)));
+  if (want_tb) HIPOK(m->tb_dev.ensure((size_t)(n * M) * TBB));
+  if (!m->st_out) HIPOK(hipStreamCreateWithFlags(&m->st_out, hipStreamNonBlocking));
+  if (!m->rec_ev) HIPOK(hipEventCreateWithFlags(&m->rec_ev, hipEventDisableTiming));
+  if (!m->ev0) HIPOK(hipEventCreate(&m->ev0));
+  if (!m->ev1) HIPOK(hipEventCreate(&m->ev1));
+  HIPOK(m->ranges.ensure((size_t)K * 8));
+  int32_t* rg = (int32_t*)m->ranges.p;   // pinned: the range copies do not wait for the host
+  // the caller's rows of chunk k (its device rows hold the same reads) leave on st_out once the
+  // chunk's records kernel has run
+  auto download = [&](int k) -> int {
+    const int64_t a0 = bounds[(size_t)k], a1 = bounds[(size_t)k + 1];
+    HIPOK(hipStreamWaitEvent(m->st_out, m->rec_ev, 0));
+    std::vector<CopySeg> out = {{ro.hits + 4 * a0, m->rec_dev.as<HitRec>() + 4 * a0, (size_t)(4 * (a1 - a0)) * sizeof(HitRec)}};
+    if (want_tb)
+      out.push_back({ro.tb + a0 * M * TBB, m->tb_dev.as<uint8_t>() + a0 * M * TBB, (size_t)((a1 - a0) * M) * TBB});
+    return staged_copy(m, out, true, m->st_out);
+  };
+  PassStats ps;
+  unsigned long long totals[4] = {0, 0, 0, 0};
+  int64_t n_fallback = 0;
+  int pass_id = 0;
+  std::vector<uint8_t> ovf((size_t)n);
+  for (int k = 0; k < K; ++k) {
+    const int64_t r0 = bounds[(size_t)k], r1 = bounds[(size_t)k + 1];
+    rg[2 * k] = (int32_t)r1;
+    rg[2 * k + 1] = (int32_t)r0;
+    HIPOK(hipMemcpyAsync(F.n_dev.p, rg + 2 * k, 8, hipMemcpyHostToDevice, m->st));
+    HIPOK(hipEventRecord(m->ev0, m->st));
+    HIPOK(hipGraphLaunch(exec, m->st));
+    HIPOK(hipEventRecord(m->ev1, m->st));
+    if (k > 0) {   // chunk k - 1's results out while chunk k runs
+      rc = download(k - 1);
+      if (rc) return rc;
+    }
+    unsigned long long st3[4];
+    HIPOK(hipMemcpyAsync(st3, F.stats.p, sizeof st3, hipMemcpyDeviceToHost, m->st));
+    HIPOK(hipMemcpyAsync(ovf.data() + r0, F.overflow.as<uint8_t>() + r0, (size_t)(r1 - r0), hipMemcpyDeviceToHost, m->st));
+    HIPOK(hipStreamSynchronize(m->st));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, m->ev0, m->ev1);
+    ps.ms_filter += ms;   // the fused kernel carries every stage
+    for (int q = 0; q < 4; ++q) totals[q] += st3[q];
+    std::vector<int64_t> over;
+    for (int64_t i = r0; i < r1; ++i)
+      if (ovf[(size_t)i]) over.push_back(i);
+    n_fallback += (int64_t)over.size();
+    if (!over.empty()) {
+      rc = staged_subset(m, over, 0, pass_id, ps);
+      if (rc) return rc;
+    }
+    HIPOK(launch_hit_records(F.slots.as<Slot>(), identity ? nullptr : m->order_dev.as<int32_t>(), r0, r1, M,
+                             F.cells.as<uint8_t>(), m->rec_dev.as<HitRec>(), want_tb ? m->tb_dev.as<uint8_t>() : nullptr,
+                             m->st));
+    HIPOK(hipEventRecord(m->rec_ev, m->st));
+  }
+  rc = download(K - 1);
+  if (rc) return rc;
+  int64_t mapped = 0;
+  for (int64_t i = 0; i < n; ++i) mapped += ro.hits[4 * i].ref_id != -1;
+  m->finish_seconds = 0;
+  return batched_stats(m, n, mapped, totals, ps, n_fallback, t0, ti, st_out);
+}
+
 static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vector<HostSlot>& out_slots,
                             storm_map_stats* st_out, const HitsOut* ho, const RecordsOut* ro, const FinishFn* fin) {
   FusedState& F = m->fs;
   const int64_t n = v->n_reads;
   if (n <= 0) return fail(STORM_ERR_INVALID, "no reads");
   if (n > (int64_t)INT32_MAX / 4) return fail(STORM_ERR_INVALID, "batch too large");
+  if (ro && !ho && !fin) {
+    const int K = record_chunks(m, v, m->want_tb && ro->tb);
+    if (K > 1) return map_records_chunked(m, v, *ro, K, st_out);
+  }
   // device order: by length (tasks then hold reads of similar length)
   std::vector<int64_t> order;
   bool sorted_view = true;

@@ -62,7 +62,8 @@ void k_fused(FusedArgs a) {
   __shared__ unsigned long long stl[4];   // candidates, filter passes, accepted, raw hits
 
   const int lane = threadIdx.x;
-  const int n = *a.n_reads;
+  const int n = a.n_reads[0];          // the launch maps reads [row0, n) of the batch
+  const int64_t row0 = a.n_reads[1];
   const int G = a.group;
   const int I = a.sc.I;
   const FilterConst fk = filter_const(a.sc.fmatch, a.sc.fmismatch, a.sc.fopen, a.sc.fext, a.sc.fthr);
@@ -308,7 +309,7 @@ void k_fused(FusedArgs a) {
     __syncthreads();
     const int task = task_sh;
     __syncthreads();
-    base = (int64_t)task * G;
+    base = row0 + (int64_t)task * G;
     PH_MARK(5);
     if (base >= n) break;
     const int nr = (n - base) < G ? (int)(n - base) : G;

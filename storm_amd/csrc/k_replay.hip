@@ -210,10 +210,10 @@ hipError_t launch_slots_init(Slot* slots, int64_t n, hipStream_t st) {
 
 // The whole hit map in the caller's order: one thread per slot writes its record and, for a hit
 // at rank < max_rank, copies its traceback cell (16 bytes at a time) to the dense output.
-__global__ __launch_bounds__(256) void k_hit_records(const Slot* slots, const int32_t* order, int64_t n, int max_rank,
-                                                     const uint8_t* cells, HitRec* out, uint8_t* tb_out) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= 4 * n) return;
+__global__ __launch_bounds__(256) void k_hit_records(const Slot* slots, const int32_t* order, int64_t r0, int64_t r1,
+                                                     int max_rank, const uint8_t* cells, HitRec* out, uint8_t* tb_out) {
+  const int64_t t = 4 * r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 4 * r1) return;
   const int k = (int)(t & 3);
   const int64_t row = order ? (int64_t)order[t >> 2] : (t >> 2);
   const Slot s = slots[t];
@@ -249,11 +249,11 @@ __global__ __launch_bounds__(256) void k_hit_records(const Slot* slots, const in
   }
 }
 
-hipError_t launch_hit_records(const Slot* slots, const int32_t* order, int64_t n, int max_rank, const uint8_t* cells,
-                              HitRec* out, uint8_t* tb_out, hipStream_t st) {
-  if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_hit_records, dim3((unsigned)((4 * n + 255) / 256)), dim3(256), 0, st, slots, order, n, max_rank,
-                     cells, out, tb_out);
+hipError_t launch_hit_records(const Slot* slots, const int32_t* order, int64_t r0, int64_t r1, int max_rank,
+                              const uint8_t* cells, HitRec* out, uint8_t* tb_out, hipStream_t st) {
+  if (r1 <= r0) return hipSuccess;
+  hipLaunchKernelGGL(k_hit_records, dim3((unsigned)((4 * (r1 - r0) + 255) / 256)), dim3(256), 0, st, slots, order, r0,
+                     r1, max_rank, cells, out, tb_out);
   return hipGetLastError();
 }
 

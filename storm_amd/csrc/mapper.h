@@ -239,6 +239,11 @@ struct storm_mapper {
   storm::PinnedMem sam_host;
   storm::PinnedMem out_stage;          // two pinned halves the results pass through to the caller
   hipEvent_t stage_ev[2] = {nullptr, nullptr};
+  // chunked records calls: the stream results leave on while the next chunk runs on st, the event
+  // that orders them after the chunk's records kernel, and the chunks' {end, first} read ranges
+  hipStream_t st_out = nullptr;
+  hipEvent_t rec_ev = nullptr;
+  storm::PinnedMem ranges;
   double finish_seconds = 0;   // time spent in a map call's finish step (SAM formatting)
   storm::DevReads sub_reads;   // the batched path's staged fallback: the handed-back reads' planes,
   storm::DevMem sub_ids, sub_slots;   // their ids in the batch and their slots
@@ -250,8 +255,9 @@ struct storm_mapper {
   storm_mapper(const storm_mapper&) = delete;
   storm_mapper& operator=(const storm_mapper&) = delete;
   ~storm_mapper() {
-    for (hipEvent_t e : {ev0, ev1, stage_ev[0], stage_ev[1]})
+    for (hipEvent_t e : {ev0, ev1, stage_ev[0], stage_ev[1], rec_ev})
       if (e) (void)hipEventDestroy(e);
+    if (st_out) (void)hipStreamDestroy(st_out);
   }
 };
 

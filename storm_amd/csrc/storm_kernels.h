@@ -130,7 +130,8 @@ struct FusedArgs {
   ReadsDev r;
   SeedSetDev seeds;
   ScoreDev sc;
-  const int32_t* n_reads;    // reads of the batch (device value, so a captured graph serves any size)
+  const int32_t* n_reads;    // {end, first}: the launch maps reads [first, end) (device values, so a
+                             // captured graph serves any size and any range of the batch)
   int group;                 // G reads per task
   int32_t* task_counter;     // cleared before the launch
   Slot* slots;               // 4 per read
@@ -283,10 +284,10 @@ hipError_t launch_sam(const SamArgs& a, bool write, hipStream_t st);
 // cells + (id * 4 + cell) * TBB, id = ids ? ids[read] : read; those entries become PASS_CELL
 hipError_t launch_store_cells(Slot* slots, int64_t slot_begin, int64_t n_slots, int pass, int max_rank, const uint8_t* tb,
                               uint8_t* cells, const int32_t* ids, hipStream_t st);
-// the n reads' slots (device order) as HitRec rows order[i] (or i) of out, and the tracebacks of
+// the slots of device rows [r0, r1) as HitRec rows order[i] (or i) of out, and the tracebacks of
 // ranks < max_rank into tb_out + (row * max_rank + rank) * TBB (tb_out may be null)
-hipError_t launch_hit_records(const Slot* slots, const int32_t* order, int64_t n, int max_rank, const uint8_t* cells,
-                              HitRec* out, uint8_t* tb_out, hipStream_t st);
+hipError_t launch_hit_records(const Slot* slots, const int32_t* order, int64_t r0, int64_t r1, int max_rank,
+                              const uint8_t* cells, HitRec* out, uint8_t* tb_out, hipStream_t st);
 hipError_t launch_subset_gather(const uint32_t* planes, const uint32_t* qplanes, const int32_t* len, const Slot* slots,
                                 const int32_t* ids, int64_t k, uint32_t* sp, uint32_t* sq, int32_t* sl, Slot* ss,
                                 hipStream_t st);

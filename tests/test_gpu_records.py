@@ -53,3 +53,45 @@ def test_records_plain_equal_batched(color, I, M, tmp_path):
             assert ref_used == int(h["ref_end"]) - int(h["ref_start"]) + 1, (r, k)
             n_tb += 1
     assert n_tb > 0.5 * len(pr.len)
+
+
+@pytest.mark.parametrize("color,chunks", [(0, "3"), (1, "5"), (1, "")], ids=["nucleotide-3", "color-5", "color-model"])
+def test_chunked_records_equal_plain(color, chunks, tmp_path, monkeypatch):
+    """A batched records call can run in chunks of consecutive reads (results of chunk k - 1 copied
+    out while chunk k runs, each chunk ordered by length on its own; STORM_RECORD_CHUNKS fixes the
+    count, unset the library's cost model picks it): its records equal the plain path's with and
+    without tracebacks, and a smaller call into the same arrays (not a multiple of the chunk count)
+    rewrites its rows and leaves the rows past it untouched."""
+    from storm_amd import lib, synth
+    if chunks:
+        monkeypatch.setenv("STORM_RECORD_CHUNKS", chunks)
+    else:
+        monkeypatch.delenv("STORM_RECORD_CHUNKS", raising=False)
+    g = synth.gc_window_genome(2_000_000, 77 + color)
+    gpath = str(tmp_path / "g.fa")
+    synth.write_fasta_fast(gpath, "chr1", g)
+    n = 150_000
+    if color:
+        pr = synth.fast_color_reads(g, n, (34, 74), 91, snp=0.01, color_err=0.03, qlo=10, qhi=35)
+    else:
+        pr = synth.fast_nucleotide_reads(g, n, (36, 120), 91, sub=0.015, indel=0.002, qlo=15, qhi=40)
+    m = lib.Mapper(gpath, color=color, allowed_indels=3, max_hits=1, z=85 if color else 100, t=100,
+                   seeds=SEEDS_C if color else SEEDS_N)
+    print(f"chunked records color={color} chunks={chunks or 'model'}: {n} reads", flush=True)
+    hp, tp, sp = m.map_records(pr, mode=lib.STORM_MODE_PLAIN)
+    hb, tb, sb = m.map_records(pr, mode=lib.STORM_MODE_BATCHED)
+    hn, tn, _ = m.map_records(pr, mode=lib.STORM_MODE_BATCHED, with_tb=False)
+    assert tn is None
+    assert np.array_equal(hp, hb)
+    assert np.array_equal(tp, tb)
+    assert np.array_equal(hp, hn)
+    assert sb["n_mapped"] == sp["n_mapped"] > 0.9 * n
+    # a smaller call into the same arrays: rows [0, k) rewritten, the rest as the first call left them
+    k = 99_991
+    hb2, tb2 = hb.copy(), tb.copy()
+    hb2[:k] = 0
+    tb2[:k] = 0
+    hb3, tb3, _ = m.map_records(pr, mode=lib.STORM_MODE_BATCHED, lo=0, hi=k, out=(hb2, tb2))
+    m.close()
+    assert np.array_equal(hb2[:k], hp[:k]) and np.array_equal(tb2[:k], tp[:k])
+    assert np.array_equal(hb2[k:], hp[k:]) and np.array_equal(tb2[k:], tp[k:])
