@@ -131,47 +131,78 @@ static SeedSetDev seed_set(const RefState& rs) {
 // reports whether that order is the view's own.
 // Longest reads first (stable within a length): the persistent kernel hands out the costliest
 // tasks first and the short ones fill the tail, and equal lengths stay contiguous for the plain path.
-// Device order of a chunked call: the reads of each chunk [bounds[k], bounds[k + 1]) of the view
-// by length within the chunk (longest first, stable), so that chunk k's device rows are its own
-// reads; hist counts the reads per length over the whole view, identity tells whether the order
-// is the view's own.
+// Device order: the reads of each chunk [bounds[k], bounds[k + 1]) of the view by length within
+// the chunk, longest first, stable (a counting sort), so that chunk k's device rows are its own
+// reads; order32, when given, receives the same order in 32 bits (for the device).  hist counts the
+// reads per length over the whole view (lengths outside 0 .. MAXL + 1 at its ends), identity tells
+// whether the order is the view's own.  Large views are counted and scattered by the host pool,
+// each thread a contiguous part of the chunk (the part's reads of one length keep their order, and
+// the parts are placed in turn within each length).
 static void length_order_chunks(const storm_reads_view* v, const std::vector<int64_t>& bounds, std::vector<int64_t>& order,
-                                bool& identity, std::vector<int64_t>& hist) {
+                                bool& identity, std::vector<int64_t>& hist, std::vector<int32_t>* order32 = nullptr) {
   const int64_t n = v->n_reads;
-  auto bucket = [&](int64_t i) { return (size_t)(MAXL + 1 - std::min(std::max((int)v->len[i], 0), MAXL + 1)); };
+  constexpr int NB = MAXL + 2;   // bucket b holds length MAXL + 1 - b
+  auto bucket = [&](int64_t i) { return (int)(MAXL + 1 - std::min(std::max((int)v->len[i], 0), MAXL + 1)); };
   order.resize((size_t)n);
-  hist.assign(MAXL + 2, 0);
+  if (order32) order32->resize((size_t)n);
+  hist.assign(NB, 0);
   identity = true;
-  std::vector<int64_t> start(MAXL + 3);
   for (size_t k = 0; k + 1 < bounds.size(); ++k) {
-    std::fill(start.begin(), start.end(), 0);
-    for (int64_t i = bounds[k]; i < bounds[k + 1]; ++i) {
-      start[bucket(i) + 1]++;
-      if (i > bounds[k] && v->len[i] > v->len[i - 1]) identity = false;
+    const int64_t lo = bounds[k], hi = bounds[k + 1];
+    const int T = (int)std::min<int64_t>(8, std::max<int64_t>(1, (hi - lo) >> 16));
+    std::vector<int64_t> cnt((size_t)T * NB, 0);
+    std::vector<uint8_t> sorted((size_t)T, 1);
+    auto part = [&](int t, int64_t& a, int64_t& b) {
+      a = lo + (hi - lo) * t / T;
+      b = lo + (hi - lo) * (t + 1) / T;
+    };
+    auto count = [&](int t) {
+      int64_t a, b;
+      part(t, a, b);
+      int64_t* c = cnt.data() + (size_t)t * NB;
+      for (int64_t i = a; i < b; ++i) {
+        c[bucket(i)]++;
+        if (i > a && v->len[i] > v->len[i - 1]) sorted[(size_t)t] = 0;
+      }
+    };
+    if (T > 1) HostPool::get().run(T, count);
+    else count(0);
+    for (int t = 0; t < T; ++t) {
+      int64_t a, b;
+      part(t, a, b);
+      if (!sorted[(size_t)t] || (t > 0 && a < b && a > lo && v->len[a] > v->len[a - 1])) identity = false;
     }
-    for (int L = 0; L <= MAXL + 1; ++L) hist[(size_t)L] += start[(size_t)(MAXL + 1 - L) + 1];
-    start[0] = bounds[k];
-    for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
-    for (int64_t i = bounds[k]; i < bounds[k + 1]; ++i) order[(size_t)start[bucket(i)]++] = i;
+    // offsets, bucket-major then part: off[t][b] = lo + reads in buckets < b + reads of bucket b
+    // in parts < t
+    int64_t run = lo;
+    for (int b = 0; b < NB; ++b)
+      for (int t = 0; t < T; ++t) {
+        const int64_t c = cnt[(size_t)t * NB + b];
+        cnt[(size_t)t * NB + b] = run;
+        run += c;
+        hist[(size_t)(MAXL + 1 - b)] += c;
+      }
+    auto scatter = [&](int t) {
+      int64_t a, b;
+      part(t, a, b);
+      int64_t* off = cnt.data() + (size_t)t * NB;
+      if (order32)
+        for (int64_t i = a; i < b; ++i) {
+          const int64_t o = off[bucket(i)]++;
+          order[(size_t)o] = i;
+          (*order32)[(size_t)o] = (int32_t)i;
+        }
+      else
+        for (int64_t i = a; i < b; ++i) order[(size_t)off[bucket(i)]++] = i;
+    };
+    if (T > 1) HostPool::get().run(T, scatter);
+    else scatter(0);
   }
 }
 
 void length_order(const storm_reads_view* v, std::vector<int64_t>& order, bool& identity, std::vector<int64_t>* hist) {
-  const int64_t n = v->n_reads;
-  auto bucket = [&](int64_t i) { return (size_t)(MAXL + 1 - std::min(std::max((int)v->len[i], 0), MAXL + 1)); };
-  std::vector<int64_t> start(MAXL + 3, 0);
-  identity = true;
-  for (int64_t i = 0; i < n; ++i) {
-    start[bucket(i) + 1]++;
-    if (i && v->len[i] > v->len[i - 1]) identity = false;
-  }
-  if (hist) {   // bucket b holds length MAXL + 1 - b
-    hist->assign(MAXL + 2, 0);
-    for (int L = 0; L <= MAXL + 1; ++L) (*hist)[(size_t)L] = start[(size_t)(MAXL + 1 - L) + 1];
-  }
-  for (size_t b = 1; b < start.size(); ++b) start[b] += start[b - 1];
-  order.resize((size_t)n);
-  for (int64_t i = 0; i < n; ++i) order[(size_t)start[bucket(i)]++] = i;
+  std::vector<int64_t> h;
+  length_order_chunks(v, {0, v->n_reads}, order, identity, hist ? *hist : h);
 }
 
 // Uploads the view's reads and arranges them on the device in order[0..) (a permutation of the
@@ -242,7 +273,8 @@ static int staged_copy(storm_mapper* m, const std::vector<CopySeg>& segs, bool t
 }
 
 int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order, DevReads& d,
-                 const std::vector<int64_t>* hist, const bool* identity_in) {
+                 const std::vector<int64_t>* hist, const bool* identity_in, const std::vector<int32_t>* order32) {
+  m->order_dev_n = -1;
   const int64_t n = v->n_reads;
   if ((int64_t)order.size() != n) return fail(STORM_ERR_INVALID, "read order does not cover the view");
   const int w = filter_lanes(m->p.allowed_indels >= 16 ? 15 : m->p.allowed_indels);
@@ -283,10 +315,16 @@ int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<i
   int rc = staged_copy(m, up, false);
   if (rc) return rc;
   if (!identity) {
-    std::vector<int32_t> o32((size_t)n);
-    for (int64_t i = 0; i < n; ++i) o32[(size_t)i] = (int32_t)order[(size_t)i];
+    std::vector<int32_t> o32;
+    const int32_t* oh = order32 && (int64_t)order32->size() == n ? order32->data() : nullptr;
+    if (!oh) {
+      o32.resize((size_t)n);
+      for (int64_t i = 0; i < n; ++i) o32[(size_t)i] = (int32_t)order[(size_t)i];
+      oh = o32.data();
+    }
     HIPOK(m->order_dev.ensure((size_t)n * 4));
-    HIPOK(hipMemcpyAsync(m->order_dev.p, o32.data(), (size_t)n * 4, hipMemcpyHostToDevice, m->st));
+    HIPOK(hipMemcpyAsync(m->order_dev.p, oh, (size_t)n * 4, hipMemcpyHostToDevice, m->st));
+    m->order_dev_n = n;
     const int32_t* od = m->order_dev.as<int32_t>();
     HIPOK(launch_permute_bytes(src.codes.as<uint8_t>(), cs, od, n, d.codes.as<uint8_t>(), m->st));
     HIPOK(launch_permute_i32(src.len.as<int32_t>(), od, n, d.len.as<int32_t>(), m->st));
@@ -302,12 +340,13 @@ static int hits_to_host(storm_mapper* m, const Slot* slots, const std::vector<in
                         const HitsOut& ho, int64_t& mapped) {
   bool identity = true;
   for (int64_t i = 0; i < n && identity; ++i) identity = order[(size_t)i] == i;
-  if (!identity) {
+  if (!identity && m->order_dev_n != n) {   // the upload left no order on the device
     std::vector<int32_t> o32((size_t)n);
     for (int64_t i = 0; i < n; ++i) o32[(size_t)i] = (int32_t)order[(size_t)i];
     HIPOK(m->order_dev.ensure((size_t)n * 4));
     HIPOK(hipMemcpyAsync(m->order_dev.p, o32.data(), (size_t)n * 4, hipMemcpyHostToDevice, m->st));
     HIPOK(hipStreamSynchronize(m->st));
+    m->order_dev_n = n;
   }
   HIPOK(m->hits_dev.ensure((size_t)(4 * n) * 16));
   int32_t* hd = m->hits_dev.as<int32_t>();
@@ -629,17 +668,22 @@ int finish_view(storm_mapper* m, const Slot* slots, const uint8_t* cells, const 
   const bool want_tb = m->want_tb && (!ro || ro->tb);
   bool identity = true;
   for (int64_t i = 0; i < n && identity; ++i) identity = order[(size_t)i] == i;
-  if (!identity) {
+  if (!identity && m->order_dev_n != n) {   // the upload left no order on the device
     std::vector<int32_t> o32((size_t)n);
     for (int64_t i = 0; i < n; ++i) o32[(size_t)i] = (int32_t)order[(size_t)i];
     HIPOK(m->order_dev.ensure((size_t)n * 4));
     HIPOK(hipMemcpyAsync(m->order_dev.p, o32.data(), (size_t)n * 4, hipMemcpyHostToDevice, m->st));
     HIPOK(hipStreamSynchronize(m->st));
+    m->order_dev_n = n;
   }
   HIPOK(m->rec_dev.ensure((size_t)(4 * n) * sizeof(HitRec)));
   if (want_tb) HIPOK(m->tb_dev.ensure((size_t)(n * M) * TBB));
+  HIPOK(m->mapped_dev.ensure(16));
+  HIPOK(hipMemsetAsync(m->mapped_dev.p, 0, 4, m->st));
   HIPOK(launch_hit_records(slots, identity ? nullptr : m->order_dev.as<int32_t>(), 0, n, M, cells, m->rec_dev.as<HitRec>(),
-                           want_tb ? m->tb_dev.as<uint8_t>() : nullptr, m->st));
+                           want_tb ? m->tb_dev.as<uint8_t>() : nullptr, m->mapped_dev.as<int>(), m->st));
+  int mapped_d = 0;
+  HIPOK(hipMemcpyAsync(&mapped_d, m->mapped_dev.p, 4, hipMemcpyDeviceToHost, m->st));
   std::vector<storm_hit> hv;
   std::vector<uint8_t> tv;
   storm_hit* hits = ro ? ro->hits : nullptr;
@@ -651,10 +695,9 @@ int finish_view(storm_mapper* m, const Slot* slots, const uint8_t* cells, const 
   }
   std::vector<CopySeg> out = {{hits, m->rec_dev.p, (size_t)(4 * n) * sizeof(HitRec)}};
   if (want_tb) out.push_back({tb, m->tb_dev.p, (size_t)(n * M) * TBB});
-  const int rc = staged_copy(m, out, true);
+  const int rc = staged_copy(m, out, true);   // the count came with the stream's first copy
   if (rc) return rc;
-  mapped = 0;
-  for (int64_t i = 0; i < n; ++i) mapped += hits[4 * i].ref_id != -1;
+  mapped = mapped_d;
   if (ro) return STORM_OK;
   out_slots.assign((size_t)(4 * n), HostSlot{});
   for (int64_t q = 0; q < 4 * n; ++q) {
@@ -679,7 +722,7 @@ static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode,
   // order groups reads by length, stable within a length
   std::vector<int64_t> order;
   bool sorted_view = true;
-  length_order(v, order, sorted_view, &m->len_hist);
+  length_order_chunks(v, {0, n}, order, sorted_view, m->len_hist, &m->order32);
   std::vector<std::pair<int64_t, int64_t>> batches;
   if (mode == STORM_MODE_PLAIN) {
     int64_t s = 0;
@@ -690,7 +733,7 @@ static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode,
   }
   const double t0 = now_s();
   DevReads& dr = m->plain_reads;   // kept across calls: a call allocates only when it outgrows them
-  int rc = upload_reads(m, v, order, dr, &m->len_hist, &sorted_view);
+  int rc = upload_reads(m, v, order, dr, &m->len_hist, &sorted_view, &m->order32);
   if (rc) return rc;
   DevMem& slots = m->plain_slots;
   HIPOK(slots.ensure((size_t)(4 * n) * sizeof(Slot)));
@@ -1079,13 +1122,13 @@ static int map_records_chunked(storm_mapper* m, const storm_reads_view* v, const
   for (int k = 0; k <= K; ++k) bounds[(size_t)k] = n * k / K;
   std::vector<int64_t> order;
   bool sorted_view = true;
-  length_order_chunks(v, bounds, order, sorted_view, m->len_hist);
+  length_order_chunks(v, bounds, order, sorted_view, m->len_hist, &m->order32);
   int maxL = 0;
   for (int L = MAXL + 1; L > 0 && !maxL; --L)
     if (m->len_hist[(size_t)L]) maxL = L;
   const double t0 = now_s();
   const double ti = m->res->index_seconds;
-  int rc = upload_reads(m, v, order, F.reads, &m->len_hist, &sorted_view);
+  int rc = upload_reads(m, v, order, F.reads, &m->len_hist, &sorted_view, &m->order32);
   if (rc) return rc;
   ScoreDev sc{};
   fill_score(m->p, sc);
@@ -1105,20 +1148,15 @@ static int map_records_chunked(storm_mapper* m, const storm_reads_view* v, const
   hipGraphExec_t exec = nullptr;
   rc = fused_graph(m, 0, tune, sig, exec);
   if (rc) return rc;
-  // the records kernel's device row -> caller row map
-  bool identity = true;
-  for (int64_t i = 0; i < n && identity; ++i) identity = order[(size_t)i] == i;
-  std::vector<int32_t> o32;
-  if (!identity) {
-    o32.resize((size_t)n);
-    for (int64_t i = 0; i < n; ++i) o32[(size_t)i] = (int32_t)order[(size_t)i];
-    HIPOK(m->order_dev.ensure((size_t)n * 4));
-    HIPOK(hipMemcpyAsync(m->order_dev.p, o32.data(), (size_t)n * 4, hipMemcpyHostToDevice, m->st));
-  }
+  // the records kernel's device row -> caller row map: left on the device by the upload
+  const bool identity = sorted_view;
+  if (!identity && m->order_dev_n != n) return fail(STORM_ERR_DEVICE, "read order missing on the device");
   const int M = tb_ranks(m->p);
   const bool want_tb = m->want_tb && ro.tb;
   HIPOK(m->rec_dev.ensure((size_t)(4 * n) * sizeof(HitRec)));
   if (want_tb) HIPOK(m->tb_dev.ensure((size_t)(n * M) * TBB));
+  HIPOK(m->mapped_dev.ensure(16));
+  HIPOK(hipMemsetAsync(m->mapped_dev.p, 0, 4, m->st));
   if (!m->st_out) HIPOK(hipStreamCreateWithFlags(&m->st_out, hipStreamNonBlocking));
   if (!m->rec_ev) HIPOK(hipEventCreateWithFlags(&m->rec_ev, hipEventDisableTiming));
   if (!m->ev0) HIPOK(hipEventCreate(&m->ev0));
@@ -1170,13 +1208,15 @@ static int map_records_chunked(storm_mapper* m, const storm_reads_view* v, const
     }
     HIPOK(launch_hit_records(F.slots.as<Slot>(), identity ? nullptr : m->order_dev.as<int32_t>(), r0, r1, M,
                              F.cells.as<uint8_t>(), m->rec_dev.as<HitRec>(), want_tb ? m->tb_dev.as<uint8_t>() : nullptr,
-                             m->st));
+                             m->mapped_dev.as<int>(), m->st));
     HIPOK(hipEventRecord(m->rec_ev, m->st));
   }
+  int mapped_d = 0;
+  HIPOK(hipMemcpyAsync(&mapped_d, m->mapped_dev.p, 4, hipMemcpyDeviceToHost, m->st));
   rc = download(K - 1);
   if (rc) return rc;
-  int64_t mapped = 0;
-  for (int64_t i = 0; i < n; ++i) mapped += ro.hits[4 * i].ref_id != -1;
+  HIPOK(hipStreamSynchronize(m->st));
+  const int64_t mapped = mapped_d;
   m->finish_seconds = 0;
   return batched_stats(m, n, mapped, totals, ps, n_fallback, t0, ti, st_out);
 }
@@ -1194,13 +1234,13 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
   // device order: by length (tasks then hold reads of similar length)
   std::vector<int64_t> order;
   bool sorted_view = true;
-  length_order(v, order, sorted_view, &m->len_hist);
+  length_order_chunks(v, {0, n}, order, sorted_view, m->len_hist, &m->order32);
   int maxL = 0;
   for (int L = MAXL + 1; L > 0 && !maxL; --L)
     if (m->len_hist[(size_t)L]) maxL = L;
   const double t0 = now_s();
   const double ti = m->res->index_seconds;
-  int rc = upload_reads(m, v, order, F.reads, &m->len_hist, &sorted_view);
+  int rc = upload_reads(m, v, order, F.reads, &m->len_hist, &sorted_view, &m->order32);
   if (rc) return rc;
   PassStats ps;
   unsigned long long totals[4] = {0, 0, 0, 0};
@@ -1269,6 +1309,7 @@ static int64_t count_lookups(const storm_mapper* m, const storm_reads_view* v) {
 int map_view(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
              storm_map_stats* st_out, const HitsOut* ho, const RecordsOut* ro, const FinishFn* fin) {
   m->finish_seconds = 0;
+  m->order_dev_n = -1;
   int rc = (mode == STORM_MODE_BATCHED) ? map_view_batched(m, v, out_slots, st_out, ho, ro, fin)
                                          : map_view_staged(m, v, mode, out_slots, st_out, ho, ro, fin);
   if (rc == STORM_OK && st_out) st_out->n_lookups = count_lookups(m, v);

@@ -211,12 +211,18 @@ hipError_t launch_slots_init(Slot* slots, int64_t n, hipStream_t st) {
 // The whole hit map in the caller's order: one thread per slot writes its record and, for a hit
 // at rank < max_rank, copies its traceback cell (16 bytes at a time) to the dense output.
 __global__ __launch_bounds__(256) void k_hit_records(const Slot* slots, const int32_t* order, int64_t r0, int64_t r1,
-                                                     int max_rank, const uint8_t* cells, HitRec* out, uint8_t* tb_out) {
+                                                     int max_rank, const uint8_t* cells, HitRec* out, uint8_t* tb_out,
+                                                     int* mapped) {
   const int64_t t = 4 * r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 4 * r1) return;
   const int k = (int)(t & 3);
   const int64_t row = order ? (int64_t)order[t >> 2] : (t >> 2);
   const Slot s = slots[t];
+  if (mapped) {   // reads whose best slot holds a hit: one add per wavefront, by its first live lane
+    const unsigned long long hit = __ballot(k == 0 && s.ref_id != -1);
+    const unsigned long long live = __ballot(1);
+    if ((int)(threadIdx.x & 63) == __ffsll((long long)live) - 1 && hit) atomicAdd(mapped, (int)__popcll(hit));
+  }
   HitRec h;
   h.score = s.score; h.ref_id = s.ref_id; h.ref_start = s.ref_start; h.ref_end = s.ref_end;
   h.read_start = s.read_start; h.read_end = s.read_end; h.sense = s.sense;
@@ -250,10 +256,10 @@ __global__ __launch_bounds__(256) void k_hit_records(const Slot* slots, const in
 }
 
 hipError_t launch_hit_records(const Slot* slots, const int32_t* order, int64_t r0, int64_t r1, int max_rank,
-                              const uint8_t* cells, HitRec* out, uint8_t* tb_out, hipStream_t st) {
+                              const uint8_t* cells, HitRec* out, uint8_t* tb_out, int* mapped, hipStream_t st) {
   if (r1 <= r0) return hipSuccess;
   hipLaunchKernelGGL(k_hit_records, dim3((unsigned)((4 * (r1 - r0) + 255) / 256)), dim3(256), 0, st, slots, order, r0,
-                     r1, max_rank, cells, out, tb_out);
+                     r1, max_rank, cells, out, tb_out, mapped);
   return hipGetLastError();
 }
 

@@ -248,6 +248,9 @@ struct storm_mapper {
   storm::DevReads sub_reads;   // the batched path's staged fallback: the handed-back reads' planes,
   storm::DevMem sub_ids, sub_slots;   // their ids in the batch and their slots
   storm::DevMem order_dev;     // device row -> caller's read index
+  int64_t order_dev_n = -1;    // reads of the current call whose order order_dev holds (-1: none)
+  std::vector<int32_t> order32;   // the current call's order in 32 bits, on its way to order_dev
+  storm::DevMem mapped_dev;    // reads with a hit, counted by the records kernel
   storm::DevMem hits_dev;      // per-slot results on their way to the caller
   storm::SharedTune* shared_tune = nullptr;   // tuner results of sibling contexts, or null
   bool want_tb = true;   // false while a call returns hits only: tracebacks stay on the device
@@ -303,8 +306,11 @@ int align_plan_view(storm_align_plan* pl, AlignArgs& a, int& maxL, int& device, 
 int64_t align_scratch_bytes(int I, int maxL, int64_t n_rec);
 int prep_device_reads(storm_mapper* m, int64_t n, int64_t code_stride, int64_t qual_stride, bool has_qual,
                       bool has_first_qual, DevReads& d);
+// order32: the order in 32 bits, when the caller has it.  A non-identity order is left in
+// m->order_dev (m->order_dev_n = the view's reads) for the call's result copies.
 int upload_reads(storm_mapper* m, const storm_reads_view* v, const std::vector<int64_t>& order, DevReads& d,
-                 const std::vector<int64_t>* hist = nullptr, const bool* identity = nullptr);
+                 const std::vector<int64_t>* hist = nullptr, const bool* identity = nullptr,
+                 const std::vector<int32_t>* order32 = nullptr);
 // one staged pass over reads [r0, r1) of dr; PASS_SPLIT (nothing written) when its raw index hits
 // exceed PASS_HIT_LIMIT
 constexpr int PASS_SPLIT = 1;

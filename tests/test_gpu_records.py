@@ -95,3 +95,32 @@ def test_chunked_records_equal_plain(color, chunks, tmp_path, monkeypatch):
     m.close()
     assert np.array_equal(hb2[:k], hp[:k]) and np.array_equal(tb2[:k], tp[:k])
     assert np.array_equal(hb2[k:], hp[k:]) and np.array_equal(tb2[k:], tp[k:])
+
+
+def test_length_order_of_large_views(tmp_path, monkeypatch):
+    """Views of 300k unsorted mixed-length reads are ordered on the host by several threads (a
+    counting sort in parts); the records of both paths, one launch and three chunks, equal those of
+    the same reads handed over already sorted longest first (no reordering at all), put back in
+    the view's order."""
+    from storm_amd import lib, synth
+    g = synth.gc_window_genome(2_000_000, 5)
+    gpath = str(tmp_path / "g.fa")
+    synth.write_fasta_fast(gpath, "chr1", g)
+    n = 300_000
+    pr = synth.fast_nucleotide_reads(g, n, (36, 100), 17, sub=0.015, indel=0.002, qlo=15, qhi=40)
+    perm = np.argsort(-pr.len, kind="stable")
+    srt = synth.PackedReads(pr.codes[perm], pr.qual[perm], pr.len[perm])
+    m = lib.Mapper(gpath, color=0, allowed_indels=3, max_hits=2, seeds=SEEDS_N)
+    print(f"length order: {n} reads", flush=True)
+    hs, ts, _ = m.map_records(srt, mode=lib.STORM_MODE_PLAIN)
+    want_h = np.empty_like(hs)
+    want_t = np.empty_like(ts)
+    want_h[perm] = hs
+    want_t[perm] = ts
+    for mode, chunks in ((lib.STORM_MODE_PLAIN, "1"), (lib.STORM_MODE_BATCHED, "1"), (lib.STORM_MODE_BATCHED, "3")):
+        monkeypatch.setenv("STORM_RECORD_CHUNKS", chunks)
+        h, t, st = m.map_records(pr, mode=mode)
+        assert np.array_equal(h, want_h), (mode, chunks)
+        assert np.array_equal(t, want_t), (mode, chunks)
+        assert st["n_mapped"] == int((want_h[:, 0]["ref_id"] != -1).sum())
+    m.close()
