@@ -720,7 +720,7 @@ static int map_view_staged(storm_mapper* m, const storm_reads_view* v, int mode,
   if (n <= 0) return fail(STORM_ERR_INVALID, "no reads");
   // plain mode: equal-length batches (the reference maps one read length per run); the device
   // order groups reads by length, stable within a length
-  std::vector<int64_t> order;
+  std::vector<int64_t>& order = m->order64;
   bool sorted_view = true;
   length_order_chunks(v, {0, n}, order, sorted_view, m->len_hist, &m->order32);
   std::vector<std::pair<int64_t, int64_t>> batches;
@@ -1008,7 +1008,8 @@ int fused_core(storm_mapper* m, int64_t n, int maxL, const FallbackFn& fallback,
   int pass_id = 0;
   for (int q = 0; q < 4; ++q) totals[q] = 0;
   n_fallback = 0;
-  std::vector<uint8_t> ovf((size_t)n);
+  std::vector<uint8_t>& ovf = m->ovf_host;
+  if ((int64_t)ovf.size() < n) ovf.resize((size_t)n);
   for (size_t r = 0; r < n_refs; ++r) {
     rc = ensure_ref_resident(m, r);
     if (rc) return rc;
@@ -1120,7 +1121,7 @@ static int map_records_chunked(storm_mapper* m, const storm_reads_view* v, const
   const int64_t n = v->n_reads;
   std::vector<int64_t> bounds((size_t)K + 1);
   for (int k = 0; k <= K; ++k) bounds[(size_t)k] = n * k / K;
-  std::vector<int64_t> order;
+  std::vector<int64_t>& order = m->order64;
   bool sorted_view = true;
   length_order_chunks(v, bounds, order, sorted_view, m->len_hist, &m->order32);
   int maxL = 0;
@@ -1177,7 +1178,8 @@ static int map_records_chunked(storm_mapper* m, const storm_reads_view* v, const
   unsigned long long totals[4] = {0, 0, 0, 0};
   int64_t n_fallback = 0;
   int pass_id = 0;
-  std::vector<uint8_t> ovf((size_t)n);
+  std::vector<uint8_t>& ovf = m->ovf_host;
+  if ((int64_t)ovf.size() < n) ovf.resize((size_t)n);
   for (int k = 0; k < K; ++k) {
     const int64_t r0 = bounds[(size_t)k], r1 = bounds[(size_t)k + 1];
     rg[2 * k] = (int32_t)r1;
@@ -1232,7 +1234,7 @@ static int map_view_batched(storm_mapper* m, const storm_reads_view* v, std::vec
     if (K > 1) return map_records_chunked(m, v, *ro, K, st_out);
   }
   // device order: by length (tasks then hold reads of similar length)
-  std::vector<int64_t> order;
+  std::vector<int64_t>& order = m->order64;
   bool sorted_view = true;
   length_order_chunks(v, {0, n}, order, sorted_view, m->len_hist, &m->order32);
   int maxL = 0;

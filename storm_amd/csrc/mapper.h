@@ -250,6 +250,10 @@ struct storm_mapper {
   storm::DevMem order_dev;     // device row -> caller's read index
   int64_t order_dev_n = -1;    // reads of the current call whose order order_dev holds (-1: none)
   std::vector<int32_t> order32;   // the current call's order in 32 bits, on its way to order_dev
+  // host arrays of a call, kept so that a call of a size seen before touches no fresh pages: the
+  // device order, the fused kernel's per-read overflow flags
+  std::vector<int64_t> order64;
+  std::vector<uint8_t> ovf_host;
   storm::DevMem mapped_dev;    // reads with a hit, counted by the records kernel
   storm::DevMem hits_dev;      // per-slot results on their way to the caller
   storm::SharedTune* shared_tune = nullptr;   // tuner results of sibling contexts, or null
