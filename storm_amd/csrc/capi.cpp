@@ -200,10 +200,6 @@ static void length_order_chunks(const storm_reads_view* v, const std::vector<int
   }
 }
 
-void length_order(const storm_reads_view* v, std::vector<int64_t>& order, bool& identity, std::vector<int64_t>* hist) {
-  std::vector<int64_t> h;
-  length_order_chunks(v, {0, v->n_reads}, order, identity, hist ? *hist : h);
-}
 
 // Uploads the view's reads and arranges them on the device in order[0..) (a permutation of the
 // view).  The caller's arrays go to the device as they are; a reordering is a device gather.
@@ -1294,7 +1290,7 @@ int load_resident(const char* genome_path, const storm_params& p, std::shared_pt
   return STORM_OK;
 }
 
-// index lookups of a call, from its length histogram (length_order)
+// index lookups of a call, from its length histogram (length_order_chunks)
 static int64_t count_lookups(const storm_mapper* m, const storm_reads_view* v) {
   (void)v;
   const int strands = ((m->p.sense_mask & 1) ? 1 : 0) + ((m->p.sense_mask & 2) ? 1 : 0);

@@ -230,7 +230,7 @@ struct storm_mapper {
   storm::DevReads plain_reads; // the plain path's reads in device order, reused across calls
   storm::DevMem plain_slots;   // the plain path's hit map slots, reused across calls
   storm::DevMem plain_cells;   // the plain path's traceback cells (4 per read), reused across calls
-  std::vector<int64_t> len_hist;   // reads per length of the current call (length_order)
+  std::vector<int64_t> len_hist;   // reads per length of the current call (length_order_chunks)
   storm::DevMem rec_dev, tb_dev;   // the hit map and tracebacks in the caller's order, on their way out
   // SAM records formatted on the device: the caller's read names, the MAPQ table of the scores,
   // caller row -> device row, per-row byte counts and offsets, the text and its host copy
@@ -385,11 +385,6 @@ struct RecordsOut {
   storm_hit* hits;
   uint8_t* tb;
 };
-// The view's reads ordered by length, longest first, stable (a counting sort); identity = the
-// view is already in that order; hist (MAXL + 2 entries), when given, receives the number of reads
-// of each length 0 .. MAXL + 1 (lengths outside that range counted at its ends).
-void length_order(const storm_reads_view* v, std::vector<int64_t>& order, bool& identity,
-                  std::vector<int64_t>* hist = nullptr);
 // ho != null: the slots go straight to the caller's arrays (no host slot assembly, no tracebacks);
 // ro != null: the records and tracebacks go to the caller's arrays instead of out_slots
 int map_view(storm_mapper* m, const storm_reads_view* v, int mode, std::vector<HostSlot>& out_slots,
