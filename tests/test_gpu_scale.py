@@ -6,7 +6,8 @@
 * The 50 Mbp color workload (GC 40-60 % per 10 kb window, reads of 34-74 colors, -i 3 -z 85
   -t 100 -M 1) at 2M reads and the 100 Mbp repeat-genome workload (segmental duplications and
   tandem repeats, 100 nt reads, -i 7 -M 4) at 1M reads: the two GPU paths return the same whole
-  hit map and the same traceback bytes for every read.  The staged path maps one read length at
+  hit map and the same traceback bytes for every read (on the repeat genome also with the batched
+  call split into three launches, each with reads handed to the staged kernels).  The staged path maps one read length at
   a time, as the reference does (reads_data.c:156-163), and is itself pinned against the CPU
   restatement per length (test_gpu_mixed.py, test_gpu_large.py); at these sizes the comparison
   covers reads with thousands of candidates and, on the repeat genome, reads handed from the fused
@@ -80,7 +81,7 @@ def test_config3_scale_batched_equals_plain():
     assert sb["n_mapped"] == sp["n_mapped"]
 
 
-def test_config4_scale_batched_equals_plain():
+def test_config4_scale_batched_equals_plain(monkeypatch):
     from storm_amd import lib, synth
     import tempfile
     d = tempfile.mkdtemp(prefix="storm_scale4_")
@@ -91,8 +92,14 @@ def test_config4_scale_batched_equals_plain():
     m = lib.Mapper(gpath, color=0, seeds=SEEDS_N, allowed_indels=7, max_hits=4)
     print("records: 1M x 100 nt on the 100 Mbp repeat genome, -i 7 -M 4, both paths", flush=True)
     bad, sb, sp = _records_equal(m, pr, 0, 1_000_000)
+    # the same batch in three launches over consecutive reads, each with its own staged fallback
+    monkeypatch.setenv("STORM_RECORD_CHUNKS", "3")
+    hc, tbc, sc = m.map_records(pr, lib.STORM_MODE_BATCHED, 0, 1_000_000)
+    hp, tbp, _ = m.map_records(pr, lib.STORM_MODE_PLAIN, 0, 1_000_000)
     m.close()
     print({k: sb[k] for k in ("n_mapped", "n_candidates", "n_filter_pass", "n_fallback")}, flush=True)
     assert sb["n_fallback"] > 0               # reads handed to the staged kernels are covered
     assert len(bad) == 0, f"{len(bad)} reads differ, first {bad[:10]}"
     assert sb["n_mapped"] == sp["n_mapped"]
+    assert sc["n_fallback"] == sb["n_fallback"]
+    assert np.array_equal(hc, hp) and np.array_equal(tbc, tbp)
