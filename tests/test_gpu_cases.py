@@ -1,6 +1,6 @@
 """Both GPU paths against the oracle on the seeded cases that pin the oracle to the reference
 binaries (test_oracle_vs_reference_binary.py): -i 0 .. 20 (every band class of the filter and the
-DP, the two-records-per-lane DP and the single-record one), -M 1-4, -u, -A, -F / -R, custom scores,
+DP), -M 1-4, -u, -A, -F / -R, custom scores,
 explicit seeds and position lists, N runs, lowercase, multi-FASTA, repeat filter firing, in
 nucleotide and color space; and the ordered mode (no -M: contextual genome map, greedy -G, -u).
 Integer and text output: the SAM must be identical."""
