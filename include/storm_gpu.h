@@ -289,7 +289,10 @@ int storm_mapper_map_hits(storm_mapper *m, const storm_reads_view *reads, int mo
 typedef struct storm_hit {
   int32_t score, ref_id, ref_start, ref_end, read_start, read_end, sense, tb_len;
 } storm_hit;
-/* In the batched mode, on a one-sequence genome, a call may run as several fused launches over
+/* The tracebacks leave the device in the call's compact width (the longest read's codes plus the
+ * indel cap) and are widened to STORM_TB_BYTES on the host (the environment variable
+ * STORM_TB_FULL_WIDTH copies whole records instead); the bytes the caller receives are the same.
+ * In the batched mode, on a one-sequence genome, a call may run as several fused launches over
  * consecutive chunks of the reads, the results of one chunk copied to hits/tb while the next runs
  * (the count from the copy volume and the band class's tuned launch time; the environment variable
  * STORM_RECORD_CHUNKS = K >= 1 fixes it).  The results are those of one launch. */

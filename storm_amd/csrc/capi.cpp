@@ -218,11 +218,30 @@ static void host_copy(void* dst, const void* src, size_t bytes) {
   });
 }
 
+// A copy of `bytes` from src to dst; with unit > 0 (device to host only) the source is records of
+// `unit` bytes that land dst_unit >= unit bytes apart in dst, the rest of each zero-filled.
 struct CopySeg {
   void* dst;
   const void* src;
   size_t bytes;
+  size_t unit = 0, dst_unit = 0;
 };
+
+// rows records of `unit` bytes from src into dst_unit-byte slots of dst (the rest of each slot
+// zeroed), on up to 8 host threads
+static void host_expand(void* dst, const void* src, size_t rows, size_t unit, size_t dst_unit) {
+  const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, (rows * dst_unit) >> 20));
+  auto part = [&](int t) {
+    const size_t a = rows * t / nt, b = rows * (t + 1) / nt;
+    for (size_t r = a; r < b; ++r) {
+      char* d = (char*)dst + r * dst_unit;
+      memcpy(d, (const char*)src + r * unit, unit);
+      memset(d + unit, 0, dst_unit - unit);
+    }
+  };
+  if (nt <= 1) part(0);
+  else HostPool::get().run(nt, part);
+}
 
 // Copies between the caller's (pageable) host memory and device memory on stream st (default: the
 // mapper's) at pinned-copy speed: the segments are cut into pieces of up to 8 MiB that pass through two pinned
@@ -242,11 +261,18 @@ static int staged_copy(storm_mapper* m, const std::vector<CopySeg>& segs, bool t
     char* dst;
     const char* src;
     size_t len;
+    size_t unit, dst_unit;
   };
   std::vector<Piece> ps;
-  for (const CopySeg& g : segs)
-    for (size_t off = 0; off < g.bytes; off += CH)
-      ps.push_back({(char*)g.dst + off, (const char*)g.src + off, std::min(CH, g.bytes - off)});
+  for (const CopySeg& g : segs) {
+    if (g.unit && (!to_host || g.dst_unit < g.unit || g.unit > CH || g.bytes % g.unit))
+      return fail(STORM_ERR_INVALID, "bad record copy");
+    const size_t step = g.unit ? CH / g.unit * g.unit : CH;   // pieces end on record boundaries
+    for (size_t off = 0; off < g.bytes; off += step) {
+      const size_t doff = g.unit ? off / g.unit * g.dst_unit : off;
+      ps.push_back({(char*)g.dst + doff, (const char*)g.src + off, std::min(step, g.bytes - off), g.unit, g.dst_unit});
+    }
+  }
   for (size_t c = 0; c <= ps.size(); ++c) {
     if (c < ps.size()) {
       char* half = (char*)m->out_stage.p + (c & 1) * CH;
@@ -262,7 +288,9 @@ static int staged_copy(storm_mapper* m, const std::vector<CopySeg>& segs, bool t
     if (to_host && c >= 1) {   // piece c - 1 out of its half while piece c is in flight
       const size_t q = c - 1;
       HIPOK(hipEventSynchronize(m->stage_ev[q & 1]));
-      host_copy(ps[q].dst, (const char*)m->out_stage.p + (q & 1) * CH, ps[q].len);
+      const char* half = (const char*)m->out_stage.p + (q & 1) * CH;
+      if (ps[q].unit) host_expand(ps[q].dst, half, ps[q].len / ps[q].unit, ps[q].unit, ps[q].dst_unit);
+      else host_copy(ps[q].dst, half, ps[q].len);
     }
   }
   return STORM_OK;
@@ -657,6 +685,22 @@ static_assert(sizeof(storm_hit) == sizeof(HitRec) && offsetof(storm_hit, tb_len)
 // The end of a map call: the device slots (device order) and the reads' traceback cells go to the
 // caller's arrays (ho: four int32 arrays; ro: whole records and tracebacks) or to host slots in
 // view order (SAM output).  Returns the mapped reads.
+// Bytes of a traceback record on its way to the caller: a path through a band of 2I + 1 columns
+// takes at most L + I steps (one per read code, plus at most I horizontal gap steps, the indel
+// cap), 2 codes per byte, so records of the call's longest read fit in (maxL + I + 1) / 2 bytes,
+// rounded up to 16; the caller's records are TBB bytes, the rest zero.  The environment variable
+// STORM_TB_FULL_WIDTH copies whole TBB-byte records instead (for tests).
+static int tb_width(const storm_mapper* m) {
+  if (getenv("STORM_TB_FULL_WIDTH")) return TBB;
+  int maxL = 0;
+  for (int L = MAXL + 1; L > 0 && !maxL; --L)
+    if ((size_t)L < m->len_hist.size() && m->len_hist[(size_t)L]) maxL = L;
+  if (maxL <= 0 || maxL > MAXL) return TBB;
+  const int I = std::max(0, std::min(m->p.allowed_indels, MAX_I));
+  const int w = ((maxL + I + 1) / 2 + 15) / 16 * 16;
+  return std::min(w, TBB);
+}
+
 int finish_view(storm_mapper* m, const Slot* slots, const uint8_t* cells, const std::vector<int64_t>& order, int64_t n,
                 const HitsOut* ho, const RecordsOut* ro, std::vector<HostSlot>& out_slots, int64_t& mapped) {
   if (ho) return hits_to_host(m, slots, order, n, *ho, mapped);
@@ -672,12 +716,13 @@ int finish_view(storm_mapper* m, const Slot* slots, const uint8_t* cells, const 
     HIPOK(hipStreamSynchronize(m->st));
     m->order_dev_n = n;
   }
+  const int tw = ro ? tb_width(m) : TBB;   // records for the caller leave compact
   HIPOK(m->rec_dev.ensure((size_t)(4 * n) * sizeof(HitRec)));
-  if (want_tb) HIPOK(m->tb_dev.ensure((size_t)(n * M) * TBB));
+  if (want_tb) HIPOK(m->tb_dev.ensure((size_t)(n * M) * tw));
   HIPOK(m->mapped_dev.ensure(16));
   HIPOK(hipMemsetAsync(m->mapped_dev.p, 0, 4, m->st));
   HIPOK(launch_hit_records(slots, identity ? nullptr : m->order_dev.as<int32_t>(), 0, n, M, cells, m->rec_dev.as<HitRec>(),
-                           want_tb ? m->tb_dev.as<uint8_t>() : nullptr, m->mapped_dev.as<int>(), m->st));
+                           want_tb ? m->tb_dev.as<uint8_t>() : nullptr, tw, m->mapped_dev.as<int>(), m->st));
   int mapped_d = 0;
   HIPOK(hipMemcpyAsync(&mapped_d, m->mapped_dev.p, 4, hipMemcpyDeviceToHost, m->st));
   std::vector<storm_hit> hv;
@@ -690,7 +735,7 @@ int finish_view(storm_mapper* m, const Slot* slots, const uint8_t* cells, const 
     if (want_tb) { tv.resize((size_t)(n * M) * TBB); tb = tv.data(); }
   }
   std::vector<CopySeg> out = {{hits, m->rec_dev.p, (size_t)(4 * n) * sizeof(HitRec)}};
-  if (want_tb) out.push_back({tb, m->tb_dev.p, (size_t)(n * M) * TBB});
+  if (want_tb) out.push_back({tb, m->tb_dev.p, (size_t)(n * M) * tw, (size_t)tw, (size_t)TBB});
   const int rc = staged_copy(m, out, true);   // the count came with the stream's first copy
   if (rc) return rc;
   mapped = mapped_d;
@@ -1151,7 +1196,8 @@ static int map_records_chunked(storm_mapper* m, const storm_reads_view* v, const
   const int M = tb_ranks(m->p);
   const bool want_tb = m->want_tb && ro.tb;
   HIPOK(m->rec_dev.ensure((size_t)(4 * n) * sizeof(HitRec)));
-  if (want_tb) HIPOK(m->tb_dev.ensure((size_t)(n * M) * TBB));
+  const int tw = tb_width(m);   // compact records, expanded into the caller's
+  if (want_tb) HIPOK(m->tb_dev.ensure((size_t)(n * M) * tw));
   HIPOK(m->mapped_dev.ensure(16));
   HIPOK(hipMemsetAsync(m->mapped_dev.p, 0, 4, m->st));
   if (!m->st_out) HIPOK(hipStreamCreateWithFlags(&m->st_out, hipStreamNonBlocking));
@@ -1167,7 +1213,8 @@ static int map_records_chunked(storm_mapper* m, const storm_reads_view* v, const
     HIPOK(hipStreamWaitEvent(m->st_out, m->rec_ev, 0));
     std::vector<CopySeg> out = {{ro.hits + 4 * a0, m->rec_dev.as<HitRec>() + 4 * a0, (size_t)(4 * (a1 - a0)) * sizeof(HitRec)}};
     if (want_tb)
-      out.push_back({ro.tb + a0 * M * TBB, m->tb_dev.as<uint8_t>() + a0 * M * TBB, (size_t)((a1 - a0) * M) * TBB});
+      out.push_back({ro.tb + a0 * M * TBB, m->tb_dev.as<uint8_t>() + a0 * M * tw, (size_t)((a1 - a0) * M) * tw, (size_t)tw,
+                     (size_t)TBB});
     return staged_copy(m, out, true, m->st_out);
   };
   PassStats ps;
@@ -1205,7 +1252,7 @@ static int map_records_chunked(storm_mapper* m, const storm_reads_view* v, const
       if (rc) return rc;
     }
     HIPOK(launch_hit_records(F.slots.as<Slot>(), identity ? nullptr : m->order_dev.as<int32_t>(), r0, r1, M,
-                             F.cells.as<uint8_t>(), m->rec_dev.as<HitRec>(), want_tb ? m->tb_dev.as<uint8_t>() : nullptr,
+                             F.cells.as<uint8_t>(), m->rec_dev.as<HitRec>(), want_tb ? m->tb_dev.as<uint8_t>() : nullptr, tw,
                              m->mapped_dev.as<int>(), m->st));
     HIPOK(hipEventRecord(m->rec_ev, m->st));
   }

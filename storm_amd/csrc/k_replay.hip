@@ -212,7 +212,7 @@ hipError_t launch_slots_init(Slot* slots, int64_t n, hipStream_t st) {
 // at rank < max_rank, copies its traceback cell (16 bytes at a time) to the dense output.
 __global__ __launch_bounds__(256) void k_hit_records(const Slot* slots, const int32_t* order, int64_t r0, int64_t r1,
                                                      int max_rank, const uint8_t* cells, HitRec* out, uint8_t* tb_out,
-                                                     int* mapped) {
+                                                     int tb_stride, int* mapped) {
   const int64_t t = 4 * r0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= 4 * r1) return;
   const int k = (int)(t & 3);
@@ -234,9 +234,10 @@ __global__ __launch_bounds__(256) void k_hit_records(const Slot* slots, const in
     const bool hit = s.ref_id != -1 && s.pass == PASS_CELL;
     const int nb = hit ? (h.tb_len + 1) / 2 : 0;
     const uint4* src = (const uint4*)(cells + ((t & ~3LL) + s.cell) * (int64_t)TBB);
-    uint4* dst = (uint4*)(tb_out + (row * max_rank + k) * (int64_t)TBB);
+    uint4* dst = (uint4*)(tb_out + (row * max_rank + k) * (int64_t)tb_stride);
 #pragma unroll
     for (int w = 0; w < TBB / 16; ++w) {
+      if (16 * w >= tb_stride) break;
       uint4 x = make_uint4(0, 0, 0, 0);
       if (16 * w < nb) {
         x = src[w];
@@ -256,10 +257,12 @@ __global__ __launch_bounds__(256) void k_hit_records(const Slot* slots, const in
 }
 
 hipError_t launch_hit_records(const Slot* slots, const int32_t* order, int64_t r0, int64_t r1, int max_rank,
-                              const uint8_t* cells, HitRec* out, uint8_t* tb_out, int* mapped, hipStream_t st) {
+                              const uint8_t* cells, HitRec* out, uint8_t* tb_out, int tb_stride, int* mapped,
+                              hipStream_t st) {
   if (r1 <= r0) return hipSuccess;
+  if (tb_stride <= 0 || tb_stride > TBB || tb_stride % 16) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_hit_records, dim3((unsigned)((4 * (r1 - r0) + 255) / 256)), dim3(256), 0, st, slots, order, r0,
-                     r1, max_rank, cells, out, tb_out, mapped);
+                     r1, max_rank, cells, out, tb_out, tb_stride, mapped);
   return hipGetLastError();
 }
 

@@ -285,10 +285,12 @@ hipError_t launch_sam(const SamArgs& a, bool write, hipStream_t st);
 hipError_t launch_store_cells(Slot* slots, int64_t slot_begin, int64_t n_slots, int pass, int max_rank, const uint8_t* tb,
                               uint8_t* cells, const int32_t* ids, hipStream_t st);
 // the slots of device rows [r0, r1) as HitRec rows order[i] (or i) of out, and the tracebacks of
-// ranks < max_rank into tb_out + (row * max_rank + rank) * TBB (tb_out may be null); *mapped (may be
-// null) is raised by the rows whose first slot holds a hit
+// ranks < max_rank into tb_out + (row * max_rank + rank) * tb_stride (tb_out may be null): the first
+// tb_stride bytes of each (a multiple of 16, at most TBB, at least the longest traceback's bytes);
+// *mapped (may be null) is raised by the rows whose first slot holds a hit
 hipError_t launch_hit_records(const Slot* slots, const int32_t* order, int64_t r0, int64_t r1, int max_rank,
-                              const uint8_t* cells, HitRec* out, uint8_t* tb_out, int* mapped, hipStream_t st);
+                              const uint8_t* cells, HitRec* out, uint8_t* tb_out, int tb_stride, int* mapped,
+                              hipStream_t st);
 hipError_t launch_subset_gather(const uint32_t* planes, const uint32_t* qplanes, const int32_t* len, const Slot* slots,
                                 const int32_t* ids, int64_t k, uint32_t* sp, uint32_t* sq, int32_t* sl, Slot* ss,
                                 hipStream_t st);

@@ -124,3 +124,44 @@ def test_length_order_of_large_views(tmp_path, monkeypatch):
         assert np.array_equal(t, want_t), (mode, chunks)
         assert st["n_mapped"] == int((want_h[:, 0]["ref_id"] != -1).sum())
     m.close()
+
+
+def test_compact_traceback_copy_equals_full_records(tmp_path, monkeypatch):
+    """Traceback records leave the device in the call's compact width (the longest read's codes
+    plus the indel cap) and are widened to STORM_TB_BYTES on the host: the caller's arrays are the
+    same as with whole records copied (STORM_TB_FULL_WIDTH), on both paths, in one launch and in
+    chunks, and every byte past a traceback's codes is zero."""
+    from storm_amd import lib, synth
+    g = synth.gc_window_genome(1_000_000, 23)
+    gpath = str(tmp_path / "g.fa")
+    synth.write_fasta_fast(gpath, "chr1", g)
+    pr = synth.fast_nucleotide_reads(g, 40_000, (40, 90), 29, sub=0.015, indel=0.004, qlo=15, qhi=40)
+    m = lib.Mapper(gpath, color=0, allowed_indels=7, max_hits=3, seeds=SEEDS_N)
+    got = {}
+    for full in (False, True):
+        if full:
+            monkeypatch.setenv("STORM_TB_FULL_WIDTH", "1")
+        else:
+            monkeypatch.delenv("STORM_TB_FULL_WIDTH", raising=False)
+        for mode, chunks in ((lib.STORM_MODE_PLAIN, "1"), (lib.STORM_MODE_BATCHED, "1"), (lib.STORM_MODE_BATCHED, "4")):
+            monkeypatch.setenv("STORM_RECORD_CHUNKS", chunks)
+            # arrays filled with 0xff first, so every byte the call leaves is seen
+            h = np.full((len(pr.len), 4), -1, lib.HIT_RECORD)
+            t = np.full((len(pr.len), 3, lib.TB_BYTES), 0xff, np.uint8)
+            m.map_records(pr, mode=mode, out=(h, t))
+            got[(full, mode, chunks)] = (h, t)
+    m.close()
+    ref_h, ref_t = got[(True, lib.STORM_MODE_PLAIN, "1")]
+    for k, (h, t) in got.items():
+        assert np.array_equal(h, ref_h), k
+        assert np.array_equal(t, ref_t), k
+    n_tb = 0
+    for r in range(len(pr.len)):
+        for k in range(3):
+            if ref_h[r, k]["ref_id"] < 0:
+                assert not ref_t[r, k].any()
+                continue
+            nb = (int(ref_h[r, k]["tb_len"]) + 1) // 2
+            assert not ref_t[r, k, nb:].any(), (r, k)
+            n_tb += 1
+    assert n_tb > 0.9 * len(pr.len)
