@@ -690,11 +690,13 @@ static_assert(sizeof(storm_hit) == sizeof(HitRec) && offsetof(storm_hit, tb_len)
 // cap), 2 codes per byte, so records of the call's longest read fit in (maxL + I + 1) / 2 bytes,
 // rounded up to 16; the caller's records are TBB bytes, the rest zero.  The environment variable
 // STORM_TB_FULL_WIDTH copies whole TBB-byte records instead (for tests).
-static int tb_width(const storm_mapper* m) {
+static int tb_width(const storm_mapper* m, int maxL = -1) {
   if (getenv("STORM_TB_FULL_WIDTH")) return TBB;
-  int maxL = 0;
-  for (int L = MAXL + 1; L > 0 && !maxL; --L)
-    if ((size_t)L < m->len_hist.size() && m->len_hist[(size_t)L]) maxL = L;
+  if (maxL < 0) {   // the call's longest read, from its length histogram
+    maxL = 0;
+    for (int L = MAXL + 1; L > 0 && !maxL; --L)
+      if ((size_t)L < m->len_hist.size() && m->len_hist[(size_t)L]) maxL = L;
+  }
   if (maxL <= 0 || maxL > MAXL) return TBB;
   const int I = std::max(0, std::min(m->p.allowed_indels, MAX_I));
   const int w = ((maxL + I + 1) / 2 + 15) / 16 * 16;
@@ -1119,11 +1121,13 @@ static int batched_stats(storm_mapper* m, int64_t n, int64_t mapped, const unsig
 // of a persistent launch run with the chip partly idle: measured 0.4-1.0 times the tuner's sample
 // launch, one task per workgroup), so K = sqrt(copy / tail) with the tail taken at its upper
 // figure, rounded down; the tuner's figure is known from the first call of a band class on, which
-// itself runs in one launch.  Measured: 5M color reads of 34-74 on 50 Mbp go in 2 chunks (host
-// share 73 -> 51 ms, launches 411 -> 418 ms); 4M x 100 nt at -i 7 -M 4 (a 25 ms tail against
-// 60 ms of copies) and 100k x 50 nt (two launches 0.26 ms longer than one, copies 0.45 ms) stay in
-// one.  The measurement build with phase timers always maps in one launch (one phase line per
-// call).  The environment variable STORM_RECORD_CHUNKS = K >= 1 fixes the count instead.
+// itself runs in one launch.  Measured with whole 144-byte traceback records: 5M color reads of
+// 34-74 on 50 Mbp gained in 2 chunks (host share 73 -> 51 ms, launches 411 -> 418 ms), while 4M
+// x 100 nt at -i 7 -M 4 (a 25 ms tail against 60 ms of copies) and 100k x 50 nt (two launches
+// 0.26 ms longer than one, copies 0.45 ms) did not; with the compact records (tb_width) the copies
+// are small enough that all three run in one launch.  The measurement build with phase timers
+// always maps in one launch (one phase line per call).  The environment variable
+// STORM_RECORD_CHUNKS = K >= 1 fixes the count instead.
 static int record_chunks(storm_mapper* m, const storm_reads_view* v, bool want_tb) {
 #ifdef STORM_FUSED_PHASES
   (void)m;
@@ -1144,7 +1148,7 @@ static int record_chunks(storm_mapper* m, const storm_reads_view* v, bool want_t
   FusedTune t;
   if (!cached_tune(m, tune_key(m, maxL), t) || t.ms <= 0) return 1;
   const double tail_ms = t.ms;
-  const double copy_ms = (double)n * (4.0 * sizeof(HitRec) + (want_tb ? (double)tb_ranks(m->p) * TBB : 0.0)) / 40e6;
+  const double copy_ms = (double)n * (4.0 * sizeof(HitRec) + (want_tb ? (double)tb_ranks(m->p) * tb_width(m, maxL) : 0.0)) / 40e6;
   const int K = (int)std::floor(std::sqrt(copy_ms / tail_ms));
   return (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)K, 8, n / 32768}));
 #endif
