@@ -51,7 +51,7 @@ VALU_INSTR_PER_CELL_W8_L75 = 4.755
 # VALU lane-instructions k_align<7,false> issues per DP cell (L x 15 band columns) on the DP stage
 # workload (one pass per record with directions, the walk of every accepted record), measured the
 # same way (profiles/k_align_I7_L75_2M_records.md)
-VALU_INSTR_PER_DP_CELL_I7_L75 = 34.07
+VALU_INSTR_PER_DP_CELL_I7_L75 = 32.53
 # HBM bytes per candidate of the same dispatch: (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00) x 1024 /
 # candidates, the factors calibrated on a device-to-device copy of known size in the same run
 # (profiles/k_filter_W8_L75_10Mcand.md)

@@ -393,12 +393,15 @@ __device__ __forceinline__ bool dp_bounded_by_filter(const ReadsDev& r, const DP
 // rows instead of one per step) and the step picks its word from registers; the read and genome
 // words the steps compare are kept in registers too and loaded again only when the walk leaves
 // them (every 32 rows or columns), not at every step.
-template <int IM, bool COLOR, int STRIDE>
+// LSTRIDE > 0: the block of rows sits in LDS instead of registers (the lane's words at
+// lrows[LSTRIDE * (q * DW + w)], lanes interleaved), so a step reads its word with one LDS load
+// rather than selecting it among RB x DW registers.
+template <int IM, bool COLOR, int STRIDE, int LSTRIDE = 0>
 __device__ __forceinline__ void tb_walk(const GenomeDev& g, const uint32_t* pl, int L, int win, int NC, int bc,
-                                        const uint32_t* tbw, uint8_t* tb, AlignRec& o) {
+                                        const uint32_t* tbw, uint8_t* tb, AlignRec& o, uint32_t* lrows = nullptr) {
   constexpr int DW = (2 * IM + 1 + 7) / 8;
   constexpr int RB = DW <= 2 ? 8 : DW <= 4 ? 4 : 2;
-  uint32_t rows[RB][DW];
+  uint32_t rows[LSTRIDE > 0 ? 1 : RB][DW];
   int lo = L;   // rows lo .. lo + RB - 1 are in `rows`
   int rw = -1;   // read word rw (codes 32 rw ..) in rl / rh
   uint32_t rl = 0, rh = 0;
@@ -410,17 +413,33 @@ __device__ __forceinline__ void tb_walk(const GenomeDev& g, const uint32_t* pl, 
   while (i >= 0 && c >= 0 && c < NC) {
     if (i < lo) {   // the RB rows up to row i, loaded together (rows below 0 are not read)
       lo = i - RB + 1;
+      if constexpr (LSTRIDE > 0) {
+        uint32_t v[RB][DW];
+#pragma unroll
+        for (int q = 0; q < RB; ++q)
+#pragma unroll
+          for (int w = 0; w < DW; ++w) v[q][w] = lo + q >= 0 ? tbw[STRIDE * ((lo + q) * DW + w)] : 0u;
+#pragma unroll
+        for (int q = 0; q < RB; ++q)
+#pragma unroll
+          for (int w = 0; w < DW; ++w) lrows[LSTRIDE * (q * DW + w)] = v[q][w];
+      } else {
+#pragma unroll
+        for (int q = 0; q < RB; ++q)
+#pragma unroll
+          for (int w = 0; w < DW; ++w) rows[q][w] = lo + q >= 0 ? tbw[STRIDE * ((lo + q) * DW + w)] : 0u;
+      }
+    }
+    uint32_t word = 0;
+    if constexpr (LSTRIDE > 0) {
+      word = lrows[LSTRIDE * ((i - lo) * DW + (c >> 3))];
+    } else {
 #pragma unroll
       for (int q = 0; q < RB; ++q)
 #pragma unroll
-        for (int w = 0; w < DW; ++w) rows[q][w] = lo + q >= 0 ? tbw[STRIDE * ((lo + q) * DW + w)] : 0u;
+        for (int w = 0; w < DW; ++w)
+          if (q == i - lo && w == (c >> 3)) word = rows[q][w];
     }
-    uint32_t word = 0;
-#pragma unroll
-    for (int q = 0; q < RB; ++q)
-#pragma unroll
-      for (int w = 0; w < DW; ++w)
-        if (q == i - lo && w == (c >> 3)) word = rows[q][w];
     o.best_i0 = (int16_t)i;
     o.best_j0 = (int16_t)(i + c);
     const int cell = (int)(word >> ((c & 7) * 4)) & 0xF;
@@ -472,9 +491,9 @@ __device__ __forceinline__ void tb_walk(const GenomeDev& g, const uint32_t* pl, 
 // only rises, run.c:568,683-685), so both stop at the first row where the bound drops below t
 // (S < t then holds) and return score = MIN_SCORE; otherwise they return the same score, S and
 // best_j.
-template <int IM, bool COLOR, bool TB = true>
+template <int IM, bool COLOR, bool TB = true, int LSTRIDE = 0>
 __device__ AlignRec dp_one(const GenomeDev& g, const ReadsDev& r, const ScoreDev& scd, int rs, int h, uint32_t* tbw,
-                           uint8_t* tb, const uint8_t* color_tab) {
+                           uint8_t* tb, const uint8_t* color_tab, uint32_t* walk_lds = nullptr) {
   const DPConst k(scd);
   DPLane<IM, COLOR> st;
   st.template begin<TB>(g, r, k, rs, h, tbw, color_tab);
@@ -486,7 +505,7 @@ __device__ AlignRec dp_one(const GenomeDev& g, const ReadsDev& r, const ScoreDev
   AlignRec o = st.result(k, bc);
   // A record can be accepted only if score > T and S >= T with T >= t (run.c:568,675-685).
   if (TB && bc >= 0 && o.score > k.t && st.S >= k.t)
-    tb_walk<IM, COLOR, 64>(g, st.pl, st.L, st.win, k.NC, bc, tbw, tb, o);
+    tb_walk<IM, COLOR, 64, LSTRIDE>(g, st.pl, st.L, st.win, k.NC, bc, tbw, tb, o, walk_lds);
   return o;
 }
 

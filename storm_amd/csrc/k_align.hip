@@ -44,6 +44,11 @@ __device__ __forceinline__ AlignRec invalid_record() {
 template <int IM, bool COLOR>
 __global__ __launch_bounds__(128) void k_align(AlignArgs a) {
   __shared__ __align__(16) uint8_t ctab[COLOR ? COLOR_TAB : 16];
+  // the traceback walk's block of direction rows, per lane (lanes interleaved): 8 KB at most,
+  // which leaves the kernel's occupancy to its registers
+  constexpr int DW = (2 * IM + 1 + 7) / 8;
+  constexpr int RB = DW <= 2 ? 8 : DW <= 4 ? 4 : 2;
+  __shared__ uint32_t walk_lds[RB * DW * 128];
   if (COLOR) {
     copy_color_table(ctab, threadIdx.x, blockDim.x);
     __syncthreads();
@@ -55,9 +60,9 @@ __global__ __launch_bounds__(128) void k_align(AlignArgs a) {
     a.out[rec] = invalid_record();
     return;
   }
-  a.out[rec] = dp_one<IM, COLOR, true>(a.g, a.r, a.sc, a.rec_rs[rec], a.rec_diag[rec],
-                                       (uint32_t*)(a.scratch + (k >> 6) * 64 * a.scratch_stride) + (k & 63),
-                                       a.tb + rec * TBB, ctab);
+  a.out[rec] = dp_one<IM, COLOR, true, 128>(a.g, a.r, a.sc, a.rec_rs[rec], a.rec_diag[rec],
+                                            (uint32_t*)(a.scratch + (k >> 6) * 64 * a.scratch_stride) + (k & 63),
+                                            a.tb + rec * TBB, ctab, walk_lds + threadIdx.x);
 }
 
 hipError_t launch_align(const AlignArgs& a, hipStream_t st) {
